@@ -1,0 +1,150 @@
+/*
+ * sfx_oracle.h -- CPU oracle for the Shadowfax face-flux path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, single-threaded
+ * restatement of the reference algorithm (Shadowfax src/riemann and the
+ * VorFace/VorCell flux pass).  It is the checker for the HIP product path and
+ * the CPU baseline timed next to it; nothing in the product links or calls it.
+ *
+ * Build rules that make it bit-reproducible against the reference compiled
+ * with g++ -O2 on x86-64: ISO C, -ffp-contract=off, no -march (no FMA), glibc
+ * libm pow/sqrt.
+ *
+ * Data conventions (shared with the product's C-ABI):
+ *   ndim in {2,3}; a state has ndim+2 components
+ *     W = [rho, vx, vy, (vz), p]   primitives
+ *     Q = [m, px, py, (pz), e]     conserved
+ *   batch arrays are SoA, component-major: X[k*N + i].
+ *
+ * Pinning status of each function is stated in the header comment of
+ * sfx_oracle.c.
+ */
+#ifndef SFX_ORACLE_H
+#define SFX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFXO_EXACT 0
+#define SFXO_HLLC 1
+
+/* per-problem / per-face status bits (replace the reference's exceptions) */
+#define SFXO_ST_SOLVER 1u    /* RiemannSolverException: NaN/negative solution */
+#define SFXO_ST_NANFLUX 2u   /* FluxCalculationException: NaN mass flux      */
+#define SFXO_ST_ITERCAP 4u   /* iteration cap reached (reference: unbounded)  */
+#define SFXO_ST_UPDATEQ 8u   /* UpdateQException: negative m or e after update */
+
+typedef struct {
+    int kind;
+    int ndim;
+    double gamma;
+    /* derived constants, ExactRiemannSolver.cpp:52-61 */
+    double gp1d2g, gm1d2g, gm1dgp1, tdgp1, tdgm1, gm1d2, tgdgm1, ginv;
+    double tol, cutoff;
+} sfxo_solver;
+
+typedef struct {
+    uint64_t neval;   /* _counterTotal / _neval */
+    uint64_t nbrent;  /* _counterBrent */
+    uint64_t nvacuum; /* _vacuum */
+    uint64_t niter;   /* Newton + Brent iterations (not in the reference) */
+} sfxo_counters;
+
+void sfxo_solver_init(sfxo_solver* s, int kind, int ndim, double gamma,
+                      double tol, double cutoff);
+
+/* ExactRiemannSolver::solve (x/t = 0 half state).  WR is overwritten when
+ * reflective, as in the reference.  Returns status bits. */
+unsigned sfxo_exact_solve(const sfxo_solver* s, double* WL, double* WR,
+                          const double* n, double* mach, int reflective,
+                          double* Whalf, sfxo_counters* c);
+
+/* RiemannSolver::solve_for_flux for the solver kind in s. v may be NULL (0).*/
+unsigned sfxo_solve_for_flux(const sfxo_solver* s, double* WL, double* WR,
+                             const double* n, const double* v, int reflective,
+                             double* F, sfxo_counters* c);
+
+/* batched solve_for_flux over SoA arrays; v, refl may be NULL; status may be
+ * NULL.  Returns OR of all status bits. */
+unsigned sfxo_solve_for_flux_batch(const sfxo_solver* s, int64_t N,
+                                   const double* WL, const double* WR,
+                                   const double* n, const double* v,
+                                   const uint8_t* refl, double* F,
+                                   uint8_t* status, sfxo_counters* c);
+
+/* batched ExactRiemannSolver::solve (half states) */
+unsigned sfxo_exact_solve_batch(const sfxo_solver* s, int64_t N,
+                                const double* WL, const double* WR,
+                                const double* n, const uint8_t* refl,
+                                double* Whalf, uint8_t* status,
+                                sfxo_counters* c);
+
+/* get_time_derivative; grad is [ndim][ndim+2] (grad[d*(ndim+2)+k]) */
+void sfxo_time_derivative(const sfxo_solver* s, const double* W,
+                          const double* grad, double* dWdt);
+
+/* VorFace::limit (scalar) and VorFace::reconstruct */
+double sfxo_limit(double phimid0, double phiL, double phiR, const double* d,
+                  double r, int ndim);
+void sfxo_reconstruct(int ndim, const double* W, const double* grad,
+                      const double* d, double* out);
+
+/* get_W / get_Q (hydro components; paq not carried).  Returns status. */
+unsigned sfxo_get_W(const sfxo_solver* s, double volume, double* Q,
+                    double* W);
+void sfxo_get_Q(const sfxo_solver* s, double volume, const double* W,
+                double* Q);
+
+/*
+ * Mesh in SoA form.  C cells, F faces.
+ *  cell arrays (component-major, length C per component):
+ *    W[(ndim+2)*C], grad[ndim*(ndim+2)*C] laid out grad[(d*(ndim+2)+k)*C+i],
+ *    centroid[ndim*C], pos[ndim*C] (generator), vel[ndim*C] (generator
+ *    velocity), mass[C] (Q.m), total_area[C], dt[C] (real time of the cell's
+ *    timestep), volume[C], active[C] (starttime == current integer time)
+ *  face arrays:
+ *    left[F], right[F] (-1: reflective mirror), area[F], midpoint[ndim*F],
+ *    shift[ndim*F]: right generator position = base + shift, where base is
+ *      pos[right] (periodic copy when shift != 0, real neighbour when 0) or,
+ *      for a mirror face, pos[left];
+ *    ghost[F]: 1 if the right generator is a periodic copy (reference id 1)
+ *  per-cell face list in the cell's own walk order (CSR):
+ *    cf_off[C+1], cf_face[cf_off[C]]
+ */
+typedef struct {
+    int ndim;
+    int64_t C, F;
+    const double *W, *grad, *centroid, *pos, *vel, *mass, *total_area, *dt,
+            *volume;
+    const uint8_t* active;
+    const int32_t *left, *right;
+    const double *area, *midpoint, *shift;
+    const uint8_t* ghost;
+    const int64_t* cf_off;
+    const int32_t* cf_face;
+} sfxo_mesh;
+
+/* VorCell::estimate_gradient for every active cell; grad_out has the mesh
+ * grad layout.  Inactive cells get zero gradients. */
+void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m, double* grad_out);
+
+/* VorFace::set_v + VorFace::calculate_flux over faces 0..F-1 in order,
+ * accumulating into dQ[(ndim+2)*C] (component-major; caller zeroes it).
+ * Optional per-face outputs (may be NULL): dQL_face[(ndim+2)*F],
+ * face_status[F].  Returns OR of status bits. */
+unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
+                             double* dQ, double* dQL_face, uint8_t* face_status,
+                             sfxo_counters* c);
+
+/* GasParticle::update_Q then get_W for every cell: Q -= dQ, dQ = 0, W = get_W.
+ * Returns OR of status bits; cell_status may be NULL. */
+unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
+                     const double* volume, double* W, uint8_t* cell_status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,32 @@
+// sfx_capi_internal.h -- the solver handle behind the opaque sfx_solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+
+#include "sfx_device.h"
+
+constexpr int kStageDoubles = 24;
+
+struct sfx_solver {
+    int kind = 0;
+    int ndim = 3;
+    int precision = 0;
+    int device = 0;
+    sfx::Consts c{};
+    bool stats_on = true;
+    unsigned long long* d_ctr = nullptr;  // CNT_N device counters
+    double* d_stage = nullptr;            // scalar entry points: device staging
+    double* h_stage = nullptr;            // pinned host staging
+    // face-flux pass work buffers, grown on demand
+    void* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace sfx {
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* where);
+int bind_device(const sfx_solver* h);
+void release_scratch(sfx_solver* h);
+}  // namespace sfx

@@ -1,0 +1,100 @@
+"""Write golden vectors for the Riemann solvers from the reference Shadowfax
+classes compiled from source (oracle/_ref, built by `make -C oracle ref`).
+
+    python tests/golden/make_golden.py
+
+Output: tests/golden/riemann_{2,3}d.npz, each holding, per problem family
+(uniform, mixed, vacuum, shock) and variant (plain: v = 0 no reflective;
+moving: random face velocity and 25% reflective faces):
+  inputs  WL, WR, n, v, refl        (float64, float32-representable)
+  outputs F_exact, st_exact, F_hllc, st_hllc, Whalf, mach, st_solve
+plus time-derivative and get_W/get_Q vectors.  gamma = 5/3, tol = 1e-8,
+cutoff = -5 (the configuration the benchmarks use) and a gamma = 1.4 set.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from shadowfax_amd import workloads as WK  # noqa: E402
+
+N = 768
+FAMILIES = ("uniform", "mixed", "vacuum", "shock")
+
+
+def riemann_set(ndim, gamma):
+    out = {}
+    rng = np.random.default_rng(1234 + ndim)
+    for fi, fam in enumerate(FAMILIES):
+        b = WK.riemann_batch(N, seed=100 + fi, family=fam, ndim=ndim)
+        for variant in ("plain", "moving"):
+            key = "%s_%s_" % (fam, variant)
+            WL, WR, n = b["WL"], b["WR"], b["n"]
+            if variant == "plain":
+                v = np.zeros((ndim, N))
+                refl = np.zeros(N, np.uint8)
+            else:
+                v = WK.f32(rng.uniform(-1.0, 1.0, (ndim, N)))
+                refl = (rng.random(N) < 0.25).astype(np.uint8)
+            out[key + "WL"] = WL
+            out[key + "WR"] = WR
+            out[key + "n"] = n
+            out[key + "v"] = v
+            out[key + "refl"] = refl
+            for kind, name in ((O.EXACT, "exact"), (O.HLLC, "hllc")):
+                F, st, _ = O.ref_solve_for_flux(kind, ndim, WL, WR, n, v=v, refl=refl,
+                                                gamma=gamma)
+                out[key + "F_" + name] = F
+                out[key + "st_" + name] = st
+            Wh, st, mach = O.ref_exact_solve(ndim, WL, WR, n, refl=refl, gamma=gamma)
+            out[key + "Whalf"] = Wh
+            out[key + "mach"] = mach
+            out[key + "st_solve"] = st
+    # predictor: random states and gradients
+    b = WK.riemann_batch(256, seed=77, family="uniform", ndim=ndim)
+    g = rng.normal(size=(ndim, ndim + 2, 256))
+    out["dwdt_W"] = b["WL"]
+    out["dwdt_grad"] = g
+    for kind, name in ((O.EXACT, "exact"), (O.HLLC, "hllc")):
+        out["dwdt_" + name] = O.ref_time_derivative(kind, ndim, b["WL"], g, gamma=gamma)
+    # get_Q / get_W with floors and vacuum edge cases
+    vol = rng.uniform(0.01, 2.0, 256)
+    out["q_vol"] = vol
+    out["q_W"] = b["WR"]
+    for kind, name in ((O.EXACT, "exact"), (O.HLLC, "hllc")):
+        Q, _ = O.ref_get_Q(kind, ndim, vol, b["WR"], gamma=gamma)
+        out["q_Q_" + name] = Q
+        Qp = Q.copy()
+        Qp[-1] *= rng.uniform(0.0, 1.2, 256)
+        Qp[0, :16] = 0.0
+        Qp[0, 16:32] = 1e-40
+        W, Qo = O.ref_get_W(kind, ndim, vol, Qp, gamma=gamma)
+        out["w_Qin_" + name] = Qp
+        out["w_W_" + name] = W
+        out["w_Qout_" + name] = Qo
+    return out
+
+
+def main():
+    if not (O.ref_available(2) and O.ref_available(3)):
+        raise SystemExit("oracle/_ref not built: make -C oracle ref")
+    for ndim in (2, 3):
+        data = riemann_set(ndim, 5.0 / 3.0)
+        g14 = riemann_set(ndim, 1.4)
+        for k in list(g14):
+            if k.startswith(("uniform_plain", "mixed_moving")):
+                data["g14_" + k] = g14[k]
+        data["gamma"] = np.array(5.0 / 3.0)
+        path = os.path.join(HERE, "riemann_%dd.npz" % ndim)
+        np.savez_compressed(path, **data)
+        print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""GPU parity of the batched Riemann solvers (through the C ABI).
+
+  * FP64 plain path vs the golden vectors of the compiled reference and vs
+    the oracle: normwise relative error <= 1e-12 (the device pow() rounds
+    differently from glibc, so bitwise equality is reported, not required);
+    status bits must agree exactly.
+  * mixed path (FP32 storage and bulk arithmetic, FP64 final Newton step):
+    normwise relative error <= 1e-6 against the FP64 results.
+  * full-size (4,194,304 problems, the benchmark configuration) properties:
+    mixed vs FP64 error, uniform state gives the Euler flux, solver counters.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from shadowfax_amd import workloads as WK
+from shadowfax_amd.metrics import normwise_error, summarize
+from shadowfax_amd.riemann import RiemannSolver, ExactRiemannSolver, HLLCRiemannSolver
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-6
+G = 5.0 / 3.0
+
+
+def _t(a, dtype, dev):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+def gpu_flux(kind, ndim, WL, WR, n, v, refl, dev, precision="fp64", f32=False, gamma=G):
+    import torch
+    s = RiemannSolver(kind, ndim, gamma, 1e-8, -5.0, precision)
+    dt = torch.float32 if f32 else torch.float64
+    N = WL.shape[1]
+    st = torch.zeros(N, dtype=torch.uint8, device=dev)
+    F = s.solve_for_flux_batch(_t(WL, dt, dev), _t(WR, dt, dev), _t(n, dt, dev),
+                               None if v is None else _t(v, dt, dev),
+                               None if refl is None else _t(refl, torch.uint8, dev),
+                               status=st)
+    torch.cuda.synchronize()
+    return F.double().cpu().numpy(), st.cpu().numpy(), s
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
+@pytest.mark.parametrize("variant", ["plain", "moving"])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fp64_matches_reference_golden(device, ndim, fam, variant, kind):
+    print("case", ndim, fam, variant, kind, flush=True)
+    g = np.load(os.path.join(HERE, "golden", "riemann_%dd.npz" % ndim))
+    key = "%s_%s_" % (fam, variant)
+    name = "exact" if kind == 0 else "hllc"
+    F, st, _ = gpu_flux(kind, ndim, g[key + "WL"], g[key + "WR"], g[key + "n"], g[key + "v"],
+                        g[key + "refl"], device)
+    ref_st = g[key + "st_" + name]
+    assert np.array_equal(st != 0, ref_st != 0)
+    ok = ref_st == 0
+    err = normwise_error(F[:, ok], g[key + "F_" + name][:, ok], g[key + "WL"][:, ok],
+                         g[key + "WR"][:, ok], G)
+    s = summarize(err)
+    bitwise = np.mean(np.all(F[:, ok] == g[key + "F_" + name][:, ok], axis=0))
+    print("fp64 %dd %s %s %s: err %s bitwise %.4f" % (ndim, fam, variant, name, s, bitwise))
+    assert s["max"] <= FP64_TOL
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("f32", [True, False])
+def test_mixed_within_1e6_of_fp64(device, ndim, fam, kind, f32):
+    print("case", ndim, fam, kind, f32, flush=True)
+    g = np.load(os.path.join(HERE, "golden", "riemann_%dd.npz" % ndim))
+    key = "%s_moving_" % fam
+    name = "exact" if kind == 0 else "hllc"
+    F, st, _ = gpu_flux(kind, ndim, g[key + "WL"], g[key + "WR"], g[key + "n"], g[key + "v"],
+                        g[key + "refl"], device, precision="mixed", f32=f32)
+    ref_st = g[key + "st_" + name]
+    ok = (ref_st == 0) & (st == 0)
+    assert ok.mean() > 0.99
+    err = normwise_error(F[:, ok], g[key + "F_" + name][:, ok], g[key + "WL"][:, ok],
+                         g[key + "WR"][:, ok], G)
+    s = summarize(err)
+    print("mixed %dd %s %s f32=%s: err %s" % (ndim, fam, name, f32, s))
+    assert s["max"] <= MIXED_TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gamma_14(device, kind):
+    g = np.load(os.path.join(HERE, "golden", "riemann_3d.npz"))
+    key = "g14_mixed_moving_"
+    name = "exact" if kind == 0 else "hllc"
+    for prec in ("fp64", "mixed"):
+        F, st, _ = gpu_flux(kind, 3, g[key + "WL"], g[key + "WR"], g[key + "n"], g[key + "v"],
+                            g[key + "refl"], device, precision=prec, gamma=1.4)
+        ok = (g[key + "st_" + name] == 0) & (st == 0)
+        err = summarize(normwise_error(F[:, ok], g[key + "F_" + name][:, ok],
+                                       g[key + "WL"][:, ok], g[key + "WR"][:, ok], 1.4))
+        print("gamma 1.4", name, prec, err)
+        assert err["max"] <= (FP64_TOL if prec == "fp64" else MIXED_TOL)
+
+
+def test_fp64_vs_oracle_and_counters(device):
+    """Counters: every problem counted; Brent/vacuum counts as in the oracle."""
+    b = WK.riemann_batch(20000, seed=3, family="mixed", ndim=3)
+    F, st, s = gpu_flux(0, 3, b["WL"], b["WR"], b["n"], None, None, device)
+    Fo, sto, c = O.solve_for_flux(O.EXACT, 3, b["WL"], b["WR"], b["n"])
+    ok = sto == 0
+    err = summarize(normwise_error(F[:, ok], Fo[:, ok], b["WL"][:, ok], b["WR"][:, ok], G))
+    assert err["max"] <= FP64_TOL
+    stats = s.stats()
+    assert stats["neval"] == 20000
+    assert stats["nvacuum"] == c.nvacuum
+    assert abs(stats["nbrent"] - c.nbrent) <= 2  # a pow() ulp can flip a Brent decision
+
+
+def test_scalar_entry_points(device):
+    """solve / solve_for_flux drop-ins (host pointers) and the reflective
+    write-back of WR."""
+    s = ExactRiemannSolver(5.0 / 3.0, ndim=3)
+    WL = np.array([1., 0., 0., 0., 1.])
+    WR = np.array([0.125, 0., 0., 0., 0.1])
+    W, mach = s.solve(WL, WR, [1., 0., 0.])
+    assert abs(W[0] - 0.47969) < 1e-4 and abs(W[1] - 0.841194) < 1e-4
+    assert abs(W[4] - 0.293945) < 1e-4
+    F = s.solve_for_flux(WL, WR, [1., 0., 0.], [0.1, 0., 0.])
+    Fo, _, _ = O.solve_for_flux(O.EXACT, 3, WL[:, None], WR[:, None], np.array([[1.], [0.], [0.]]),
+                                v=np.array([[0.1], [0.], [0.]]))
+    assert np.allclose(F, Fo[:, 0], rtol=1e-13, atol=1e-15)
+    WR2 = WR.copy()
+    s.solve_for_flux(WL, WR2, [1., 0., 0.], None, reflective=True)
+    assert np.array_equal(WR2, WL)
+    h = HLLCRiemannSolver(5.0 / 3.0, ndim=2)
+    F2 = h.solve_for_flux([1., 0., 0., 1.], [0.125, 0., 0., 0.1], [1., 0.])
+    Fo2, _, _ = O.solve_for_flux(O.HLLC, 2, np.array([[1.], [0.], [0.], [1.]]),
+                                 np.array([[0.125], [0.], [0.], [0.1]]), np.array([[1.], [0.]]))
+    assert np.allclose(F2, Fo2[:, 0], rtol=1e-13, atol=1e-15)
+    assert s.tag() == "EXAC" and h.tag() == "HLLC"
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_full_size_config1(device, kind):
+    """4,194,304 problems of the benchmark configuration: mixed (FP32
+    storage) vs FP64 on the GPU, plus a 20,000-problem oracle spot check of
+    the FP64 path."""
+    import torch
+    cfg = WK.CONFIG1
+    b = WK.riemann_batch(cfg["N"], seed=cfg["seed"], family=cfg["family"], ndim=3)
+    print("generated", flush=True)
+    F64, st64, _ = gpu_flux(kind, 3, b["WL"], b["WR"], b["n"], None, None, device)
+    F32, st32, _ = gpu_flux(kind, 3, b["WL"], b["WR"], b["n"], None, None, device,
+                            precision="mixed", f32=True)
+    ok = (st64 == 0) & (st32 == 0)
+    assert ok.mean() > 0.999
+    err = summarize(normwise_error(F32[:, ok], F64[:, ok], b["WL"][:, ok], b["WR"][:, ok], G))
+    print("config1 kind", kind, "mixed vs fp64", err)
+    assert err["max"] <= MIXED_TOL
+    idx = np.arange(0, cfg["N"], cfg["N"] // 20000)[:20000]
+    sub = {k: b[k][:, idx] for k in ("WL", "WR", "n")}
+    Fo, sto, _ = O.solve_for_flux(kind, 3, sub["WL"], sub["WR"], sub["n"])
+    okk = (sto == 0) & ok[idx]
+    e2 = summarize(normwise_error(F64[:, idx][:, okk], Fo[:, okk], sub["WL"][:, okk],
+                                  sub["WR"][:, okk], G))
+    assert e2["max"] <= FP64_TOL
+    del F64, F32
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_uniform_state_gives_euler_flux_gpu(device, kind):
+    b = WK.riemann_batch(100000, seed=11, ndim=3)
+    WL = b["WL"]
+    n = b["n"]
+    for prec, f32, tol in (("fp64", False, 1e-13), ("mixed", True, 1e-6)):
+        F, st, _ = gpu_flux(kind, 3, WL, WL, n, None, None, device, precision=prec, f32=f32)
+        rho, p, vel = WL[0], WL[4], WL[1:4]
+        un = (vel * n).sum(0)
+        E = p / (G - 1) + 0.5 * rho * (vel * vel).sum(0)
+        R = np.empty_like(WL)
+        R[0] = rho * un
+        R[1:4] = rho * vel * un + p * n
+        R[4] = (E + p) * un
+        err = summarize(normwise_error(F, R, WL, WL, G))
+        print(kind, prec, err)
+        assert err["max"] <= tol
