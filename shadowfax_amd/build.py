@@ -5,7 +5,9 @@
 Per-file flags:
   * FP64 parity kernels: -ffp-contract=off, so expressions keep the
     association of the reference C++ (no FMA contraction);
-  * mixed kernels: contraction allowed, fast FP32 divide/sqrt;
+  * mixed kernels: contraction within expressions (-ffp-contract=on, which
+    honours the per-function contract(off) of the shared FP64 code), fast
+    FP32 divide/sqrt;
   * all device code: packed FP32 instructions disabled
     (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -28,10 +30,10 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
 
 SOURCES = [
     ("riemann_fp64.hip", ["-ffp-contract=off"]),
-    ("riemann_mixed.hip", ["-ffp-contract=fast",
+    ("riemann_mixed.hip", ["-ffp-contract=on",
                            "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("flux_pass.hip", ["-ffp-contract=off"]),
-    ("flux_pass_mixed.hip", ["-ffp-contract=fast",
+    ("flux_pass_mixed.hip", ["-ffp-contract=on",
                              "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sfx_capi.cpp", []),
     ("sfx_mesh_capi.cpp", []),

@@ -9,614 +9,12 @@
 // problem with v = 0).  The exact solver's Newton/Brent loops run per lane
 // with an iteration cap; divergence handling is left to the hardware here
 // (this is the parity path; the mixed path is the fast one).
+#include "riemann_fp64_impl.h"
 #include "sfx_device.h"
 #include "sfx_launch.h"
 
 namespace sfx {
 namespace fp64 {
-
-template <int ND>
-struct S {
-    static constexpr int NC = ND + 2;
-    static constexpr int P = ND + 1;
-};
-
-// ---------------------------------------------------------------- exact --
-
-template <int ND>
-__device__ __forceinline__ double ex_soundspeed(const Consts& c, const double* W) {
-    // ExactRiemannSolver.cpp:681-687
-    if(W[0] != 0.) return sqrt(c.gamma * W[S<ND>::P] / W[0]);
-    return 0.;
-}
-
-template <int ND>
-__device__ __forceinline__ double ex_fb(const Consts& c, double p, const double* W, double a) {
-    // ExactRiemannSolver.cpp:556-566
-    const int P = S<ND>::P;
-    if(p > W[P]) {
-        double A = c.tdgp1 / W[0];
-        double B = c.gm1dgp1 * W[P];
-        return (p - W[P]) * sqrt(A / (p + B));
-    }
-    return c.tdgm1 * a * (pow(p / W[P], c.gm1d2g) - 1.);
-}
-
-template <int ND>
-__device__ __forceinline__ double ex_gb(const Consts& c, double p, const double* W) {
-    // ExactRiemannSolver.cpp:575-579
-    double A = c.tdgp1 / W[0];
-    double B = c.gm1dgp1 * W[S<ND>::P];
-    return sqrt(A / (p + B));
-}
-
-template <int ND>
-__device__ __forceinline__ double ex_fprimeb(const Consts& c, double p, const double* W, double a) {
-    // ExactRiemannSolver.cpp:589-599
-    const int P = S<ND>::P;
-    if(p > W[P]) {
-        double A = c.tdgp1 / W[0];
-        double B = c.gm1dgp1 * W[P];
-        return (1. - 0.5 * (p - W[P]) / (B + p)) * sqrt(A / (p + B));
-    }
-    return 1. / (W[0] * a) * pow(p / W[P], -c.gp1d2g);
-}
-
-template <int ND>
-__device__ __forceinline__ double ex_f(const Consts& c, double p, const double* WL,
-                                       const double* WR, double vL, double vR,
-                                       double aL, double aR) {
-    return ex_fb<ND>(c, p, WL, aL) + ex_fb<ND>(c, p, WR, aR) + (vR - vL);
-}
-
-template <int ND>
-__device__ __forceinline__ double ex_fprime(const Consts& c, double p, const double* WL,
-                                            const double* WR, double aL, double aR) {
-    return ex_fprimeb<ND>(c, p, WL, aL) + ex_fprimeb<ND>(c, p, WR, aR);
-}
-
-template <int ND>
-__device__ double ex_guess_p(const Consts& c, const double* WL, const double* WR,
-                             double vL, double vR, double aL, double aR) {
-    // ExactRiemannSolver.cpp:644-673
-    const int P = S<ND>::P;
-    double pguess;
-    double pmin = smin(WL[P], WR[P]);
-    double pmax = smax(WL[P], WR[P]);
-    double qmax = pmax / pmin;
-    double ppv = 0.5 * (WL[P] + WR[P]) - 0.125 * (vR - vL) * (WL[0] + WR[0]) * (aL + aR);
-    ppv = smax(c.tol * 0.5 * (WL[P] + WR[P]), ppv);
-    if(qmax <= 2. && pmin <= ppv && ppv <= pmax) {
-        pguess = ppv;
-    } else if(ppv < pmin) {
-        pguess = pow((aL + aR - c.gm1d2 * (vR - vL)) /
-                             (aL / pow(WL[P], c.gm1d2g) + aR / pow(WR[P], c.gm1d2g)),
-                     c.tgdgm1);
-    } else {
-        double gL = ex_gb<ND>(c, ppv, WL), gR = ex_gb<ND>(c, ppv, WR);
-        pguess = (gL * WL[P] + gR * WR[P] - vR + vL) / (gL + gR);
-    }
-    return smax(c.tol * 0.5 * (WL[P] + WR[P]), pguess);
-}
-
-// BrentsMethodSolve, ExactRiemannSolver.cpp:462-546
-template <int ND>
-__device__ double ex_brent(const Consts& c, double lo, double hi, const double* WL,
-                           const double* WR, double vL, double vR, double aL, double aR,
-                           unsigned& iters, uint8_t& st) {
-    const double tol = c.tol;
-    double a = lo, b = hi, cc = 0., d = 1e230;
-    double fa = ex_f<ND>(c, a, WL, WR, vL, vR, aL, aR);
-    double fb = ex_f<ND>(c, b, WL, WR, vL, vR, aL, aR);
-    double fc = 0., s = 0., fs = 0.;
-    if(fa * fb >= 0) return (fa < fb) ? a : b;
-    if(fabs(fa) < fabs(fb)) {
-        double t = a; a = b; b = t;
-        t = fa; fa = fb; fb = t;
-    }
-    cc = a;
-    fc = fa;
-    bool mflag = true;
-    int it = 0;
-    while(!(fb == 0) && (fabs(a - b) > tol * (a + b) * 0.5)) {
-        if(++it > c.itercap) {
-            st |= ST_ITERCAP;
-            break;
-        }
-        if((fa != fc) && (fb != fc))
-            s = a * fb * fc / (fa - fb) / (fa - fc) + b * fa * fc / (fb - fa) / (fb - fc) +
-                cc * fa * fb / (fc - fa) / (fc - fb);
-        else
-            s = b - fb * (b - a) / (fb - fa);
-        double tmp2 = (3. * a + b) / 4.;
-        if(!(((s > tmp2) && (s < b)) || ((s < tmp2) && (s > b))) ||
-           (mflag && (fabs(s - b) >= (fabs(b - cc) / 2.))) ||
-           (!mflag && (fabs(s - b) >= (fabs(cc - d) / 2.))) ||
-           (mflag && (fabs(b - cc) < tol * (b + cc) * 0.5)) ||
-           (!mflag && (fabs(cc - d) < tol * (cc + d) * 0.5))) {
-            s = (a + b) / 2.;
-            mflag = true;
-        } else {
-            mflag = false;
-        }
-        fs = ex_f<ND>(c, s, WL, WR, vL, vR, aL, aR);
-        d = cc;
-        cc = b;
-        fc = fb;
-        if(fa * fs < 0.) {
-            b = s;
-            fb = fs;
-        } else {
-            a = s;
-            fa = fs;
-        }
-        if(fabs(fa) < fabs(fb)) {
-            double t = a; a = b; b = t;
-            t = fa; fa = fb; fb = t;
-        }
-    }
-    iters += it;
-    return b;
-}
-
-template <int ND>
-__device__ __forceinline__ uint8_t ex_check(const double* sol) {
-    const int P = S<ND>::P;
-    bool bad = (sol[0] < 0.) | (sol[0] != sol[0]) | (sol[1] != sol[1]) | (sol[2] != sol[2]);
-    if(ND == 3) bad |= (sol[3] != sol[3]);
-    bad |= (sol[P] < 0.) | (sol[P] != sol[P]);
-    return bad ? ST_SOLVER : 0;
-}
-
-// solve_vacuum, ExactRiemannSolver.cpp:304-446
-template <int ND>
-__device__ uint8_t ex_solve_vacuum(const Consts& c, const double* WL, const double* WR,
-                                   double vL, double vR, double aL, double aR,
-                                   const double* n, double* sol) {
-    const int NC = S<ND>::NC, P = S<ND>::P;
-    const double g = c.gamma;
-#pragma unroll
-    for(int k = 0; k < NC; k++) sol[k] = 0.;
-    if(!(WR[0] != 0.) && !(WL[0] != 0.)) return 0;
-    double vhalf;
-    if(!(WR[0] != 0.)) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) sol[k] = WL[k];
-        if(vL < aL) {
-            double SL = vL + 2. * aL / (g - 1.);
-            if(SL > 0.) {
-                double base = 2. / (g + 1.) + (g - 1.) / (g + 1.) / aL * vL;
-                sol[0] = WL[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (aL + 0.5 * (g - 1.) * vL) - vL;
-                sol[P] = WL[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-#pragma unroll
-                for(int k = 0; k < NC; k++) sol[k] = 0.;
-                vhalf = 0.;
-            }
-        } else {
-            vhalf = 0.;
-        }
-    } else if(!(WL[0] != 0.)) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) sol[k] = WR[k];
-        if(-aR < vR) {
-            double SR = vR - 2. * aR / (g - 1.);
-            if(SR >= 0.) {
-#pragma unroll
-                for(int k = 0; k < NC; k++) sol[k] = 0.;
-                vhalf = 0.;
-            } else {
-                double base = 2. / (g + 1.) - (g - 1.) / (g + 1.) / aR * vR;
-                sol[0] = WR[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (-aR + 0.5 * (g - 1.) * vR) - vR;
-                sol[P] = WR[P] * pow(base, 2. * g / (g - 1.));
-            }
-        } else {
-            vhalf = 0.;
-        }
-    } else {
-        double SR = vR - 2. * aR / (g - 1.);
-        double SL = vL + 2. * aL / (g - 1.);
-        if(SR > 0. && SL < 0.) {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = 0.;
-            vhalf = 0.;
-        } else if(SL >= 0.) {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WL[k];
-            if(aL > vL) {
-                double base = 2. / (g + 1.) + (g - 1.) / (g + 1.) / aL * vL;
-                sol[0] = WL[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (aL + 0.5 * (g - 1.) * vL) - vL;
-                sol[P] = WL[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-                vhalf = 0.;
-            }
-        } else {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WR[k];
-            if(-aR < vR) {
-                double base = 2. / (g + 1.) - (g - 1.) / (g + 1.) / aR * vR;
-                sol[0] = WR[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (-aR + 0.5 * (g - 1.) * vR) - vR;
-                sol[P] = WR[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-                vhalf = 0.;
-            }
-        }
-    }
-#pragma unroll
-    for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n[d];
-    return ex_check<ND>(sol);
-}
-
-// ExactRiemannSolver::solve, ExactRiemannSolver.cpp:108-290.
-// Counters: iters (Newton+Brent steps), brent (0/1), vac (0/1).
-template <int ND>
-__device__ uint8_t ex_solve(const Consts& c, const double* WL, double* WR, const double* n,
-                            bool reflective, double* sol, double* mach,
-                            unsigned& iters, unsigned& brent, unsigned& vac) {
-    const int NC = S<ND>::NC, P = S<ND>::P;
-    uint8_t st = 0;
-    double vL = WL[1] * n[0] + WL[2] * n[1];
-    double vR = WR[1] * n[0] + WR[2] * n[1];
-    if(ND == 3) {
-        vL = WL[1] * n[0] + WL[2] * n[1] + WL[3] * n[ND - 1];
-        vR = WR[1] * n[0] + WR[2] * n[1] + WR[3] * n[ND - 1];
-    }
-    if(reflective) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) WR[k] = WL[k];
-        vR = -vL;
-    }
-    double aL = ex_soundspeed<ND>(c, WL);
-    double aR = ex_soundspeed<ND>(c, WR);
-    if(!(WL[0] != 0.) || !(WR[0] != 0.)) {
-        vac = 1;
-        return ex_solve_vacuum<ND>(c, WL, WR, vL, vR, aL, aR, n, sol);
-    }
-    if(2. * aL / (c.gamma - 1.) + 2. * aR / (c.gamma - 1.) <= vR - vL) {
-        vac = 1;
-        st |= ex_solve_vacuum<ND>(c, WL, WR, vL, vR, aL, aR, n, sol);
-    } else {
-        const double tol = c.tol;
-        double p = 0.;
-        double pguess = ex_guess_p<ND>(c, WL, WR, vL, vR, aL, aR);
-        double fp = ex_f<ND>(c, p, WL, WR, vL, vR, aL, aR);
-        double fpguess = ex_f<ND>(c, pguess, WL, WR, vL, vR, aL, aR);
-        int it = 0;
-        if(fp < c.cutoff) {
-            while(fabs(p - pguess) > tol * 0.5 * (p + pguess)) {
-                if(++it > c.itercap) {
-                    st |= ST_ITERCAP;
-                    break;
-                }
-                p = pguess;
-                pguess = pguess - fpguess / ex_fprime<ND>(c, pguess, WL, WR, aL, aR);
-                if(pguess < 0.) pguess = 0.;
-                fpguess = ex_f<ND>(c, pguess, WL, WR, vL, vR, aL, aR);
-            }
-            p = pguess;
-        } else {
-            if(fp * fpguess >= 0.) {
-                while(fabs(p - pguess) > tol * 0.5 * (p + pguess) && fpguess < 0.) {
-                    if(++it > c.itercap) {
-                        st |= ST_ITERCAP;
-                        break;
-                    }
-                    p = pguess;
-                    pguess = pguess - fpguess / ex_fprime<ND>(c, pguess, WL, WR, aL, aR);
-                    fpguess = ex_f<ND>(c, pguess, WL, WR, vL, vR, aL, aR);
-                }
-            }
-            if(fabs(p - pguess) > tol * 0.5 * (p + pguess) && fpguess > 0.) {
-                brent = 1;
-                p = ex_brent<ND>(c, 0., pguess, WL, WR, vL, vR, aL, aR, iters, st);
-            } else {
-                p = pguess;
-            }
-        }
-        iters += it;
-
-        double u = 0.5 * (vL + vR) + 0.5 * (ex_fb<ND>(c, p, WR, aR) - ex_fb<ND>(c, p, WL, aL));
-        if(mach) {
-            if(p > WL[P]) *mach = smax(*mach, sqrt(c.gp1d2g * p / WL[P] + c.gm1d2g));
-            if(p > WR[P]) *mach = smax(*mach, sqrt(c.gp1d2g * p / WR[P] + c.gm1d2g));
-        }
-        double vhalf;
-        if(u < 0) {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WR[k];
-            double pdpR = p / WR[P];
-            if(p > WR[P]) {
-                double SR = vR + aR * sqrt(c.gp1d2g * pdpR + c.gm1d2g);
-                if(SR > 0) {
-                    sol[0] = WR[0] * (pdpR + c.gm1dgp1) / (c.gm1dgp1 * pdpR + 1.);
-                    sol[P] = p;
-                    vhalf = u - vR;
-                } else {
-                    vhalf = 0.;
-                }
-            } else {
-                double SHR = vR + aR;
-                if(SHR > 0) {
-                    double STR = u + aR * pow(pdpR, c.gm1d2g);
-                    if(STR <= 0) {
-                        double base = c.tdgp1 - c.gm1dgp1 / aR * vR;
-                        sol[0] = WR[0] * pow(base, c.tdgm1);
-                        vhalf = c.tdgp1 * (-aR + c.gm1d2 * vR) - vR;
-                        sol[P] = WR[P] * pow(base, c.tgdgm1);
-                    } else {
-                        sol[0] = WR[0] * pow(pdpR, c.ginv);
-                        sol[P] = p;
-                        vhalf = u - vR;
-                    }
-                } else {
-                    vhalf = 0.;
-                }
-            }
-        } else {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WL[k];
-            double pdpL = p / WL[P];
-            if(p > WL[P]) {
-                double SL = vL - aL * sqrt(c.gp1d2g * pdpL + c.gm1d2g);
-                if(SL < 0) {
-                    sol[0] = WL[0] * (pdpL + c.gm1dgp1) / (c.gm1dgp1 * pdpL + 1.);
-                    sol[P] = p;
-                    vhalf = u - vL;
-                } else {
-                    vhalf = 0.;
-                }
-            } else {
-                double SHL = vL - aL;
-                if(SHL < 0) {
-                    double STL = u - aL * pow(pdpL, c.gm1d2g);
-                    if(STL > 0) {
-                        double base = c.tdgp1 + c.gm1dgp1 / aL * vL;
-                        sol[0] = WL[0] * pow(base, c.tdgm1);
-                        vhalf = c.tdgp1 * (aL + c.gm1d2 * vL) - vL;
-                        sol[P] = WL[P] * pow(base, c.tgdgm1);
-                    } else {
-                        sol[0] = WL[0] * pow(pdpL, c.ginv);
-                        vhalf = u - vL;
-                        sol[P] = p;
-                    }
-                } else {
-                    vhalf = 0.;
-                }
-            }
-        }
-#pragma unroll
-        for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n[d];
-    }
-    st |= ex_check<ND>(sol);
-    return st;
-}
-
-// solve_for_flux + get_flux, ExactRiemannSolver.cpp:815-880
-template <int ND>
-__device__ uint8_t ex_solve_for_flux(const Consts& c, const double* WL, double* WR,
-                                     const double* n, const double* v, bool reflective,
-                                     double* F, unsigned& iters, unsigned& brent,
-                                     unsigned& vac) {
-    const int NC = S<ND>::NC, P = S<ND>::P;
-    double W[NC];
-    double maxmach = 0.;
-    uint8_t st = ex_solve<ND>(c, WL, WR, n, reflective, W, &maxmach, iters, brent, vac);
-#pragma unroll
-    for(int d = 0; d < ND; d++) W[1 + d] += v[d];
-    double Fi[ND][NC];
-#pragma unroll
-    for(int i = 0; i < ND; i++) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) Fi[i][k] = 0.;
-    }
-    if(W[0] != 0.) {
-        double v2 = W[1] * W[1] + W[2] * W[2];
-        if(ND == 3) v2 = W[1] * W[1] + W[2] * W[2] + W[ND] * W[ND];
-        double E = 0.5 * W[0] * v2 + W[P] / (c.gamma - 1.);
-#pragma unroll
-        for(int i = 0; i < ND; i++) {
-            double rel = W[1 + i] - v[i];
-            Fi[i][0] = W[0] * rel;
-#pragma unroll
-            for(int d = 0; d < ND; d++) Fi[i][1 + d] = W[0] * rel * W[1 + d];
-            Fi[i][1 + i] += W[P];
-            Fi[i][P] = rel * E + W[P] * W[1 + i];
-        }
-    }
-#pragma unroll
-    for(int k = 0; k < NC; k++) {
-        double acc = Fi[0][k] * n[0] + Fi[1][k] * n[1];
-        if(ND == 3) acc = acc + Fi[ND - 1][k] * n[ND - 1];
-        F[k] = acc;
-    }
-    return st;
-}
-
-// ----------------------------------------------------------------- HLLC --
-
-template <int ND>
-__device__ __forceinline__ void deboost(const Consts&, const double* v, double* flux) {
-    // HLLCRiemannSolver.cpp:198-206 and :392-400
-    const int P = S<ND>::P;
-    double vn2 = v[0] * v[0] + v[1] * v[1];
-    if(ND == 3) vn2 = v[0] * v[0] + v[1] * v[1] + v[ND - 1] * v[ND - 1];
-#pragma unroll
-    for(int d = 0; d < ND; d++) flux[1 + d] += v[d] * flux[0];
-    if(ND == 3)
-        flux[P] += v[0] * flux[1] + v[1] * flux[2] + v[ND - 1] * flux[ND] + 0.5 * vn2 * flux[0];
-    else
-        flux[P] += v[0] * flux[1] + v[1] * flux[2] + 0.5 * vn2 * flux[0];
-}
-
-// HLLCRiemannSolver::vacuum_flux, HLLCRiemannSolver.cpp:44-209
-template <int ND>
-__device__ void hllc_vacuum_flux(const Consts& c, const double* WL, const double* WR,
-                                 const double* n, const double* v, double uL, double uR,
-                                 double aL, double aR, double* flux) {
-    const int NC = S<ND>::NC, P = S<ND>::P;
-    const double g = c.gamma;
-    double sol[NC];
-    double vhalf;
-    if(!(WR[0] != 0.)) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) sol[k] = WL[k];
-        if(aL > uL) {
-            double SL = uL + 2. * aL / (g - 1.);
-            if(SL > 0.) {
-                double base = 2. / (g + 1.) + (g - 1.) / (g + 1.) / aL * uL;
-                sol[0] = WL[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (aL + 0.5 * (g - 1.) * uL) - uL;
-                sol[P] = WL[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-#pragma unroll
-                for(int k = 0; k < NC; k++) sol[k] = 0.;
-                vhalf = 0.;
-            }
-        } else {
-            vhalf = 0.;
-        }
-    } else if(!(WL[0] != 0.)) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) sol[k] = WR[k];
-        if(-uR < aR) {
-            double SR = uR - 2. * aR / (g - 1.);
-            if(SR >= 0.) {
-#pragma unroll
-                for(int k = 0; k < NC; k++) sol[k] = 0.;
-                vhalf = 0.;
-            } else {
-                double base = 2. / (g + 1.) - (g - 1.) / (g + 1.) / aR * uR;
-                sol[0] = WR[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (-aR + 0.5 * (g - 1.) * uR) - uR;
-                sol[P] = WR[P] * pow(base, 2. * g / (g - 1.));
-            }
-        } else {
-            vhalf = 0.;
-        }
-    } else {
-        double SR = uR - 2. * aR / (g - 1.);
-        double SL = uL + 2. * aL / (g - 1.);
-        if(SR > 0. && SL < 0.) {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = 0.;
-            vhalf = 0.;
-        } else if(SL >= 0.) {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WL[k];
-            if(uL < aL) {
-                double base = 2. / (g + 1.) + (g - 1.) / (g + 1.) / aL * uL;
-                sol[0] = WL[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (aL + 0.5 * (g - 1.) * uL) - uL;
-                sol[P] = WL[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-                vhalf = 0.;
-            }
-        } else {
-#pragma unroll
-            for(int k = 0; k < NC; k++) sol[k] = WR[k];
-            if(-uR < aR) {
-                double base = 2. / (g + 1.) - (g - 1.) / (g + 1.) / aR * uR;
-                sol[0] = WR[0] * pow(base, 2. / (g - 1.));
-                vhalf = 2. / (g + 1.) * (-aR + 0.5 * (g - 1.) * uR) - uR;
-                sol[P] = WR[P] * pow(base, 2. * g / (g - 1.));
-            } else {
-                vhalf = 0.;
-            }
-        }
-    }
-#pragma unroll
-    for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n[d];
-#pragma unroll
-    for(int k = 0; k < NC; k++) flux[k] = 0.;
-    if(!(sol[0] != 0.)) return;
-    double vsol = sol[1] * n[0] + sol[2] * n[1];
-    double v2 = sol[1] * sol[1] + sol[2] * sol[2];
-    if(ND == 3) {
-        vsol += sol[ND] * n[ND - 1];
-        v2 += sol[ND] * sol[ND];
-    }
-    flux[0] = sol[0] * vsol;
-#pragma unroll
-    for(int d = 0; d < ND; d++) flux[1 + d] = sol[0] * sol[1 + d] * vsol + sol[P] * n[d];
-    double e = sol[P] / (g - 1.) / sol[0] + 0.5 * v2;
-    flux[P] = sol[0] * e * vsol + sol[P] * vsol;
-    deboost<ND>(c, v, flux);
-}
-
-// HLLCRiemannSolver::solve_for_flux, HLLCRiemannSolver.cpp:258-403
-template <int ND>
-__device__ uint8_t hllc_solve_for_flux(const Consts& c, const double* WL, double* WR,
-                                       const double* n, const double* v, bool reflective,
-                                       double* flux, unsigned& vac) {
-    const int NC = S<ND>::NC, P = S<ND>::P;
-    const double g = c.gamma;
-#pragma unroll
-    for(int k = 0; k < NC; k++) flux[k] = 0.;
-    if(!(WL[0] != 0.) && (reflective || !(WR[0] != 0.))) return 0;
-    double uL = WL[1] * n[0] + WL[2] * n[1];
-    double uR = WR[1] * n[0] + WR[2] * n[1];
-    if(ND == 3) {
-        uL = WL[1] * n[0] + WL[2] * n[1] + WL[ND] * n[ND - 1];
-        uR = WR[1] * n[0] + WR[2] * n[1] + WR[ND] * n[ND - 1];
-    }
-    if(reflective) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) WR[k] = WL[k];
-        uR = -uL;
-    }
-    double aL = sqrt(g * WL[P] / WL[0]);
-    double aR = sqrt(g * WR[P] / WR[0]);
-    if(!(WL[0] != 0.) || !(WR[0] != 0.) ||
-       2. * aL / (g - 1.) + 2. * aR / (g - 1.) <= uR - uL) {
-        vac = 1;
-        hllc_vacuum_flux<ND>(c, WL, WR, n, v, uL, uR, aL, aR, flux);
-        return 0;
-    }
-    double rhobar = 0.5 * (WL[0] + WR[0]);
-    double abar = 0.5 * (aL + aR);
-    double pPVRS = 0.5 * (WL[P] + WR[P]) - 0.5 * (uR - uL) * rhobar * abar;
-    double pstar = smax(0., pPVRS);
-    double qL = 1.;
-    if(pstar > WL[P]) qL = sqrt(1. + 0.5 * (g + 1.) / g * (pstar / WL[P] - 1.));
-    double qR = 1.;
-    if(pstar > WR[P]) qR = sqrt(1. + 0.5 * (g + 1.) / g * (pstar / WR[P] - 1.));
-    double SL = uL - aL * qL;
-    double SR = uR + aR * qR;
-    double Sstar = (WR[P] - WL[P] + WL[0] * uL * (SL - uL) - WR[0] * uR * (SR - uR)) /
-                   (WL[0] * (SL - uL) - WR[0] * (SR - uR));
-    // both branches of :314-385 have the same shape; pick the side
-    const bool left = Sstar >= 0.;
-    const double* WK = left ? WL : WR;
-    const double uK = left ? uL : uR;
-    const double SK = left ? SL : SR;
-    flux[0] = WK[0] * uK;
-#pragma unroll
-    for(int d = 0; d < ND; d++) flux[1 + d] = WK[0] * WK[1 + d] * uK + WK[P] * n[d];
-    double v2 = WK[1] * WK[1] + WK[2] * WK[2];
-    if(ND == 3) v2 += WK[ND] * WK[ND];
-    double eK = WK[P] / (g - 1.) / WK[0] + 0.5 * v2;
-    flux[P] = WK[0] * eK * uK + WK[P] * uK;
-    if(left ? (SK < 0.) : (SK > 0.)) {
-        double U[NC];
-        U[0] = 1.;
-#pragma unroll
-        for(int d = 0; d < ND; d++) U[1 + d] = WK[1 + d] + (Sstar - uK) * n[d];
-        U[P] = eK + (Sstar - uK) * (Sstar + WK[P] / (WK[0] * (SK - uK)));
-        double fac = WK[0] * (SK - uK) / (SK - Sstar);
-#pragma unroll
-        for(int k = 0; k < NC; k++) U[k] *= fac;
-        flux[0] += SK * (U[0] - WK[0]);
-#pragma unroll
-        for(int d = 0; d < ND; d++) flux[1 + d] += SK * (U[1 + d] - WK[0] * WK[1 + d]);
-        flux[P] += SK * (U[P] - WK[0] * eK);
-    }
-    deboost<ND>(c, v, flux);
-    return 0;
-}
 
 // --------------------------------------------------------------- kernels --
 
