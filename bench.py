@@ -54,6 +54,49 @@ def time_variant(fn, steps, warmup, stream):
             "ms_p50": float(np.median(per)), "wall_s": wall}
 
 
+def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
+    """Config 2: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC,
+    gradients + face fluxes + ordered gather (one full pass per step)."""
+    import torch
+    from shadowfax_amd import mesh as M
+    from shadowfax_amd.fluxpass import FluxPass
+    from shadowfax_amd.riemann import RiemannSolver
+    t0 = time.perf_counter()
+    m = M.voronoi_periodic(M.jittered_grid(n, 0.25, seed=20240603))
+    b = M.prepare(m, M.kh_ics(m), 5.0 / 3.0, kind=kind)
+    t_mesh = time.perf_counter() - t0
+    C, F = b["C"], b["F"]
+    stream = torch.cuda.current_stream()
+    out = {"C": C, "F": F, "host_mesh_s": t_mesh, "solver": "hllc" if kind else "exact"}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = RiemannSolver(kind, 2, 5.0 / 3.0, 1e-8, -5.0, prec)
+        s.enable_stats(False)
+        fp = FluxPass(s, M.DeviceMesh(b, prec, dev))
+        parts = {"gradients": fp.gradients, "faces": lambda fp=fp: fp.face_fluxes(status=False),
+                 "gather": fp.gather}
+        r = time_variant(lambda fp=fp: fp.run(status=False), steps, warmup, stream)
+        for name, fn in parts.items():
+            r["ms_" + name] = time_variant(fn, steps, 1, stream)["ms_mean"]
+        r["face_fluxes_per_s"] = F / (r["ms_mean"] * 1e-3)
+        fp.run()
+        torch.cuda.synchronize()
+        res[prec] = (fp.m.dQ.cpu().numpy(), fp.m.dQL.cpu().numpy())
+        out[prec] = r
+    dQ64, dQL64 = res["fp64"]
+    dQmx = res["mixed"][0]
+    out["mixed_speedup"] = out["fp64"]["ms_mean"] / out["mixed"]["ms_mean"]
+    out["mixed_vs_fp64_rel_maxdQ"] = float(np.abs(dQmx - dQ64).max() / np.abs(dQ64).max())
+    out["mixed_conservation_residual"] = float(np.abs(dQmx.sum(1)).max() / np.abs(dQ64).max())
+    # algorithmic bytes of the FP64 pass: faces {iL, iR, A, midpoint, shift}
+    # 48 B, cells {W, grad, centroid, pos, vel, m, A_tot, dt} 168 B read +
+    # dQ 32 B written
+    alg = 48 * F + 200 * C
+    out["alg_bytes_fp64"] = alg
+    out["fp64"]["achieved_GBs"] = alg / (out["fp64"]["ms_mean"] * 1e-3) / 1e9
+    return out
+
+
 def cpu_baseline(b, N):
     """Reference ExactRiemannSolver compiled from source when oracle/_ref is
     present, else the oracle restatement; one thread, the full batch."""
@@ -87,6 +130,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--N", type=int, default=None)
+    ap.add_argument("--no-fluxpass", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the face-flux path on a single "
@@ -164,6 +208,9 @@ def main():
                                   "exact solver, mixed precision",
                       "N": N, "solver": "exact", "precision": "mixed", "seed": cfg["seed"]},
            "roofline": roof, "variants": variants, "errors": errors}
+    if not args.no_fluxpass:
+        log("flux pass 2D (config 2)")
+        out["fluxpass_2d_kh"] = bench_fluxpass2d(args.steps, args.warmup, dev)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(b, N)

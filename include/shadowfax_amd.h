@@ -135,6 +135,141 @@ int sfx_exact_solve_batch(sfx_solver* h, int64_t N, const double* WL,
                           const uint8_t* reflective, double* Whalf,
                           double* mach, uint8_t* status, void* stream);
 
+
+/* ---- meshes (host side; tessellation stays on the host) -------------- */
+
+/* A Voronoi mesh as the reference's VorTess hands it over (src/VorTess.cpp:
+ * 285-396, src/VorFace.cpp:293-403, src/VorCell.cpp:257-411):
+ *   cells in generator order; per-cell faces in walk (counter-clockwise)
+ *   order; each face created by its first real cell ("left"); a periodic
+ *   copy (reference ghost id 1) never owns a face, so periodic boundary faces
+ *   appear twice; right = -1 marks a reflective mirror.
+ *   shift: right generator position minus the base position (pos[right] for
+ *   a periodic copy or a real neighbour, pos[left] for a mirror).
+ * Arrays are component-major; owned by the library, freed by
+ * sfx_host_mesh_free(). */
+typedef struct {
+    int ndim;
+    int64_t C, F;
+    double* pos;        /* [ndim][C] generator positions */
+    double* volume;     /* [C] */
+    double* centroid;   /* [ndim][C] */
+    int32_t* left;      /* [F] */
+    int32_t* right;     /* [F] */
+    double* area;       /* [F] */
+    double* midpoint;   /* [ndim][F] */
+    double* shift;      /* [ndim][F] */
+    uint8_t* ghost;     /* [F] right generator is a periodic copy */
+    int32_t* twin;      /* [F] other copy of a periodic face, or -1 */
+    int64_t* cf_off;    /* [C+1] */
+    int32_t* cf_face;   /* [cf_off[C]] faces of each cell, walk order */
+} sfx_host_mesh;
+
+/* Periodic Voronoi tessellation of C generators in [0, box[d]) (ndim 2 or
+ * 3), built on the host with nthreads threads (0: all cores).  Stands in for
+ * VorTess::construct (src/VorTess.cpp:153-396), whose Delaunay predicates
+ * need Boost.Multiprecision. */
+int sfx_voronoi_periodic(int ndim, int64_t C, const double* pos, const double* box,
+                         int nthreads, sfx_host_mesh* out);
+void sfx_host_mesh_free(sfx_host_mesh* m);
+
+/* Per-face geometry the flux-pass kernels read, from a raw mesh (host, FP64,
+ * the reference's expressions: src/VorFace.cpp:553-660,728-789):
+ *   dL = midpoint - centroid_L,
+ *   dR = (midpoint - centroid_R) + (pos_R - xRgen)      (right real cell),
+ *   rRL = xRgen - pos_L,  mc = midpoint - 0.5 (pos_L + xRgen),
+ *   rcent = |centroid_L - centroid_R| (0 for mirrors),
+ * with xRgen = base + shift.  All outputs [ndim][F] except rcent [F]. */
+int sfx_face_geometry(int ndim, int64_t C, int64_t F, const double* pos,
+                      const double* centroid, const int32_t* left,
+                      const int32_t* right, const double* midpoint,
+                      const double* shift, double* dL, double* dR, double* rRL,
+                      double* mc, double* rcent);
+
+/* Per-cell face entries in walk order: cf_entry[j] = 2*face + side, side 1
+ * when the cell is the face's right (used by the gradient kernel). */
+int sfx_mesh_cell_entries(int64_t C, int64_t F, const int32_t* left,
+                          const int64_t* cf_off, const int32_t* cf_face,
+                          int32_t* cf_entry);
+
+/* Gather index of the deterministic scatter: for every cell, the (face,
+ * side) entries it receives in ascending face order -- the order in which
+ * VorTessManager::hydro's face loop (src/VorTessManager.cpp:205) applies
+ * increase_dQ.  Every face updates its left cell; its right cell only when
+ * that is a real local cell (not a periodic copy, not a mirror),
+ * src/VorFace.cpp:913-955.  gather_off: [C+1]; gather_entry may be NULL to
+ * only size it (gather_off[C] entries). */
+int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left,
+                          const int32_t* right, const uint8_t* ghost,
+                          int64_t* gather_off, int32_t* gather_entry);
+
+/* ---- face-flux pass (device pointers) -------------------------------- */
+
+/* Device mesh for the flux pass.  "T" arrays are double for an
+ * SFX_PREC_FP64 handle and float for an SFX_PREC_MIXED handle. */
+typedef struct {
+    int ndim;
+    int64_t C, F;
+    /* cells */
+    const void* W;             /* T [ndim+2][C] primitives */
+    const void* vel;           /* T [ndim][C] generator velocities */
+    const double* mass;        /* [C] Q.m */
+    const double* total_area;  /* [C] sum of the cell's face areas */
+    const double* dt;          /* [C] real time of the cell's time step */
+    const double* volume;      /* [C] */
+    const uint8_t* active;     /* [C] starttime == current time, or NULL */
+    /* faces */
+    const int32_t* left;       /* [F] */
+    const int32_t* right;      /* [F] -1: mirror */
+    const uint8_t* ghost;      /* [F] */
+    const int32_t* twin;       /* [F] or NULL (mixed pass: one evaluation per
+                                  periodic pair when given) */
+    const void* area;          /* T [F] */
+    const void* dL;            /* T [ndim][F]  (sfx_face_geometry) */
+    const void* dR;            /* T [ndim][F] */
+    const void* rRL;           /* T [ndim][F] */
+    const void* mc;            /* T [ndim][F] */
+    const void* rcent;         /* T [F] */
+    /* topology */
+    const int64_t* cf_off;     /* [C+1] */
+    const int32_t* cf_entry;   /* sfx_mesh_cell_entries */
+    const int64_t* gather_off; /* [C+1] */
+    const int32_t* gather_entry; /* sfx_mesh_gather_index */
+} sfx_mesh;
+
+/* VorCell::estimate_gradient for every active cell (src/VorCell.cpp:464-530,
+ * loop src/Simulation.cpp:414-420).  grad: T [ndim][ndim+2][C]
+ * (grad[(d*(ndim+2)+k)*C + i] = dW_k/dx_d); inactive cells are untouched. */
+int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream);
+
+/* VorFace::set_v + calculate_flux for every face (src/VorFace.cpp:586-866):
+ * dQL: [ndim+2][F] the flux exchanged through each face, dt*A*F after the
+ * mass-flux limiter, seen from the left cell (the right cell receives -dQL);
+ * face_status: [F] or NULL. */
+int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad,
+                    double* dQL, uint8_t* face_status, void* stream);
+
+/* Deterministic scatter: dQ[(k)*C + i] = sum of the cell's entries in
+ * ascending face order (overwrites dQ). */
+int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL,
+                  double* dQ, void* stream);
+
+/* VorTessManager::hydro (src/VorTessManager.cpp:198-212) preceded by the
+ * gradient estimation: sfx_gradients + sfx_face_fluxes + sfx_gather_dQ.
+ * grad (T [ndim][ndim+2][C]) and dQL ([ndim+2][F]) are caller-owned work
+ * buffers; dQ receives the per-cell change [ndim+2][C]. */
+int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad,
+                       double* dQL, double* dQ, uint8_t* face_status,
+                       void* stream);
+
+/* GasParticle::update_Q then get_W (src/utilities/GasParticle.cpp:617-624,
+ * src/riemann/ExactRiemannSolver.cpp:716-766): Q -= dQ, dQ = 0,
+ * W = get_W(volume, Q) (W is T [ndim+2][C]); cell_status bit
+ * SFX_ST_UPDATEQ where m or e went negative. */
+int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ,
+                    const double* volume, void* W, uint8_t* cell_status,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,7 @@ def lib():
                                  _dp]
         L.sfxo_gradients.argtypes = [ct.POINTER(SolverParams),
                                      ct.POINTER(MeshStruct), _dp]
+        L.sfxo_face_velocities.argtypes = [ct.POINTER(MeshStruct), _dp, _dp]
         L.sfxo_face_flux_pass.restype = ct.c_uint
         L.sfxo_face_flux_pass.argtypes = [ct.POINTER(SolverParams),
                                           ct.POINTER(MeshStruct), _dp, _dp,
@@ -238,6 +239,16 @@ def gradients(kind, mesh, gamma=5.0 / 3.0):
     out = np.zeros((ndim, ndim + 2, int(mesh["C"])))
     lib().sfxo_gradients(ct.byref(s), ct.byref(m), _p(out))
     return out
+
+
+def face_velocities(mesh):
+    """(vface, normal), each (ndim, F)."""
+    m, keep = mesh_struct(mesh)
+    nd, F = int(mesh["ndim"]), int(mesh["F"])
+    v = np.zeros((nd, F))
+    n = np.zeros((nd, F))
+    lib().sfxo_face_velocities(ct.byref(m), _p(v), _p(n))
+    return v, n
 
 
 def face_flux_pass(kind, mesh, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0):

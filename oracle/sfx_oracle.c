@@ -951,12 +951,7 @@ void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m,
         double delta[3][5];
         for(int d = 0; d < 3; d++)
             for(int k = 0; k < 5; k++) delta[d][k] = 0.;
-        if(m->active && !m->active[i]) {
-            for(int d = 0; d < nd; d++)
-                for(int k = 0; k < NC; k++)
-                    grad_out[(int64_t)(d * NC + k) * C + i] = 0.;
-            continue;
-        }
+        if(m->active && !m->active[i]) continue;  /* keeps its gradients */
         double W[5], Wmax[5], Wmin[5], xc[3];
         for(int k = 0; k < NC; k++) W[k] = Wmax[k] = Wmin[k] = CELL(m->W, k, i, C);
         for(int d = 0; d < nd; d++) xc[d] = CELL(m->pos, d, i, C);
@@ -1059,6 +1054,25 @@ static void face_velocity(const sfxo_mesh* m, int64_t f, const double* xl,
                (m->midpoint[2 * F + f] - 0.5 * (xl[2] + xr[2]));
     fac /= rRL2;
     for(int d = 0; d < nd; d++) vface[d] = 0.5 * (vl[d] + vr[d]) + fac * rRL[d];
+}
+
+/* VorFace::set_v and VorFace::get_normal for every face (testing aid) */
+void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal) {
+    int nd = m->ndim;
+    int64_t C = m->C, F = m->F;
+    for(int64_t f = 0; f < F; f++) {
+        double xl[3], xr[3], v[3], dd[4];
+        for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, m->left[f], C);
+        right_gen_pos(m, f, xr);
+        face_velocity(m, f, xl, xr, v);
+        for(int k = 0; k < nd; k++) dd[k] = xl[k] - xr[k];
+        dd[3] = (nd == 3) ? sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2])
+                          : sqrt(dd[0] * dd[0] + dd[1] * dd[1]);
+        for(int k = 0; k < nd; k++) {
+            vface[k * F + f] = v[k];
+            normal[k * F + f] = -dd[k] / dd[3];
+        }
+    }
 }
 
 /* one side's predictor + reconstruction + limiter, src/VorFace.cpp:714-804.

@@ -128,8 +128,12 @@ typedef struct {
 } sfxo_mesh;
 
 /* VorCell::estimate_gradient for every active cell; grad_out has the mesh
- * grad layout.  Inactive cells get zero gradients. */
+ * grad layout.  Inactive cells are left untouched (src/Simulation.cpp:
+ * 414-420 only re-estimates active cells). */
 void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m, double* grad_out);
+
+/* VorFace::set_v and get_normal per face: vface, normal [ndim][F] */
+void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal);
 
 /* VorFace::set_v + VorFace::calculate_flux over faces 0..F-1 in order,
  * accumulating into dQ[(ndim+2)*C] (component-major; caller zeroes it).

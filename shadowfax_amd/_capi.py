@@ -30,14 +30,13 @@ class Stats(ct.Structure):
 
 
 class MeshDesc(ct.Structure):
-    """sfx_mesh (include/shadowfax_amd.h); all device pointers."""
+    """sfx_mesh (include/shadowfax_amd.h); device pointers."""
     _fields_ = [("ndim", ct.c_int), ("C", i64), ("F", i64),
-                ("W", vp), ("centroid", vp), ("pos", vp), ("vel", vp),
-                ("mass", vp), ("total_area", vp), ("dt", vp), ("volume", vp),
-                ("active", vp), ("left", vp), ("right", vp), ("area", vp),
-                ("midpoint", vp), ("shift", vp), ("ghost", vp),
-                ("cf_off", vp), ("cf_face", vp), ("gather_off", vp),
-                ("gather_idx", vp), ("twin", vp)]
+                ("W", vp), ("vel", vp), ("mass", vp), ("total_area", vp), ("dt", vp),
+                ("volume", vp), ("active", vp),
+                ("left", vp), ("right", vp), ("ghost", vp), ("twin", vp),
+                ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
+                ("cf_off", vp), ("cf_entry", vp), ("gather_off", vp), ("gather_entry", vp)]
 
 
 _lib = None
@@ -92,13 +91,11 @@ def lib():
 def _mesh_sigs():
     md = ct.POINTER(MeshDesc)
     return {
-        "sfx_gradients": [vp, md, vp, vp, vp],
-        "sfx_face_flux_pass": [vp, md, vp, vp, vp, vp, vp],
-        "sfx_face_fluxes": [vp, md, vp, vp, vp, vp, vp],
+        "sfx_gradients": [vp, md, vp, vp],
+        "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],
+        "sfx_face_flux_pass": [vp, md, vp, vp, vp, vp, vp],
         "sfx_update_step": [vp, i64, vp, vp, vp, vp, vp, vp],
-        "sfx_mesh_gather_index": [ct.c_int64, ct.c_int64, vp, vp, vp, vp, vp, vp],
-        "sfx_mesh_workspace_bytes": [vp, md],
     }
 
 

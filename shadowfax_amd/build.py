@@ -36,7 +36,8 @@ SOURCES = [
     ("flux_pass_mixed.hip", ["-ffp-contract=on",
                              "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sfx_capi.cpp", []),
-    ("sfx_mesh_capi.cpp", []),
+    ("sfx_mesh_capi.cpp", ["-ffp-contract=off"]),
+    ("mesh_voronoi.cpp", ["-ffp-contract=off"]),
 ]
 
 

@@ -1,0 +1,586 @@
+// flux_pass_impl.h -- the VorCell/VorFace flux pass as gfx950 kernels.
+//
+//   K2 gradients   one lane per cell: VorCell::estimate_gradient
+//                  (Shadowfax src/VorCell.cpp:464-530)
+//   K3 face fluxes one lane per face: VorFace::set_v + calculate_flux
+//                  (src/VorFace.cpp:586-660,670-866) -> dQL per face
+//   K4 gather      one lane per cell: the per-cell sum of the face
+//                  contributions in ascending face order, which is the
+//                  order VorTessManager::hydro's forward face loop
+//                  (src/VorTessManager.cpp:205-210) applies them through
+//                  GasParticle::increase_dQ (src/utilities/GasParticle.cpp:550)
+//   K5 update      one lane per cell: GasParticle::update_Q + get_W
+//                  (src/utilities/GasParticle.cpp:617-624,
+//                  src/riemann/ExactRiemannSolver.cpp:716-766)
+//
+// Template parameters: ND (2, 3), KIND (0 exact, 1 HLLC), T (storage type of
+// cell states, gradients and face geometry) and R (arithmetic type of the
+// reconstruction).  The FP64 parity pass is T = R = double in a TU compiled
+// with -ffp-contract=off, every expression in the reference's association;
+// the mixed pass is T = R = float with the mixed Riemann solvers and FP64
+// dQ.  Faces carry geometry precomputed on the host from the tessellation
+// with the reference's FP64 expressions (see sfx_mesh_capi.cpp):
+//   dL = midpoint - centroid_L,  dR = midpoint - centroid_R + (pos_R - x_Rgen)
+//   rRL = x_Rgen - x_L,  mc = midpoint - (x_L + x_Rgen)/2,
+//   rcent = |centroid_L - centroid_R| (unshifted centroids, as the reference)
+#pragma once
+
+#include "riemann_fp64_impl.h"
+#include "riemann_mixed_impl.h"
+#include "sfx_device.h"
+
+namespace sfx {
+namespace flux {
+
+// topology entries: 2*face + side (side 1: the cell is the face's right)
+__device__ __forceinline__ int32_t ent_face(int32_t e) { return e >> 1; }
+__device__ __forceinline__ bool ent_right(int32_t e) { return e & 1; }
+
+template <typename T>
+struct MeshView {
+    int64_t C, F;
+    const T* W;           // [NC][C]
+    const T* vel;         // [ND][C]
+    const double* mass;   // [C]
+    const double* total_area;
+    const double* dt;
+    const double* volume;
+    const uint8_t* active;  // may be null: all active
+    const int32_t* left;
+    const int32_t* right;
+    const uint8_t* ghost;
+    const int32_t* twin;
+    const T* area;
+    const T* dL;     // [ND][F]
+    const T* dR;     // [ND][F]
+    const T* rRL;    // [ND][F]
+    const T* mc;     // [ND][F]
+    const T* rcent;  // [F]
+    const int64_t* cf_off;
+    const int32_t* cf_entry;
+    const int64_t* gather_off;
+    const int32_t* gather_entry;
+};
+
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return fmin(a, b); }
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
+template <typename R>
+__device__ __forceinline__ R smaxr(R a, R b) { return (a < b) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R sminr(R a, R b) { return (b < a) ? b : a; }
+
+// Mirror of a velocity through the face: transform, negate the normal
+// component, invtransform (src/VorFace.cpp:417-542); the basis depends only
+// on rRL = right - left.
+template <int ND, typename R>
+__device__ __forceinline__ void mirror_velocity(R* vel, const R* rRL) {
+    if(ND == 2) {
+        // d = left - right = -rRL
+        R d0 = -rRL[0], d1 = -rRL[1];
+        R d2 = sqrt(d0 * d0 + d1 * d1);
+        R cost = -d0 / d2;
+        R sint = -d1 / d2;
+        R W1 = vel[0] * cost + vel[1] * sint;
+        R W2 = -vel[0] * sint + vel[1] * cost;
+        W1 = -W1;
+        R V1 = W1 * cost - W2 * sint;
+        R V2 = W1 * sint + W2 * cost;
+        vel[0] = V1;
+        vel[1] = V2;
+    } else {
+        R n0 = rRL[0], n1 = rRL[1], n2 = rRL[ND - 1];
+        R n3 = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        n0 /= n3;
+        n1 /= n3;
+        n2 /= n3;
+        R o0, o1, p0, p1, p2;
+        if(n0 != R(0) || n1 != R(0)) {
+            o0 = -n1;
+            o1 = n0;
+            R o3 = sqrt(o0 * o0 + o1 * o1);
+            o0 /= o3;
+            o1 /= o3;
+            p0 = -n2 * o1;
+            p1 = n2 * o0;
+            p2 = o3;
+        } else {
+            o0 = 0;
+            o1 = 1;
+            p0 = -1;
+            p1 = 0;
+            p2 = 0;
+        }
+        R v0 = vel[0], v1 = vel[1], v2 = vel[ND - 1];
+        R t1 = v0 * n0 + v1 * n1 + v2 * n2;
+        R t2 = v0 * o0 + v1 * o1;
+        R t3 = v0 * p0 + v1 * p1 + v2 * p2;
+        t1 = -t1;
+        vel[0] = t1 * n0 + t2 * o0 + t3 * p0;
+        vel[1] = t1 * n1 + t2 * o1 + t3 * p1;
+        vel[ND - 1] = t1 * n2 + t3 * p2;
+    }
+}
+
+// get_time_derivative (ExactRiemannSolver.cpp:892-934 / HLLCRiemannSolver.cpp:
+// 415-450); g[d][k] = dW_k/dx_d.  The 2D dWdt[2] uses gradients[1].vy()
+// twice, as the reference does (:920 / :443).
+template <int ND, typename R>
+__device__ __forceinline__ void time_derivative(R gamma, const R* W, const R (*g)[ND + 2],
+                                                R* dWdt) {
+    if(ND == 3) {
+        const R* gx = g[0];
+        const R* gy = g[1];
+        const R* gz = g[ND - 1];
+        dWdt[0] = -W[0] * (gx[1] + gy[2] + gz[3]) - W[1] * gx[0] - W[2] * gy[0] - W[3] * gz[0];
+        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - W[3] * gz[1] - gx[4] / W[0];
+        dWdt[2] = -W[1] * gx[2] - W[2] * gy[2] - W[3] * gz[2] - gy[4] / W[0];
+        dWdt[3] = -W[1] * gx[3] - W[2] * gy[3] - W[3] * gz[3] - gz[4] / W[0];
+        dWdt[ND + 1] = -gamma * W[4] * (gx[1] + gy[2] + gz[3]) - W[1] * gx[4] - W[2] * gy[4] -
+                       W[3] * gz[4];
+    } else {
+        const R* gx = g[0];
+        const R* gy = g[1];
+        dWdt[0] = -W[1] * gx[0] - W[2] * gy[0] - W[0] * (gx[1] + gy[2]);
+        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - gx[3] / W[0];
+        dWdt[2] = -W[1] * gy[2] - W[2] * gy[2] - gy[3] / W[0];
+        dWdt[ND + 1] = -gamma * W[3] * (gx[1] + gy[2]) - W[1] * gx[3] - W[2] * gy[3];
+    }
+}
+
+// VorFace::limit, scalar (src/VorFace.hpp:92-137); dnrm = |d| precomputed
+template <typename R>
+__device__ __forceinline__ R limit1(R phimid0, R phiL, R phiR, R dnrm, R r) {
+    const R psi1 = R(0.5), psi2 = R(0.25);
+    R delta1 = psi1 * fabs(phiL - phiR);
+    R delta2 = psi2 * fabs(phiL - phiR);
+    R phimin = rmin(phiL, phiR);
+    R phimax = rmax(phiL, phiR);
+    R phibar = phiL + dnrm / r * (phiR - phiL);
+    R phiplus = ((phimax + delta1) * phimax > R(0)) ? phimax + delta1
+                                                    : phimax / (R(1) + delta1 / fabs(phimax));
+    R phiminus = ((phimin - delta1) * phimin > R(0)) ? phimin - delta1
+                                                     : phimin / (R(1) + delta1 / fabs(phimin));
+    if(phiL == phiR) return phiL;
+    if(phiL < phiR) return rmax(phiminus, rmin(phibar + delta2, phimid0));
+    return rmin(phiplus, rmax(phibar - delta2, phimid0));
+}
+
+// One side of VorFace::calculate_flux's predictor (src/VorFace.cpp:714-804)
+template <int ND, typename R>
+__device__ __forceinline__ void predict_side(R gamma, const R* Wraw, const R* Wother,
+                                             const R (*g)[ND + 2], const R* d, R r,
+                                             bool limited, R dtp, R* out) {
+    const int NC = ND + 2, P = ND + 1;
+    R dWdt[NC], deltaW[NC], Wt[NC];
+    time_derivative<ND, R>(gamma, Wraw, g, dWdt);
+#pragma unroll
+    for(int k = 0; k < NC; k++) deltaW[k] = R(0.5) * dtp * dWdt[k];
+    // VorFace::reconstruct (src/VorFace.hpp:174-196)
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        if(ND == 3)
+            Wt[k] = Wraw[k] + (g[0][k] * d[0] + g[1][k] * d[1] + g[ND - 1][k] * d[ND - 1]);
+        else
+            Wt[k] = Wraw[k] + (g[0][k] * d[0] + g[1][k] * d[1]);
+    }
+    if(limited) {
+        R dnrm = (ND == 3) ? sqrt(d[0] * d[0] + d[1] * d[1] + d[ND - 1] * d[ND - 1])
+                           : sqrt(d[0] * d[0] + d[1] * d[1]);
+#pragma unroll
+        for(int k = 0; k < NC; k++) Wt[k] = limit1<R>(Wt[k], Wraw[k], Wother[k], dnrm, r);
+    }
+    if(Wt[0] < R(0) || Wt[P] < R(0)) {
+#pragma unroll
+        for(int k = 0; k < NC; k++) Wt[k] = Wraw[k];
+    }
+#pragma unroll
+    for(int k = NC; k--;) Wt[k] += deltaW[k];
+    if(Wt[0] < R(0) || Wt[P] < R(0)) {
+#pragma unroll
+        for(int k = 0; k < NC; k++) Wt[k] -= deltaW[k];
+    }
+#pragma unroll
+    for(int k = 0; k < NC; k++) out[k] = Wt[k];
+}
+
+// ------------------------------------------------------------- K2 grads --
+
+template <int ND, typename T, typename R>
+__global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restrict__ grad) {
+    const int NC = ND + 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= m.C) return;
+    if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
+    const int64_t C = m.C, F = m.F;
+    R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        W[k] = Wmax[k] = Wmin[k] = (R)m.W[k * C + i];
+#pragma unroll
+        for(int l = 0; l < ND; l++) delta[l][k] = R(0);
+    }
+    const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
+    for(int64_t j = j1; j-- > j0;) {
+        const int32_t e = m.cf_entry[j];
+        const int32_t f = ent_face(e);
+        const bool isR = ent_right(e);
+        const R A = (R)m.area[f];
+        if(!(A != R(0))) continue;
+        R c[ND], rij[ND], rr[ND];
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            c[l] = (R)m.mc[l * F + f];
+            rr[l] = (R)m.rRL[l * F + f];
+            rij[l] = isR ? rr[l] : -rr[l];
+        }
+        R rnorm = (ND == 3) ? sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[ND - 1] * rij[ND - 1])
+                            : sqrt(rij[0] * rij[0] + rij[1] * rij[1]);
+        R Wj[NC];
+        const int32_t Rc = m.right[f];
+        if(!isR && Rc < 0) {
+#pragma unroll
+            for(int k = 0; k < NC; k++) Wj[k] = W[k];
+            mirror_velocity<ND, R>(Wj + 1, rr);
+        } else {
+            const int64_t o = isR ? m.left[f] : Rc;
+#pragma unroll
+            for(int k = 0; k < NC; k++) Wj[k] = (R)m.W[k * C + o];
+        }
+#pragma unroll
+        for(int l = ND; l--;) {
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                R t = (Wj[k] - W[k]) * c[l] / rnorm - R(0.5) * (W[k] + Wj[k]) * rij[l] / rnorm;
+                delta[l][k] += A * t;
+            }
+        }
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            Wmax[k] = smaxr<R>(Wmax[k], Wj[k]);
+            Wmin[k] = sminr<R>(Wmin[k], Wj[k]);
+        }
+    }
+    const R V = (R)m.volume[i];
+#pragma unroll
+    for(int l = ND; l--;)
+#pragma unroll
+        for(int k = 0; k < NC; k++) delta[l][k] /= V;
+    R alpha[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) alpha[k] = R(1);
+    for(int64_t j = j1; j-- > j0;) {
+        const int32_t e = m.cf_entry[j];
+        const int32_t f = ent_face(e);
+        if(!((R)m.area[f] != R(0))) continue;
+        const T* dsrc = ent_right(e) ? m.dR : m.dL;
+        R dd[ND];
+#pragma unroll
+        for(int l = 0; l < ND; l++) dd[l] = (R)dsrc[l * F + f];
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            R dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
+            if(ND == 3) dp += delta[ND - 1][k] * dd[ND - 1];
+            R q1 = (dp != R(0)) ? (Wmax[k] - W[k]) / dp : R(1);
+            R q2 = (dp != R(0)) ? (Wmin[k] - W[k]) / dp : R(1);
+            alpha[k] = sminr<R>(alpha[k], smaxr<R>(q1, q2));
+        }
+    }
+#pragma unroll
+    for(int l = ND; l--;)
+#pragma unroll
+        for(int k = 0; k < NC; k++) grad[(int64_t)(l * NC + k) * C + i] = (T)(delta[l][k] * alpha[k]);
+}
+
+// ------------------------------------------------------- K3 face fluxes --
+
+template <int ND, typename T, typename R>
+__device__ __forceinline__ void load_grad(const T* grad, int64_t C, int64_t cell,
+                                          R (*g)[ND + 2]) {
+    const int NC = ND + 2;
+#pragma unroll
+    for(int l = 0; l < ND; l++)
+#pragma unroll
+        for(int k = 0; k < NC; k++) g[l][k] = (R)grad[(int64_t)(l * NC + k) * C + cell];
+}
+
+// MIXED: 0 = FP64 parity solvers, 1 = mixed solvers (and, for a periodic
+// face whose other copy has a lower index, no evaluation: the gather applies
+// that copy's flux with the opposite sign).
+template <int ND, int KIND, int MIXED, typename T, typename R>
+__global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
+                                                        Consts cd, mixed::CF cf,
+                                                        double* __restrict__ dQL,
+                                                        uint8_t* __restrict__ fstatus,
+                                                        unsigned long long* __restrict__ ctr) {
+    const int NC = ND + 2, P = ND + 1;
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned iters = 0, brent = 0, vac = 0, fail = 0, solved = 0;
+    if(f < m.F) {
+        const int64_t C = m.C, F = m.F;
+        const int32_t L = m.left[f], Rc = m.right[f];
+        const bool has_r = Rc >= 0;
+        uint8_t st = 0;
+        double out[NC];
+#pragma unroll
+        for(int k = 0; k < NC; k++) out[k] = 0.0;
+        const R A = (R)m.area[f];
+        const bool actL = m.active ? m.active[L] != 0 : true;
+        const bool actR = has_r ? (m.active ? m.active[Rc] != 0 : true) : false;
+        bool skip = !(A != R(0)) || (!has_r && !actL) || (has_r && !actR && !actL);
+        if(MIXED && m.twin && m.ghost[f] && m.twin[f] >= 0 && m.twin[f] < f) skip = true;
+        if(!skip) {
+            solved = 1;
+            R rr[ND], mcv[ND], vl[ND], vr[ND], vface[ND];
+#pragma unroll
+            for(int l = 0; l < ND; l++) {
+                rr[l] = (R)m.rRL[l * F + f];
+                mcv[l] = (R)m.mc[l * F + f];
+                vl[l] = (R)m.vel[l * C + L];
+                vr[l] = has_r ? (R)m.vel[l * C + Rc] : R(0);
+            }
+            // VorFace::set_v (src/VorFace.cpp:586-660)
+            if(has_r || ND == 2) {
+                if(!has_r) {
+#pragma unroll
+                    for(int l = 0; l < ND; l++) vr[l] = vl[l];
+                    mirror_velocity<ND, R>(vr, rr);
+                }
+                R rRL2 = rr[0] * rr[0] + rr[1] * rr[1];
+                if(ND == 3) rRL2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[ND - 1] * rr[ND - 1];
+                R fac = (vl[0] - vr[0]) * mcv[0];
+                fac += (vl[1] - vr[1]) * mcv[1];
+                if(ND == 3) fac += (vl[ND - 1] - vr[ND - 1]) * mcv[ND - 1];
+                fac /= rRL2;
+#pragma unroll
+                for(int l = 0; l < ND; l++) vface[l] = R(0.5) * (vl[l] + vr[l]) + fac * rr[l];
+            } else {
+#pragma unroll
+                for(int l = 0; l < ND; l++) vface[l] = R(0);
+            }
+            // dt, :695-699
+            double dt = m.dt[L];
+            if(has_r) dt = smin(dt, m.dt[Rc]);
+            R WL[NC], WR[NC];
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                WL[k] = (R)m.W[k * C + L];
+                WR[k] = has_r ? (R)m.W[k * C + Rc] : R(0);
+            }
+#pragma unroll
+            for(int l = 0; l < ND; l++) {
+                WL[1 + l] -= vface[l];
+                WR[1 + l] -= vface[l];
+            }
+            const R rc = has_r ? (R)m.rcent[f] : R(0);
+            R WRrec[NC], WLrec[NC];
+            const R gam = (R)cd.gamma;
+            if(has_r) {
+                R g[ND][NC], d[ND];
+                load_grad<ND, T, R>(grad, C, Rc, g);
+#pragma unroll
+                for(int l = 0; l < ND; l++) d[l] = (R)m.dR[l * F + f];
+                predict_side<ND, R>(gam, WR, WL, g, d, rc, true, (R)dt, WRrec);
+            }
+            {
+                R g[ND][NC], d[ND];
+                load_grad<ND, T, R>(grad, C, L, g);
+#pragma unroll
+                for(int l = 0; l < ND; l++) d[l] = (R)m.dL[l * F + f];
+                predict_side<ND, R>(gam, WL, WR, g, d, rc, has_r, (R)dt, WLrec);
+            }
+            if(WL[0] != R(0)) {
+#pragma unroll
+                for(int k = 0; k < NC; k++) WL[k] = WLrec[k];
+            }
+            if(WR[0] != R(0)) {
+#pragma unroll
+                for(int k = 0; k < NC; k++) WR[k] = WRrec[k];
+            }
+            // VorFace::get_normal (src/VorFace.cpp:553-571): n = -(xL - xR)/|xL - xR|
+            R nrm = (ND == 3) ? sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[ND - 1] * rr[ND - 1])
+                              : sqrt(rr[0] * rr[0] + rr[1] * rr[1]);
+            double flux[NC];
+            if(!MIXED) {
+                double n64[ND], v64[ND], wl[NC], wr[NC];
+#pragma unroll
+                for(int l = 0; l < ND; l++) {
+                    n64[l] = (double)(rr[l] / nrm);
+                    v64[l] = (double)vface[l];
+                }
+#pragma unroll
+                for(int k = 0; k < NC; k++) {
+                    wl[k] = (double)WL[k];
+                    wr[k] = (double)WR[k];
+                }
+                if(KIND == 0)
+                    st |= fp64::ex_solve_for_flux<ND>(cd, wl, wr, n64, v64, !has_r, flux, iters,
+                                                      brent, vac);
+                else
+                    st |= fp64::hllc_solve_for_flux<ND>(cd, wl, wr, n64, v64, !has_r, flux, vac);
+            } else {
+                double n64[ND], v64[ND], wl64[NC], wr64[NC];
+                const double nrm64 = sqrt((double)rr[0] * (double)rr[0] +
+                                          (double)rr[1] * (double)rr[1] +
+                                          (ND == 3 ? (double)rr[ND - 1] * (double)rr[ND - 1] : 0.0));
+#pragma unroll
+                for(int l = 0; l < ND; l++) {
+                    n64[l] = (double)rr[l] / nrm64;
+                    v64[l] = (double)vface[l];
+                }
+#pragma unroll
+                for(int k = 0; k < NC; k++) {
+                    wl64[k] = (double)WL[k];
+                    wr64[k] = (double)WR[k];
+                }
+                if(KIND == 0) {
+                    float wlf[NC], wrf[NC], nf[ND];
+#pragma unroll
+                    for(int k = 0; k < NC; k++) {
+                        wlf[k] = (float)WL[k];
+                        wrf[k] = (float)WR[k];
+                    }
+#pragma unroll
+                    for(int l = 0; l < ND; l++) nf[l] = (float)n64[l];
+                    st |= mixed::exact_mixed<ND>(cf, cd, wlf, wrf, nf, !has_r, wl64, wr64, n64,
+                                                 v64, flux, iters, vac);
+                } else {
+                    st |= fp64::hllc_solve_for_flux<ND>(cd, wl64, wr64, n64, v64, !has_r, flux,
+                                                        vac);
+                }
+            }
+            // dQL = (dt A) F, dQR = -dQL (src/VorFace.cpp:834-835)
+            const double dA = dt * (double)A;
+            double dQ[NC];
+#pragma unroll
+            for(int k = 0; k < NC; k++) dQ[k] = dA * flux[k];
+            // mass flux limiter, :840-866; dQR stays -dQL bitwise
+            double timefac = 1.;
+            if(has_r) {
+                double dtmax = m.dt[L];
+                dtmax = smax(dtmax, m.dt[Rc]);
+                timefac = dt / dtmax;
+            }
+            double areafac = (double)A / m.total_area[L];
+            areafac *= timefac;
+            if(dQ[0] > areafac * m.mass[L]) {
+                double corfac = fabs(areafac * m.mass[L] / dQ[0]);
+#pragma unroll
+                for(int k = 0; k < NC; k++) dQ[k] *= corfac;
+            }
+            if(has_r) {
+                double af = (double)A / m.total_area[Rc];
+                af *= timefac;
+                // dQR[0] = -dQL[0]
+                if(-dQ[0] > af * m.mass[Rc]) {
+                    double corfac = fabs(af * m.mass[Rc] / -dQ[0]);
+#pragma unroll
+                    for(int k = 0; k < NC; k++) dQ[k] = -(-dQ[k] * corfac);
+                }
+            }
+            if(flux[0] != flux[0]) st |= ST_NANFLUX;
+#pragma unroll
+            for(int k = 0; k < NC; k++) out[k] = dQ[k];
+            fail = st != 0;
+        }
+#pragma unroll
+        for(int k = 0; k < NC; k++) dQL[(int64_t)k * F + f] = out[k];
+        if(fstatus) fstatus[f] = st;
+    }
+    const unsigned cv[CNT_N] = {solved, brent, vac, iters, fail};
+    block_add_counters(ctr, cv);
+}
+
+// ------------------------------------------------------------ K4 gather --
+
+template <int ND, int MIXED>
+__global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
+                                                     const int64_t* __restrict__ goff,
+                                                     const int32_t* __restrict__ gent,
+                                                     const uint8_t* __restrict__ ghost,
+                                                     const int32_t* __restrict__ twin,
+                                                     const double* __restrict__ dQL,
+                                                     double* __restrict__ dQ) {
+    const int NC = ND + 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= C) return;
+    double acc[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) acc[k] = 0.0;
+    const int64_t j0 = goff[i], j1 = goff[i + 1];
+    for(int64_t j = j0; j < j1; j++) {
+        const int32_t e = gent[j];
+        int32_t f = ent_face(e);
+        bool neg = ent_right(e);
+        if(MIXED && twin && ghost[f]) {
+            const int32_t t = twin[f];
+            if(t >= 0 && t < f) {
+                f = t;       // the other copy carries this face's flux
+                neg = !neg;  // seen from its right cell
+            }
+        }
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            const double x = dQL[(int64_t)k * F + f];
+            acc[k] += neg ? -x : x;
+        }
+    }
+#pragma unroll
+    for(int k = 0; k < NC; k++) dQ[(int64_t)k * C + i] = acc[k];
+}
+
+// ------------------------------------------------------------ K5 update --
+
+// GasParticle::update_Q then get_W (use_energy = true); T: storage of W.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, double* __restrict__ Q,
+                                                     double* __restrict__ dQ,
+                                                     const double* __restrict__ volume,
+                                                     T* __restrict__ Wout,
+                                                     uint8_t* __restrict__ cstatus,
+                                                     unsigned long long* __restrict__ ctr) {
+    const int NC = ND + 2, P = ND + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned fail = 0;
+    if(i < C) {
+        double q[NC], w[NC];
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            q[k] = Q[k * C + i] - dQ[k * C + i];
+            dQ[k * C + i] = 0.0;
+        }
+        uint8_t st = (q[0] < 0. || q[P] < 0.) ? ST_UPDATEQ : 0;
+        const double g = cd.gamma, V = volume[i];
+#pragma unroll
+        for(int k = 0; k < NC; k++) w[k] = 0.;
+        if(q[0] != 0.) {
+            w[0] = q[0] / V;
+#pragma unroll
+            for(int d = 0; d < ND; d++) w[1 + d] = q[1 + d] / q[0];
+            double p2 = q[1] * q[1] + q[2] * q[2];
+            if(ND == 3) p2 = q[1] * q[1] + q[2] * q[2] + q[ND] * q[ND];
+            w[P] = (g - 1.) * (q[P] - 0.5 * p2 / q[0]) / V;
+            if(w[P] < 1.e-30) {
+                w[P] = 1.e-30;
+                q[P] = 0.5 * p2 / q[0] + w[P] * V / (g - 1.);
+            }
+            if(w[0] < 1.e-30) {
+#pragma unroll
+                for(int k = 0; k < NC; k++) w[k] = 0.;
+            }
+        }
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            Q[k * C + i] = q[k];
+            Wout[k * C + i] = (T)w[k];
+        }
+        if(cstatus) cstatus[i] = st;
+        fail = st != 0;
+    }
+    const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
+    block_add_counters(ctr, cv);
+}
+
+}  // namespace flux
+}  // namespace sfx

@@ -1,0 +1,72 @@
+// flux_pass_mixed.hip -- mixed instantiation of the flux-pass kernels
+// (flux_pass_impl.h): FP32 storage of cell states, gradients and face
+// geometry, FP32 reconstruction and limiting, the mixed Riemann solvers
+// (riemann_mixed_impl.h), FP64 face fluxes and FP64 accumulation.  Each
+// periodic boundary face is evaluated once and applied with opposite signs
+// to its two cells, so mass, momentum and energy are conserved to FP64
+// round-off (the reference evaluates both copies, src/VorFace.cpp:913-948).
+#include "flux_pass_impl.h"
+#include "sfx_launch.h"
+
+namespace sfx {
+
+static inline dim3 grid_for(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+template <>
+hipError_t launch_gradients<float>(int ndim, const flux::MeshView<float>& m, float* grad,
+                                   hipStream_t s) {
+    if(m.C <= 0) return hipSuccess;
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::gradient_kernel<3, float, float>), grid_for(m.C, 256),
+                           dim3(256), 0, s, m, grad);
+    else
+        hipLaunchKernelGGL((flux::gradient_kernel<2, float, float>), grid_for(m.C, 256),
+                           dim3(256), 0, s, m, grad);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_face_fluxes<float>(int ndim, int kind, const flux::MeshView<float>& m,
+                                     const float* grad, const Consts& c, double* dQL,
+                                     uint8_t* fstatus, unsigned long long* ctr, hipStream_t s) {
+    if(m.F <= 0) return hipSuccess;
+    const mixed::CF cf = mixed::make_cf(c);
+#define SFX_FACE(ND, K)                                                                   \
+    hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, float>), grid_for(m.F, 256), \
+                       dim3(256), 0, s, m, grad, c, cf, dQL, fstatus, ctr)
+    if(ndim == 3 && kind == 0) SFX_FACE(3, 0);
+    else if(ndim == 3) SFX_FACE(3, 1);
+    else if(kind == 0) SFX_FACE(2, 0);
+    else SFX_FACE(2, 1);
+#undef SFX_FACE
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* goff,
+                               const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
+                               const double* dQL, double* dQ, hipStream_t s) {
+    if(C <= 0) return hipSuccess;
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::gather_kernel<3, 1>), grid_for(C, 256), dim3(256), 0, s, C, F,
+                           goff, gent, ghost, twin, dQL, dQ);
+    else
+        hipLaunchKernelGGL((flux::gather_kernel<2, 1>), grid_for(C, 256), dim3(256), 0, s, C, F,
+                           goff, gent, ghost, twin, dQL, dQ);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_update<float>(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
+                                const double* volume, float* W, uint8_t* cstatus,
+                                unsigned long long* ctr, hipStream_t s) {
+    if(C <= 0) return hipSuccess;
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::update_kernel<3, float>), grid_for(C, 256), dim3(256), 0, s, c,
+                           C, Q, dQ, volume, W, cstatus, ctr);
+    else
+        hipLaunchKernelGGL((flux::update_kernel<2, float>), grid_for(C, 256), dim3(256), 0, s, c,
+                           C, Q, dQ, volume, W, cstatus, ctr);
+    return hipGetLastError();
+}
+
+}  // namespace sfx

@@ -51,13 +51,8 @@ __global__ __launch_bounds__(256) void solve_for_flux_kernel(
         if(status) status[i] = st;
         fail = st != 0;
     }
-    if(ctr) {
-        wave_add(ctr + CNT_NEVAL, valid);
-        wave_add(ctr + CNT_NBRENT, brent);
-        wave_add(ctr + CNT_NVACUUM, vac);
-        wave_add(ctr + CNT_NITER, iters);
-        wave_add(ctr + CNT_NFAIL, fail);
-    }
+    const unsigned cv[CNT_N] = {valid, brent, vac, iters, fail};
+    block_add_counters(ctr, cv);
 }
 
 template <int ND>
@@ -91,13 +86,8 @@ __global__ __launch_bounds__(256) void exact_solve_kernel(
         if(status) status[i] = st;
         fail = st != 0;
     }
-    if(ctr) {
-        wave_add(ctr + CNT_NEVAL, valid);
-        wave_add(ctr + CNT_NBRENT, brent);
-        wave_add(ctr + CNT_NVACUUM, vac);
-        wave_add(ctr + CNT_NITER, iters);
-        wave_add(ctr + CNT_NFAIL, fail);
-    }
+    const unsigned cv[CNT_N] = {valid, brent, vac, iters, fail};
+    block_add_counters(ctr, cv);
 }
 
 }  // namespace fp64

@@ -1,336 +1,10 @@
-// riemann_mixed.hip -- mixed-precision Riemann solvers for gfx950.
-//
-// Storage type T is float (52 B in + 20 B out per 3D problem) or double.
-//
-// Exact solver (same problem as Shadowfax ExactRiemannSolver::solve_for_flux,
-// src/riemann/ExactRiemannSolver.cpp:108-290,815-880): the pressure
-// iteration -- the transcendental-heavy bulk of the work -- runs in FP32 on
-// the VALU's fast paths (v_log_f32 / v_exp_f32 / v_rcp_f32 / v_sqrt_f32);
-// one Newton step in FP64 then gives p* and u* to ~1e-13, and the sampling
-// of the solution at x/t = 0 and the flux assembly run in FP64 (a few dozen
-// operations; at Mach numbers ~1e3 the flux is too ill-conditioned for an
-// FP32 sampled state to meet 1e-6).
-//   * f(p) = fL + fR + du is monotone increasing and concave, so Newton from
-//     the left of the root converges monotonically; a bracket [lo, hi]
-//     catches a first step from the right that leaves it (no serial Brent
-//     state machine, a bounded and nearly uniform iteration count per wave);
-//   * the reference's pure-Newton scheme (f(0) < cutoff) stops at p* = 0
-//     when its first step goes negative; that outcome is reproduced;
-//   * pow(x, -(g+1)/(2g)) = pow(x, (g-1)/(2g)) / x: one transcendental per
-//     side per step; for g = 5/3 every other power is an integer power of
-//     x^(1/5), and the FP64 fifth root is an FP32 seed plus one Newton step.
-//
-// HLLC (HLLCRiemannSolver::solve_for_flux, src/riemann/HLLCRiemannSolver.cpp:
-// 258-403): HBM-bound, so FP32 storage carries the speedup and the
-// arithmetic runs in FP64 registers (the reference's expressions, shared
-// with the plain path through riemann_fp64_impl.h) at no cost in time.
-#include "riemann_fp64_impl.h"
-#include "sfx_device.h"
+// riemann_mixed.hip -- batched mixed-precision solve_for_flux kernels; the
+// algorithm and its precision split are described in riemann_mixed_impl.h.
+#include "riemann_mixed_impl.h"
 #include "sfx_launch.h"
 
 namespace sfx {
 namespace mixed {
-
-// FP32 consts
-struct CF {
-    float gamma, gp1d2g, gm1d2g, gm1dgp1, tdgp1, tdgm1, gm1d2, tgdgm1, ginv, ginvm1;
-    float gm1inv;      // 1/(gamma-1)
-    float q_fac;       // 0.5*(gamma+1)/gamma (HLLC)
-    double gamma_d, tdgp1_d, tdgm1_d, gm1dgp1_d, gm1d2g_d, cutoff_d;
-    float halftol;     // 0.5*tol: floor of the initial guess
-    int fast53;
-    int itmax;
-};
-
-__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float sqrtf_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float log2f_fast(float x) { return __builtin_amdgcn_logf(x); }
-__device__ __forceinline__ float exp2f_fast(float x) { return __builtin_amdgcn_exp2f(x); }
-// x^e for x >= 0 (x = 0 gives 0 for e > 0, inf for e < 0)
-__device__ __forceinline__ float powf_fast(float x, float e) {
-    return exp2f_fast(e * log2f_fast(x));
-}
-__device__ __forceinline__ float ipow3(float b) { return b * b * b; }
-__device__ __forceinline__ float ipow5(float b) {
-    float b2 = b * b;
-    return b2 * b2 * b;
-}
-
-// x^(1/5) in FP64 for x > 0: exponent split so the FP32 seed stays in range,
-// then one Newton step on y^5 = z.
-__device__ __forceinline__ double fifth_root_d(double x) {
-    int e = __builtin_amdgcn_frexp_exp(x);      // x = m 2^e, m in [0.5, 1)
-    double m = __builtin_amdgcn_frexp_mant(x);
-    int q = (e >= 0) ? e / 5 : -((-e + 4) / 5);  // floor(e/5)
-    int r = e - 5 * q;                           // 0..4
-    double z = __builtin_ldexp(m, r);            // [0.5, 16)
-    double y = (double)exp2f_fast(0.2f * log2f_fast((float)z));
-    double y2 = y * y;
-    y = 0.8 * y + 0.2 * z / (y2 * y2);
-    return __builtin_ldexp(y, q);
-}
-
-// x^gm1d2g in FP64
-__device__ __forceinline__ double pow_gm1d2g_d(const CF& c, double x) {
-    if(c.fast53) return fifth_root_d(x);
-    return pow(x, c.gm1d2g_d);
-}
-
-template <int ND, typename T>
-struct IO {
-    static constexpr int NC = ND + 2;
-    static constexpr int P = ND + 1;
-};
-
-// One side of the pressure function, FP32: value and derivative.
-struct SideF {
-    float rho, p, a, A, B, rpinv, tda, ra_inv;
-};
-
-__device__ __forceinline__ SideF make_side(const CF& c, float rho, float p, float a) {
-    SideF s;
-    s.rho = rho;
-    s.p = p;
-    s.a = a;
-    s.A = c.tdgp1 * rcpf(rho);
-    s.B = c.gm1dgp1 * p;
-    s.rpinv = rcpf(p);
-    s.tda = c.tdgm1 * a;
-    s.ra_inv = rcpf(rho * a);
-    return s;
-}
-
-__device__ __forceinline__ void side_f(const CF& c, const SideF& s, float p, float& f, float& df) {
-    if(p > s.p) {
-        float r = rcpf(p + s.B);
-        float sq = sqrtf_fast(s.A * r);
-        float dp = p - s.p;
-        f = dp * sq;
-        df = (1.f - 0.5f * dp * r) * sq;
-    } else {
-        float x = p * s.rpinv;
-        float y = powf_fast(x, c.gm1d2g);
-        f = s.tda * (y - 1.f);
-        df = s.ra_inv * y * rcpf(x);
-    }
-}
-
-// FP64 side evaluation at p for the final Newton step
-__device__ __forceinline__ void side_f_d(const CF& c, double rho, double pK, double a, double p,
-                                         double& f, double& df) {
-    if(p > pK) {
-        double A = c.tdgp1_d / rho;
-        double B = c.gm1dgp1_d * pK;
-        double r = 1.0 / (p + B);
-        double sq = sqrt(A * r);
-        double dp = p - pK;
-        f = dp * sq;
-        df = (1.0 - 0.5 * dp * r) * sq;
-    } else {
-        double x = p / pK;
-        double y = pow_gm1d2g_d(c, x);
-        f = c.tdgm1_d * a * (y - 1.0);
-        df = y / (rho * a * x);
-    }
-}
-
-// x^e in FP64 for the sampling: x^(1/5) powers for g = 5/3, pow otherwise
-__device__ __forceinline__ double ipow3d(double b) { return b * b * b; }
-__device__ __forceinline__ double ipow5d(double b) {
-    double b2 = b * b;
-    return b2 * b2 * b;
-}
-
-// Flux of the sampled face-frame state (rho, s[ND], p) through a face moving
-// with v, FP64: F = F(W + v).n with the mass flux relative to the face.
-// Same quantity as ExactRiemannSolver::solve_for_flux's per-axis sum
-// (ExactRiemannSolver.cpp:863-877), evaluated in projected form.
-template <int ND>
-__device__ __forceinline__ void state_flux_d(double gm1inv, double rho, const double* s,
-                                             double p, const double* n, const double* v,
-                                             double* F) {
-    const int NC = ND + 2, P = ND + 1;
-    if(!(rho != 0.0)) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) F[k] = 0.0;
-        return;
-    }
-    double w[ND];
-    double rel = 0.0, vn = 0.0, w2 = 0.0;
-#pragma unroll
-    for(int d = 0; d < ND; d++) {
-        w[d] = s[d] + v[d];
-        rel += s[d] * n[d];
-        vn += v[d] * n[d];
-        w2 += w[d] * w[d];
-    }
-    const double F0 = rho * rel;
-    F[0] = F0;
-#pragma unroll
-    for(int d = 0; d < ND; d++) F[1 + d] = F0 * w[d] + p * n[d];
-    const double E = 0.5 * rho * w2 + p * gm1inv;
-    F[P] = rel * E + p * (rel + vn);
-}
-
-// Exact Riemann solver, mixed precision.  WL/WR: FP32 copies for the
-// iteration; WL64/WR64/n64/v64: the same values in FP64.  F: FP64 flux.
-template <int ND>
-__device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, float* WR,
-                               const float* n, bool reflective, const double* WL64,
-                               double* WR64, const double* n64, const double* v64, double* F,
-                               unsigned& iters, unsigned& vac) {
-    const int NC = ND + 2, P = ND + 1;
-    float uL = 0.f, uR = 0.f;
-#pragma unroll
-    for(int d = 0; d < ND; d++) {
-        uL += WL[1 + d] * n[d];
-        uR += WR[1 + d] * n[d];
-    }
-    if(reflective) {
-#pragma unroll
-        for(int k = 0; k < NC; k++) {
-            WR[k] = WL[k];
-            WR64[k] = WL64[k];
-        }
-        uR = -uL;
-    }
-    double sol[NC];
-    const bool vacuum_in = !(WL64[0] != 0.0) || !(WR64[0] != 0.0);
-    const float aL = (WL[0] != 0.f) ? sqrtf_fast(c.gamma * WL[P] * rcpf(WL[0])) : 0.f;
-    const float aR = (WR[0] != 0.f) ? sqrtf_fast(c.gamma * WR[P] * rcpf(WR[0])) : 0.f;
-    // FP64 sound speeds and normal velocities: branch decisions, the final
-    // Newton step and the sampling use them
-    double uLd = 0.0, uRd = 0.0;
-#pragma unroll
-    for(int d = 0; d < ND; d++) {
-        uLd += WL64[1 + d] * n64[d];
-        uRd += WR64[1 + d] * n64[d];
-    }
-    if(reflective) uRd = -uLd;
-    const double aLd = (WL64[0] != 0.0) ? sqrt(cd.gamma * WL64[P] / WL64[0]) : 0.0;
-    const double aRd = (WR64[0] != 0.0) ? sqrt(cd.gamma * WR64[P] / WR64[0]) : 0.0;
-    if(vacuum_in || 2. * aLd / (cd.gamma - 1.) + 2. * aRd / (cd.gamma - 1.) <= uRd - uLd) {
-        // closed forms, no iteration: the FP64 reference expressions
-        vac = 1;
-        fp64::ex_solve_vacuum<ND>(cd, WL64, WR64, uLd, uRd, aLd, aRd, n64, sol);
-    } else {
-        const SideF sL = make_side(c, WL[0], WL[P], aL);
-        const SideF sR = make_side(c, WR[0], WR[P], aR);
-        const float du = uR - uL;
-        // f(0) decides between the reference's two iteration schemes
-        // (ExactRiemannSolver.cpp:143-176)
-        const double f0d = -(cd.tdgm1 * aLd) + -(cd.tdgm1 * aRd) + (uRd - uLd);
-        const bool pure_newton = f0d < cd.cutoff;
-        // initial guess, ExactRiemannSolver.cpp:644-673 (two-rarefaction /
-        // primitive-variable / two-shock)
-        const float pmin = fminf(WL[P], WR[P]), pmax = fmaxf(WL[P], WR[P]);
-        const float psum = WL[P] + WR[P];
-        float ppv = 0.5f * psum - 0.125f * du * (WL[0] + WR[0]) * (aL + aR);
-        const float pfloor = c.halftol * psum;
-        ppv = fmaxf(pfloor, ppv);
-        float p;
-        if(pmax <= 2.f * pmin && pmin <= ppv && ppv <= pmax) {
-            p = ppv;
-        } else if(ppv < pmin) {
-            float num = aL + aR - c.gm1d2 * du;
-            float den = aL * powf_fast(WL[P], -c.gm1d2g) + aR * powf_fast(WR[P], -c.gm1d2g);
-            float z = num * rcpf(den);
-            p = c.fast53 ? ipow5(z) : powf_fast(z, c.tgdgm1);
-        } else {
-            float gL = sqrtf_fast(sL.A * rcpf(ppv + sL.B));
-            float gR = sqrtf_fast(sR.A * rcpf(ppv + sR.B));
-            p = (gL * WL[P] + gR * WR[P] - du) * rcpf(gL + gR);
-        }
-        p = fmaxf(pfloor, p);
-        // Bracketed Newton in FP32.  In the reference's pure-Newton scheme a
-        // first step below zero is clamped to p = 0, where f' is infinite,
-        // and the iteration stops there with p* = 0
-        // (ExactRiemannSolver.cpp:146-154); that outcome is reproduced.
-        float lo = 0.f, hi = __builtin_inff();
-        bool stuck = false;
-        int it = 0;
-        for(; it < c.itmax; it++) {
-            float fl, dl, fr, dr;
-            side_f(c, sL, p, fl, dl);
-            side_f(c, sR, p, fr, dr);
-            const float fv = fl + fr + du;
-            if(fv < 0.f) lo = p;
-            else hi = p;
-            float pn = p - fv * rcpf(dl + dr);
-            if(it == 0 && pure_newton && fv > 0.f && pn < 0.f) {
-                stuck = true;
-                break;
-            }
-            if(!(pn >= lo && pn <= hi))
-                pn = (hi < __builtin_inff()) ? ((lo > 0.f) ? sqrtf_fast(lo) * sqrtf_fast(hi)
-                                                           : 0.01f * hi)
-                                             : 2.f * p;
-            const bool done = fabsf(pn - p) <= 2e-6f * (pn + p);
-            p = pn;
-            if(done) break;
-        }
-        iters += it + 1;
-        double ps, us;
-        if(stuck) {
-            // p* = 0: fK(0) = -2 aK/(g-1)
-            ps = 0.0;
-            us = 0.5 * (uLd + uRd) + 0.5 * (-(cd.tdgm1 * aRd) - -(cd.tdgm1 * aLd));
-        } else {
-            // FP64 final Newton step for p* (and u*)
-            const double pd = (double)p;
-            double fLd, dLd, fRd, dRd;
-            side_f_d(c, WL64[0], WL64[P], aLd, pd, fLd, dLd);
-            side_f_d(c, WR64[0], WR64[P], aRd, pd, fRd, dRd);
-            const double step = (fLd + fRd + (uRd - uLd)) / (dLd + dRd);
-            ps = pd - step;
-            if(!(ps > 0.0)) ps = pd;  // keep the FP32 root if the step would leave (0, inf)
-            const double delta = ps - pd;
-            us = 0.5 * (uLd + uRd) + 0.5 * ((fRd + dRd * delta) - (fLd + dLd * delta));
-        }
-        // sampling at x/t = 0 in FP64, ExactRiemannSolver.cpp:190-271
-        const bool right = us < 0.0;
-        const double* WK = right ? WR64 : WL64;
-        const double aK = right ? aRd : aLd;
-        const double uK = right ? uRd : uLd;
-        const double sgn = right ? 1.0 : -1.0;  // right-going (+) or left-going (-) wave
-        const double pK = WK[P];
-#pragma unroll
-        for(int k = 0; k < NC; k++) sol[k] = WK[k];
-        double vhalf = 0.0;
-        const double pdp = ps / pK;
-        if(ps > pK) {
-            const double S = uK + sgn * aK * sqrt(cd.gp1d2g * pdp + cd.gm1d2g);
-            if(sgn * S > 0.0) {
-                sol[0] = WK[0] * (pdp + cd.gm1dgp1) / (cd.gm1dgp1 * pdp + 1.);
-                sol[P] = ps;
-                vhalf = us - uK;
-            }
-        } else {
-            const double SH = uK + sgn * aK;
-            if(sgn * SH > 0.0) {
-                const double y = (pdp > 0.0) ? pow_gm1d2g_d(c, pdp) : 0.0;
-                const double ST = us + sgn * aK * y;
-                if(right ? (ST <= 0.0) : (ST > 0.0)) {
-                    const double base = cd.tdgp1 - sgn * cd.gm1dgp1 / aK * uK;
-                    sol[0] = WK[0] * (c.fast53 ? ipow3d(base) : pow(base, cd.tdgm1));
-                    sol[P] = pK * (c.fast53 ? ipow5d(base) : pow(base, cd.tgdgm1));
-                    vhalf = cd.tdgp1 * (-sgn * aK + cd.gm1d2 * uK) - uK;
-                } else {
-                    sol[0] = WK[0] * (c.fast53 ? ipow3d(y) : pow(pdp, cd.ginv));
-                    sol[P] = ps;
-                    vhalf = us - uK;
-                }
-            }
-        }
-#pragma unroll
-        for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n64[d];
-    }
-    state_flux_d<ND>(1.0 / (cd.gamma - 1.0), sol[0], sol + 1, sol[P], n64, v64, F);
-    bool bad = (sol[0] < 0.0) | (sol[0] != sol[0]) | (sol[P] < 0.0) | (sol[P] != sol[P]);
-#pragma unroll
-    for(int d = 0; d < ND; d++) bad |= (sol[1 + d] != sol[1 + d]);
-    return bad ? ST_SOLVER : 0;
-}
 
 template <int ND, int KIND, typename T>
 __global__ __launch_bounds__(256) void solve_for_flux_kernel(
@@ -373,38 +47,8 @@ __global__ __launch_bounds__(256) void solve_for_flux_kernel(
         if(status) status[i] = st;
         fail = st != 0;
     }
-    if(ctr) {
-        wave_add(ctr + CNT_NEVAL, valid);
-        wave_add(ctr + CNT_NVACUUM, vac);
-        wave_add(ctr + CNT_NITER, iters);
-        wave_add(ctr + CNT_NFAIL, fail);
-    }
-}
-
-static CF make_cf(const Consts& c) {
-    CF f;
-    f.gamma = (float)c.gamma;
-    f.gp1d2g = (float)c.gp1d2g;
-    f.gm1d2g = (float)c.gm1d2g;
-    f.gm1dgp1 = (float)c.gm1dgp1;
-    f.tdgp1 = (float)c.tdgp1;
-    f.tdgm1 = (float)c.tdgm1;
-    f.gm1d2 = (float)c.gm1d2;
-    f.tgdgm1 = (float)c.tgdgm1;
-    f.ginv = (float)c.ginv;
-    f.ginvm1 = (float)(c.ginv - 1.0);
-    f.gm1inv = (float)(1.0 / (c.gamma - 1.0));
-    f.q_fac = (float)(0.5 * (c.gamma + 1.0) / c.gamma);
-    f.gamma_d = c.gamma;
-    f.tdgp1_d = c.tdgp1;
-    f.tdgm1_d = c.tdgm1;
-    f.gm1dgp1_d = c.gm1dgp1;
-    f.gm1d2g_d = c.gm1d2g;
-    f.cutoff_d = c.cutoff;
-    f.halftol = (float)(0.5 * c.tol);
-    f.fast53 = c.fast53;
-    f.itmax = 24;
-    return f;
+    const unsigned cv[CNT_N] = {valid, 0u, vac, iters, fail};
+    block_add_counters(ctr, cv);
 }
 
 }  // namespace mixed

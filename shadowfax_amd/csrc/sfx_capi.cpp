@@ -101,12 +101,12 @@ int sfx_solver_create(int kind, int ndim, double gamma, double tolerance, double
         delete h;
         return hip_error(e, "hipGetDevice");
     }
-    e = hipMalloc(&h->d_ctr, sizeof(unsigned long long) * CNT_N);
+    e = hipMalloc(&h->d_ctr, sizeof(unsigned long long) * CNT_N * kCtrSlots);
     if(e != hipSuccess) {
         delete h;
         return hip_error(e, "hipMalloc(counters)");
     }
-    e = hipMemset(h->d_ctr, 0, sizeof(unsigned long long) * CNT_N);
+    e = hipMemset(h->d_ctr, 0, sizeof(unsigned long long) * CNT_N * kCtrSlots);
     if(e != hipSuccess) {
         hipFree(h->d_ctr);
         delete h;
@@ -152,9 +152,12 @@ int sfx_solver_get_stats(sfx_solver* h, sfx_stats* out) {
     if(rc) return rc;
     hipError_t e = hipDeviceSynchronize();
     if(e != hipSuccess) return hip_error(e, "hipDeviceSynchronize");
-    unsigned long long v[CNT_N];
-    e = hipMemcpy(v, h->d_ctr, sizeof v, hipMemcpyDeviceToHost);
+    unsigned long long slots[CNT_N * kCtrSlots];
+    e = hipMemcpy(slots, h->d_ctr, sizeof slots, hipMemcpyDeviceToHost);
     if(e != hipSuccess) return hip_error(e, "hipMemcpy(counters)");
+    unsigned long long v[CNT_N] = {0, 0, 0, 0, 0};
+    for(int j = 0; j < kCtrSlots; j++)
+        for(int k = 0; k < CNT_N; k++) v[k] += slots[j * CNT_N + k];
     out->neval = v[CNT_NEVAL];
     out->nbrent = v[CNT_NBRENT];
     out->nvacuum = v[CNT_NVACUUM];
@@ -168,7 +171,7 @@ int sfx_solver_reset_stats(sfx_solver* h) {
     int rc = bind_device(h);
     if(rc) return rc;
     hipError_t e = hipDeviceSynchronize();
-    if(e == hipSuccess) e = hipMemset(h->d_ctr, 0, sizeof(unsigned long long) * CNT_N);
+    if(e == hipSuccess) e = hipMemset(h->d_ctr, 0, sizeof(unsigned long long) * CNT_N * kCtrSlots);
     if(e != hipSuccess) return hip_error(e, "reset_stats");
     return SFX_OK;
 }

@@ -28,8 +28,12 @@ enum : uint8_t {
     ST_UPDATEQ = 8,  // negative mass or energy after update (UpdateQException)
 };
 
-// counters[] slots accumulated by the batch kernels
+// counters accumulated by the batch kernels; the device array holds
+// kCtrSlots copies (kCtrSlots x CNT_N) that blocks add into round-robin, so
+// atomics from thousands of blocks do not queue on one address; readers sum
+// the copies.
 enum { CNT_NEVAL = 0, CNT_NBRENT, CNT_NVACUUM, CNT_NITER, CNT_NFAIL, CNT_N };
+constexpr int kCtrSlots = 256;
 
 // std::max / std::min as the reference's StateVector uses them
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
@@ -37,12 +41,26 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ float smaxf(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float sminf(float a, float b) { return (b < a) ? b : a; }
 
-// Sum of a per-lane counter over the wavefront, added once per wave.
-__device__ __forceinline__ void wave_add(unsigned long long* ctr, unsigned v) {
+// Adds per-lane counters v[0..CNT_N) over the whole block: wave sums by
+// shuffles, block sums in LDS, then one global atomic per counter into slot
+// blockIdx.x % kCtrSlots.  Every thread of the block must call it (it
+// synchronizes the block); ctr == nullptr is a uniform no-op.
+__device__ __forceinline__ void block_add_counters(unsigned long long* ctr,
+                                                   const unsigned (&v)[CNT_N]) {
     if(!ctr) return;
+    __shared__ unsigned long long s_ctr[CNT_N];
+    if(threadIdx.x < CNT_N) s_ctr[threadIdx.x] = 0;
+    __syncthreads();
 #pragma unroll
-    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+    for(int k = 0; k < CNT_N; k++) {
+        unsigned x = v[k];
+#pragma unroll
+        for(int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if((threadIdx.x & 63) == 0 && x) atomicAdd(&s_ctr[k], (unsigned long long)x);
+    }
+    __syncthreads();
+    if(threadIdx.x < CNT_N && s_ctr[threadIdx.x])
+        atomicAdd(ctr + (blockIdx.x % kCtrSlots) * CNT_N + threadIdx.x, s_ctr[threadIdx.x]);
 }
 
 }  // namespace sfx

@@ -27,4 +27,26 @@ hipError_t launch_solve_for_flux_mixed(int kind, int ndim, const Consts& c, int6
                                        const uint8_t* refl, T* F, uint8_t* status,
                                        unsigned long long* ctr, hipStream_t stream);
 
+// flux pass (flux_pass.hip: T = double; flux_pass_mixed.hip: T = float)
+namespace flux {
+template <typename T>
+struct MeshView;
+}
+template <typename T>
+hipError_t launch_gradients(int ndim, const flux::MeshView<T>& m, T* grad, hipStream_t s);
+template <typename T>
+hipError_t launch_face_fluxes(int ndim, int kind, const flux::MeshView<T>& m, const T* grad,
+                              const Consts& c, double* dQL, uint8_t* fstatus,
+                              unsigned long long* ctr, hipStream_t s);
+template <typename T>
+hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
+                         const double* volume, T* W, uint8_t* cstatus, unsigned long long* ctr,
+                         hipStream_t s);
+hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* goff,
+                              const int32_t* gent, const double* dQL, double* dQ,
+                              hipStream_t s);
+hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* goff,
+                               const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
+                               const double* dQL, double* dQ, hipStream_t s);
+
 }  // namespace sfx

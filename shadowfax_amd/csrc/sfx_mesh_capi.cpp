@@ -1,0 +1,290 @@
+// sfx_mesh_capi.cpp -- C ABI for meshes (host tessellation, per-face
+// geometry, gather index) and for the face-flux pass kernels.
+//
+// Host arithmetic here must reproduce the reference's FP64 expressions bit
+// for bit; this file is compiled with -ffp-contract=off (no FMA), which the
+// x86-64 baseline has no instruction for anyway.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "flux_pass_impl.h"
+#include "sfx_capi_internal.h"
+#include "sfx_launch.h"
+#include "sfx_mesh_host.h"
+#include "shadowfax_amd.h"
+
+using namespace sfx;
+
+namespace {
+
+template <typename V>
+typename V::value_type* dup(const V& v) {
+    using T = typename V::value_type;
+    T* p = (T*)std::malloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if(p && !v.empty()) std::memcpy(p, v.data(), sizeof(T) * v.size());
+    return p;
+}
+
+template <typename T>
+flux::MeshView<T> view(const sfx_mesh* m) {
+    flux::MeshView<T> v;
+    v.C = m->C;
+    v.F = m->F;
+    v.W = (const T*)m->W;
+    v.vel = (const T*)m->vel;
+    v.mass = m->mass;
+    v.total_area = m->total_area;
+    v.dt = m->dt;
+    v.volume = m->volume;
+    v.active = m->active;
+    v.left = m->left;
+    v.right = m->right;
+    v.ghost = m->ghost;
+    v.twin = m->twin;
+    v.area = (const T*)m->area;
+    v.dL = (const T*)m->dL;
+    v.dR = (const T*)m->dR;
+    v.rRL = (const T*)m->rRL;
+    v.mc = (const T*)m->mc;
+    v.rcent = (const T*)m->rcent;
+    v.cf_off = m->cf_off;
+    v.cf_entry = m->cf_entry;
+    v.gather_off = m->gather_off;
+    v.gather_entry = m->gather_entry;
+    return v;
+}
+
+int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool faces,
+               bool gradients, bool gather) {
+    if(!h || !m) return set_error(SFX_ERR_ARG, "%s: NULL handle or mesh", who);
+    if(m->ndim != h->ndim)
+        return set_error(SFX_ERR_ARG, "%s: mesh ndim %d != solver ndim %d", who, m->ndim,
+                         h->ndim);
+    if(m->C < 0 || m->F < 0) return set_error(SFX_ERR_ARG, "%s: negative size", who);
+    if(m->C > INT32_MAX || m->F > (INT32_MAX >> 1))
+        return set_error(SFX_ERR_ARG, "%s: mesh too large for 32-bit indices", who);
+    if(gradients && (!m->W || !m->area || !m->rRL || !m->mc || !m->dL || !m->dR ||
+                     !m->volume || !m->cf_off || !m->cf_entry || !m->left || !m->right))
+        return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass", who);
+    if(faces && (!m->W || !m->vel || !m->mass || !m->total_area || !m->dt || !m->left ||
+                 !m->right || !m->ghost || !m->area || !m->dL || !m->dR || !m->rRL ||
+                 !m->mc || !m->rcent))
+        return set_error(SFX_ERR_ARG, "%s: missing array for the face pass", who);
+    if(gather && (!m->gather_off || !m->gather_entry))
+        return set_error(SFX_ERR_ARG, "%s: missing gather index", who);
+    if(gather && h->precision == SFX_PREC_MIXED && m->twin && !m->ghost)
+        return set_error(SFX_ERR_ARG, "%s: twin given without ghost flags", who);
+    return SFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfx_voronoi_periodic(int ndim, int64_t C, const double* pos, const double* box,
+                         int nthreads, sfx_host_mesh* out) {
+    if(!pos || !box || !out) return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: NULL");
+    if(C <= 0) return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: C must be > 0");
+    std::memset(out, 0, sizeof *out);
+    HostMesh m;
+    int rc;
+    if(ndim == 2)
+        rc = voronoi2d_periodic(C, pos, box[0], box[1], m, nthreads);
+    else if(ndim == 3)
+        rc = voronoi3d_periodic(C, pos, box[0], box[1], box[2], m, nthreads);
+    else
+        return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: ndim must be 2 or 3");
+    if(rc == -4) return set_error(SFX_ERR_UNSUPPORTED, "3D tessellation not available");
+    if(rc != 0)
+        return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: tessellation failed (%d)", rc);
+    out->ndim = ndim;
+    out->C = m.C;
+    out->F = m.F;
+    out->pos = dup(m.pos);
+    out->volume = dup(m.volume);
+    out->centroid = dup(m.centroid);
+    out->left = dup(m.left);
+    out->right = dup(m.right);
+    out->area = dup(m.area);
+    out->midpoint = dup(m.midpoint);
+    out->shift = dup(m.shift);
+    out->ghost = dup(m.ghost);
+    out->twin = dup(m.twin);
+    out->cf_off = dup(m.cf_off);
+    out->cf_face = dup(m.cf_face);
+    if(!out->pos || !out->left || !out->cf_face) {
+        sfx_host_mesh_free(out);
+        return set_error(SFX_ERR_NOMEM, "sfx_voronoi_periodic: out of host memory");
+    }
+    return SFX_OK;
+}
+
+void sfx_host_mesh_free(sfx_host_mesh* m) {
+    if(!m) return;
+    void* ps[] = {m->pos,      m->volume, m->centroid, m->left,  m->right,
+                  m->area,     m->midpoint, m->shift,  m->ghost, m->twin,
+                  m->cf_off,   m->cf_face};
+    for(void* p : ps) std::free(p);
+    std::memset(m, 0, sizeof *m);
+}
+
+int sfx_face_geometry(int ndim, int64_t C, int64_t F, const double* pos, const double* centroid,
+                      const int32_t* left, const int32_t* right, const double* midpoint,
+                      const double* shift, double* dL, double* dR, double* rRL, double* mc,
+                      double* rcent) {
+    if(ndim != 2 && ndim != 3) return set_error(SFX_ERR_ARG, "sfx_face_geometry: ndim");
+    if(!pos || !centroid || !left || !right || !midpoint || !shift || !dL || !dR || !rRL ||
+       !mc || !rcent)
+        return set_error(SFX_ERR_ARG, "sfx_face_geometry: NULL array");
+    for(int64_t f = 0; f < F; f++) {
+        const int64_t L = left[f], R = right[f];
+        if(L < 0 || L >= C || R >= C)
+            return set_error(SFX_ERR_ARG, "sfx_face_geometry: face %lld index out of range",
+                             (long long)f);
+        const int64_t base = (R >= 0) ? R : L;
+        double xl[3], xr[3];
+        for(int d = 0; d < ndim; d++) {
+            xl[d] = pos[d * C + L];
+            xr[d] = pos[d * C + base] + shift[d * F + f];
+        }
+        // VorFace::calculate_flux, src/VorFace.cpp:731-734 (right) and
+        // :777-780 (left): d = midpoint - centroid + (cell pos - generator pos)
+        for(int n = ndim; n--;) {
+            dL[n * F + f] = midpoint[n * F + f] - centroid[n * C + L] + (xl[n] - xl[n]);
+            if(R >= 0)
+                dR[n * F + f] = midpoint[n * F + f] - centroid[n * C + R] + (pos[n * C + R] - xr[n]);
+            else
+                dR[n * F + f] = 0.0;
+            rRL[n * F + f] = xr[n] - xl[n];
+            mc[n * F + f] = midpoint[n * F + f] - 0.5 * (xl[n] + xr[n]);
+        }
+        if(R >= 0) {
+            double cd[3];
+            for(int d = 0; d < ndim; d++) cd[d] = centroid[d * C + L] - centroid[d * C + R];
+            rcent[f] = (ndim == 3) ? std::sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2])
+                                   : std::sqrt(cd[0] * cd[0] + cd[1] * cd[1]);
+        } else {
+            rcent[f] = 0.0;
+        }
+    }
+    return SFX_OK;
+}
+
+int sfx_mesh_cell_entries(int64_t C, int64_t F, const int32_t* left, const int64_t* cf_off,
+                          const int32_t* cf_face, int32_t* cf_entry) {
+    if(!left || !cf_off || !cf_face || !cf_entry)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_cell_entries: NULL array");
+    if(F > (INT32_MAX >> 1)) return set_error(SFX_ERR_ARG, "sfx_mesh_cell_entries: F too large");
+    for(int64_t i = 0; i < C; i++)
+        for(int64_t j = cf_off[i]; j < cf_off[i + 1]; j++) {
+            const int32_t f = cf_face[j];
+            if(f < 0 || f >= F) return set_error(SFX_ERR_ARG, "sfx_mesh_cell_entries: face");
+            cf_entry[j] = 2 * f + (left[f] == i ? 0 : 1);
+        }
+    return SFX_OK;
+}
+
+int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left, const int32_t* right,
+                          const uint8_t* ghost, int64_t* gather_off, int32_t* gather_entry) {
+    if(!left || !right || !gather_off)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_gather_index: NULL array");
+    std::vector<int64_t> cnt(C + 1, 0);
+    for(int64_t f = 0; f < F; f++) {
+        if(left[f] < 0 || left[f] >= C || right[f] >= C)
+            return set_error(SFX_ERR_ARG, "sfx_mesh_gather_index: face %lld index out of range",
+                             (long long)f);
+        cnt[left[f] + 1]++;
+        if(right[f] >= 0 && !(ghost && ghost[f])) cnt[right[f] + 1]++;
+    }
+    for(int64_t i = 0; i < C; i++) cnt[i + 1] += cnt[i];
+    std::memcpy(gather_off, cnt.data(), sizeof(int64_t) * (C + 1));
+    if(!gather_entry) return SFX_OK;
+    // faces in ascending order: every cell's list comes out sorted
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for(int64_t f = 0; f < F; f++) {
+        gather_entry[fill[left[f]]++] = (int32_t)(2 * f);
+        if(right[f] >= 0 && !(ghost && ghost[f])) gather_entry[fill[right[f]]++] = (int32_t)(2 * f + 1);
+    }
+    return SFX_OK;
+}
+
+int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream) {
+    int rc = check_mesh(h, m, "sfx_gradients", false, true, false);
+    if(rc) return rc;
+    if(!grad) return set_error(SFX_ERR_ARG, "sfx_gradients: grad is NULL");
+    if((rc = bind_device(h))) return rc;
+    hipError_t e = (h->precision == SFX_PREC_FP64)
+                           ? launch_gradients<double>(h->ndim, view<double>(m), (double*)grad,
+                                                      (hipStream_t)stream)
+                           : launch_gradients<float>(h->ndim, view<float>(m), (float*)grad,
+                                                     (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "gradient kernel launch");
+    return SFX_OK;
+}
+
+int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad, double* dQL,
+                    uint8_t* face_status, void* stream) {
+    int rc = check_mesh(h, m, "sfx_face_fluxes", true, false, false);
+    if(rc) return rc;
+    if(!grad || !dQL) return set_error(SFX_ERR_ARG, "sfx_face_fluxes: NULL grad or dQL");
+    if((rc = bind_device(h))) return rc;
+    unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
+    hipError_t e = (h->precision == SFX_PREC_FP64)
+                           ? launch_face_fluxes<double>(h->ndim, h->kind, view<double>(m),
+                                                        (const double*)grad, h->c, dQL,
+                                                        face_status, ctr, (hipStream_t)stream)
+                           : launch_face_fluxes<float>(h->ndim, h->kind, view<float>(m),
+                                                       (const float*)grad, h->c, dQL,
+                                                       face_status, ctr, (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "face flux kernel launch");
+    return SFX_OK;
+}
+
+int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL, double* dQ,
+                  void* stream) {
+    int rc = check_mesh(h, m, "sfx_gather_dQ", false, false, true);
+    if(rc) return rc;
+    if(!dQL || !dQ) return set_error(SFX_ERR_ARG, "sfx_gather_dQ: NULL dQL or dQ");
+    if((rc = bind_device(h))) return rc;
+    hipError_t e =
+            (h->precision == SFX_PREC_FP64)
+                    ? launch_gather_fp64(h->ndim, m->C, m->F, m->gather_off, m->gather_entry, dQL,
+                                         dQ, (hipStream_t)stream)
+                    : launch_gather_mixed(h->ndim, m->C, m->F, m->gather_off, m->gather_entry,
+                                          m->ghost, m->twin, dQL, dQ, (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "gather kernel launch");
+    return SFX_OK;
+}
+
+int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, double* dQL, double* dQ,
+                       uint8_t* face_status, void* stream) {
+    int rc = check_mesh(h, m, "sfx_face_flux_pass", true, true, true);
+    if(rc) return rc;
+    if((rc = sfx_gradients(h, m, grad, stream))) return rc;
+    if((rc = sfx_face_fluxes(h, m, grad, dQL, face_status, stream))) return rc;
+    return sfx_gather_dQ(h, m, dQL, dQ, stream);
+}
+
+int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const double* volume,
+                    void* W, uint8_t* cell_status, void* stream) {
+    if(!h) return set_error(SFX_ERR_ARG, "sfx_update_step: NULL handle");
+    if(C < 0) return set_error(SFX_ERR_ARG, "sfx_update_step: C < 0");
+    if(C == 0) return SFX_OK;
+    if(!Q || !dQ || !volume || !W) return set_error(SFX_ERR_ARG, "sfx_update_step: NULL array");
+    int rc = bind_device(h);
+    if(rc) return rc;
+    unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
+    hipError_t e = (h->precision == SFX_PREC_FP64)
+                           ? launch_update<double>(h->ndim, h->c, C, Q, dQ, volume, (double*)W,
+                                                   cell_status, ctr, (hipStream_t)stream)
+                           : launch_update<float>(h->ndim, h->c, C, Q, dQ, volume, (float*)W,
+                                                  cell_status, ctr, (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "update kernel launch");
+    return SFX_OK;
+}
+
+}  // extern "C"

@@ -81,6 +81,52 @@ def riemann_set(ndim, gamma):
     return out
 
 
+MESH_KEYS = ("pos", "volume", "centroid", "left", "right", "area", "midpoint", "shift",
+             "ghost", "twin", "cf_off", "cf_face", "W", "vel")
+
+
+def mirrorize(m):
+    """Turn the periodic faces that cross x = 0 / x = 1 into reflective
+    mirror faces of their left cell (mirror generator reflected through the
+    wall), so the mirror code paths are exercised."""
+    m = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in m.items()}
+    pos = m["pos"]
+    sel = (m["ghost"] == 1) & (m["shift"][0] != 0)
+    xl = pos[:, m["left"][sel]]
+    mir = xl.copy()
+    mir[0] = np.where(m["shift"][0][sel] > 0, 2.0 - xl[0], -xl[0])
+    m["right"][sel] = -1
+    m["ghost"][sel] = 0
+    m["twin"][sel] = -1
+    m["shift"][:, sel] = mir - xl
+    return m
+
+
+def mesh_set():
+    """Small 2D meshes with KH-like states plus noise; reference gradients
+    (VorCell::estimate_gradient), face velocities (VorFace::set_v) and normals
+    (VorFace::get_normal) from the compiled reference."""
+    from shadowfax_amd import mesh as M
+    out = {}
+    rng = np.random.default_rng(42)
+    for name, n, mirror in (("per12", 12, False), ("mir10", 10, True)):
+        m = M.voronoi_periodic(M.jittered_grid(n, 0.25, seed=11 + n))
+        if mirror:
+            m = mirrorize(m)
+        W = M.kh_ics(m)
+        W[0] *= 1.0 + 0.2 * rng.random(m["C"])
+        W[3] *= 1.0 + 0.3 * rng.random(m["C"])
+        m["W"] = W
+        m["vel"] = W[1:3] + 0.05 * rng.normal(size=(2, m["C"]))
+        g, vf, nr = O.ref_mesh_gradients(m)
+        for k in MESH_KEYS:
+            out[name + "_" + k] = m[k]
+        out[name + "_ref_grad"] = g
+        out[name + "_ref_vface"] = vf
+        out[name + "_ref_normal"] = nr
+    return out
+
+
 def main():
     if not (O.ref_available(2) and O.ref_available(3)):
         raise SystemExit("oracle/_ref not built: make -C oracle ref")
@@ -94,6 +140,9 @@ def main():
         path = os.path.join(HERE, "riemann_%dd.npz" % ndim)
         np.savez_compressed(path, **data)
         print(path, os.path.getsize(path))
+    path = os.path.join(HERE, "mesh_2d.npz")
+    np.savez_compressed(path, **mesh_set())
+    print(path, os.path.getsize(path))
 
 
 if __name__ == "__main__":

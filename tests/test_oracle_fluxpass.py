@@ -1,0 +1,201 @@
+"""CPU tests pinning the oracle's flux pass.
+
+  * gradients (VorCell::estimate_gradient), face velocities (VorFace::set_v)
+    and normals (VorFace::get_normal): golden vectors from the reference
+    classes compiled from source, on periodic and mirror-face meshes;
+    bit-exact;
+  * VorFace::limit: hand-computed values of the GIZMO pairwise limiter
+    (Hopkins 2015, appendix B: psi1 = 0.5, psi2 = 0.25);
+  * VorFace::calculate_flux (not runnable from the reference here: it needs
+    a TimeLine, whose constructors need Boost): invariants -- zero gradients
+    reduce each face to solve_for_flux on the boosted states (bit-exact),
+    uniform comoving flow gives zero net change, the periodic pass conserves
+    to round-off, the mass-flux limiter caps the outflow at the area share of
+    the cell's mass, the scatter is the ordered per-cell sum of the face
+    fluxes, and the update step is update_Q + get_W.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from shadowfax_amd import mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = 5.0 / 3.0
+KEYS = ("pos", "volume", "centroid", "left", "right", "area", "midpoint", "shift", "ghost",
+        "twin", "cf_off", "cf_face", "W", "vel")
+
+
+def golden_mesh(name):
+    z = np.load(os.path.join(HERE, "golden", "mesh_2d.npz"))
+    m = {k: z[name + "_" + k] for k in KEYS}
+    m["ndim"] = 2
+    m["C"] = m["volume"].shape[0]
+    m["F"] = m["area"].shape[0]
+    m["box"] = np.ones(2)
+    ref = {k: z[name + "_ref_" + k] for k in ("grad", "vface", "normal")}
+    return m, ref
+
+
+def bundle(m, kind=1, dt=None):
+    return M.prepare(m, m["W"], G, kind=kind, vel=m["vel"], dt=dt)
+
+
+@pytest.mark.parametrize("name", ["per12", "mir10"])
+def test_gradients_match_reference_bitwise(name):
+    m, ref = golden_mesh(name)
+    assert np.array_equal(O.gradients(1, m), ref["grad"])
+
+
+@pytest.mark.parametrize("name", ["per12", "mir10"])
+def test_face_velocity_and_normal_match_reference_bitwise(name):
+    m, ref = golden_mesh(name)
+    v, n = O.face_velocities(m)
+    assert np.array_equal(v, ref["vface"])
+    assert np.array_equal(n, ref["normal"])
+
+
+# (phimid0, phiL, phiR, |d|/r) -> limited value, worked by hand from
+# phibar = phiL + |d|/r (phiR - phiL), delta1 = |phiL - phiR|/2,
+# delta2 = |phiL - phiR|/4, phi+- = phimax/min +- delta1 unless that flips
+# the sign (then phi/(1 + delta1/|phi|)).
+LIMITER_CASES = [
+    (3.0, 1.0, 2.0, 0.5, 1.75),      # capped at phibar + delta2 = 1.5 + 0.25
+    (1.2, 1.0, 2.0, 0.5, 1.2),       # inside: unchanged
+    (0.1, 1.0, 2.0, 0.5, 0.5),       # floored at phi- = 1 - 0.5
+    (3.0, 2.0, 1.0, 0.5, 2.5),       # capped at phi+ = 2 + 0.5
+    (0.0, 2.0, 1.0, 0.5, 1.25),      # floored at phibar - delta2 = 1.5 - 0.25
+    (0.5, -0.1, -0.5, 0.5, -0.1 / 3.0),  # phi+ = -0.1/(1 + 0.2/0.1)
+    (7.0, 4.0, 4.0, 0.5, 4.0),       # equal neighbours: phiL
+]
+
+
+@pytest.mark.parametrize("case", LIMITER_CASES)
+def test_limiter_hand_computed(case):
+    phimid0, phiL, phiR, ratio, want = case
+    d = np.array([0.3, 0.4])  # |d| = 0.5
+    got = O.limit(phimid0, phiL, phiR, d, 0.5 / ratio, 2)
+    assert abs(got - want) <= 1e-15 * max(1.0, abs(want))
+
+
+def test_zero_gradients_reduce_to_solve_for_flux():
+    """With grad = 0 the predictor and limiter leave the boosted states
+    unchanged, so dQL = (dt A) solve_for_flux(WL - v, WR - v, n, v)."""
+    m, _ = golden_mesh("mir10")
+    for kind in (O.EXACT, O.HLLC):
+        b = bundle(m, kind)
+        b["grad"] = np.zeros((2, 4, m["C"]))
+        dQ, dQL, st, _ = O.face_flux_pass(kind, b)
+        v, n = O.face_velocities(b)
+        L, R = b["left"], b["right"]
+        WL = b["W"][:, L].copy()
+        WR = np.where(R >= 0, b["W"][:, np.maximum(R, 0)], 0.0)
+        WL[1:3] -= v
+        WR[1:3] -= v
+        F, sts, _ = O.solve_for_flux(kind, 2, WL, WR, n, v=v, refl=(R < 0).astype(np.uint8))
+        dt = np.minimum(b["dt"][L], np.where(R >= 0, b["dt"][np.maximum(R, 0)], np.inf))
+        want = (dt * b["area"]) * F
+        live = b["area"] != 0
+        assert np.array_equal(dQL[:, live], want[:, live])
+
+
+def test_uniform_comoving_flow_gives_no_net_change():
+    m, _ = golden_mesh("per12")
+    W = np.zeros((4, m["C"]))
+    W[0], W[1], W[2], W[3] = 1.3, 0.4, -0.2, 2.0
+    m = dict(m)
+    m["W"], m["vel"] = W, W[1:3].copy()
+    for kind in (O.EXACT, O.HLLC):
+        b = bundle(m, kind)
+        b["grad"] = O.gradients(kind, b)
+        assert np.abs(b["grad"]).max() == 0.0
+        dQ, dQL, st, _ = O.face_flux_pass(kind, b)
+        assert np.abs(dQ).max() <= 1e-13 * np.abs(dQL).max()
+
+
+@pytest.mark.parametrize("kind", [O.EXACT, O.HLLC])
+def test_periodic_pass_conserves(kind):
+    m = M.voronoi_periodic(M.jittered_grid(20, 0.25, seed=2))
+    b = M.prepare(m, M.kh_ics(m), G, kind=kind)
+    b["grad"] = O.gradients(kind, b)
+    dQ, dQL, st, _ = O.face_flux_pass(kind, b)
+    assert st.max() == 0
+    # periodic faces are evaluated once from each side: the two copies agree
+    # to round-off, so the totals vanish to round-off
+    assert np.abs(dQ.sum(1)).max() <= 1e-13 * np.abs(dQ).max()
+    tw = b["twin"]
+    g = np.where((b["ghost"] == 1) & (tw >= 0))[0]
+    assert np.allclose(dQL[:, g], -dQL[:, tw[g]], rtol=1e-10, atol=1e-16)
+
+
+def test_scatter_is_ordered_sum_of_face_fluxes():
+    m, _ = golden_mesh("per12")
+    b = bundle(m, 1)
+    b["grad"] = O.gradients(1, b)
+    dQ, dQL, st, _ = O.face_flux_pass(1, b)
+    acc = np.zeros_like(dQ)
+    for f in range(b["F"]):  # the reference's forward face loop
+        acc[:, b["left"][f]] += dQL[:, f]
+        if b["right"][f] >= 0 and not b["ghost"][f]:
+            acc[:, b["right"][f]] += -dQL[:, f]
+    assert np.array_equal(acc, dQ)
+    # the gather index lists exactly these contributions, in face order
+    off, ent = b["gather_off"], b["gather_entry"]
+    acc2 = np.zeros_like(dQ)
+    for i in range(b["C"]):
+        for e in ent[off[i]:off[i + 1]]:
+            x = dQL[:, e >> 1]
+            acc2[:, i] += -x if (e & 1) else x
+    assert np.array_equal(acc2, dQ)
+
+
+def test_mass_flux_limiter_caps_outflow():
+    m, _ = golden_mesh("per12")
+    b = bundle(m, 1, dt=50.0)  # absurd time step: limiter must act
+    b["grad"] = np.zeros((2, 4, m["C"]))
+    dQ, dQL, st, _ = O.face_flux_pass(1, b)
+    L, R = b["left"], b["right"]
+    capL = b["area"] / b["total_area"][L] * b["mass"][L]
+    capR = b["area"] / b["total_area"][np.maximum(R, 0)] * b["mass"][np.maximum(R, 0)]
+    assert np.all(dQL[0] <= capL * (1 + 1e-14))
+    real = R >= 0
+    assert np.all(-dQL[0][real] <= capR[real] * (1 + 1e-14))
+    assert (np.abs(dQL[0] - capL) <= 1e-14 * capL).sum() > 10
+
+
+def test_update_is_update_Q_then_get_W():
+    m, _ = golden_mesh("per12")
+    b = bundle(m, 1)
+    b["grad"] = O.gradients(1, b)
+    dQ, dQL, st, _ = O.face_flux_pass(1, b)
+    Q, W, cst = O.update(1, 2, b["Q"], dQ, b["volume"])
+    Wr, Qr = O.get_W(1, 2, b["volume"], b["Q"] - dQ)
+    assert np.array_equal(W, Wr) and np.array_equal(Q, Qr)
+    assert cst.max() == 0
+
+
+def test_linear_field_gradient_is_exact():
+    """The Voronoi gradient estimate is exact for linear fields; the slope
+    limiter leaves it unchanged away from extrema."""
+    m = M.voronoi_periodic(M.jittered_grid(24, 0.25, seed=4))
+    W = np.zeros((4, m["C"]))
+    x, y = m["pos"]
+    W[0] = 2.0 + 0.3 * x - 0.2 * y
+    W[3] = 3.0 + 0.1 * x + 0.4 * y
+    W[1] = 0.5 * x
+    W[2] = 0.25 * y
+    mm = dict(m)
+    mm["W"], mm["vel"] = W, np.zeros((2, m["C"]))
+    g = O.gradients(0, mm)
+    # periodic images break linearity at the box edge: interior cells only
+    inner = (x > 0.15) & (x < 0.85) & (y > 0.15) & (y < 0.85)
+    want = {(0, 0): 0.3, (1, 0): -0.2, (0, 3): 0.1, (1, 3): 0.4, (0, 1): 0.5, (1, 2): 0.25}
+    for (d, k), val in want.items():
+        gi = g[d, k, inner]
+        exact = np.isclose(gi, val, rtol=1e-9, atol=1e-12)
+        assert exact.mean() > 0.98
+        # where the limiter acts it only scales the slope by alpha in [0, 1]
+        a = gi[~exact] / val
+        assert np.all((a >= 0.0) & (a <= 1.0 + 1e-9))
