@@ -54,23 +54,21 @@ def time_variant(fn, steps, warmup, stream):
             "ms_p50": float(np.median(per)), "wall_s": wall}
 
 
-def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
-    """Config 2: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC,
-    gradients + face fluxes + ordered gather (one full pass per step)."""
+def bench_fluxpass(b, kind, steps, warmup, dev, alg_bytes):
+    """One full face-flux pass per step (gradients + face fluxes + ordered
+    gather) over a prepared mesh, in the FP64 and the mixed path; per-kernel
+    times, mixed-vs-FP64 errors and conservation."""
     import torch
-    from shadowfax_amd import mesh as M
     from shadowfax_amd.fluxpass import FluxPass
+    from shadowfax_amd.metrics import fluxpass_errors
+    from shadowfax_amd import mesh as M
     from shadowfax_amd.riemann import RiemannSolver
-    t0 = time.perf_counter()
-    m = M.voronoi_periodic(M.jittered_grid(n, 0.25, seed=20240603))
-    b = M.prepare(m, M.kh_ics(m), 5.0 / 3.0, kind=kind)
-    t_mesh = time.perf_counter() - t0
-    C, F = b["C"], b["F"]
+    C, F, nd = b["C"], b["F"], b["ndim"]
     stream = torch.cuda.current_stream()
-    out = {"C": C, "F": F, "host_mesh_s": t_mesh, "solver": "hllc" if kind else "exact"}
+    out = {"C": C, "F": F, "solver": "hllc" if kind else "exact"}
     res = {}
     for prec in ("fp64", "mixed"):
-        s = RiemannSolver(kind, 2, 5.0 / 3.0, 1e-8, -5.0, prec)
+        s = RiemannSolver(kind, nd, 5.0 / 3.0, 1e-8, -5.0, prec)
         s.enable_stats(False)
         fp = FluxPass(s, M.DeviceMesh(b, prec, dev))
         parts = {"gradients": fp.gradients, "faces": lambda fp=fp: fp.face_fluxes(status=False),
@@ -81,20 +79,90 @@ def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
         r["face_fluxes_per_s"] = F / (r["ms_mean"] * 1e-3)
         fp.run()
         torch.cuda.synchronize()
-        res[prec] = (fp.m.dQ.cpu().numpy(), fp.m.dQL.cpu().numpy())
+        res[prec] = (fp.m.dQL_soa(), fp.m.dQ_soa(),
+                     int((fp.m.fstatus != 0).sum().item()))
+        r["failed_faces"] = res[prec][2]
         out[prec] = r
-    dQ64, dQL64 = res["fp64"]
-    dQmx = res["mixed"][0]
+        del fp
+        torch.cuda.empty_cache()
     out["mixed_speedup"] = out["fp64"]["ms_mean"] / out["mixed"]["ms_mean"]
-    out["mixed_vs_fp64_rel_maxdQ"] = float(np.abs(dQmx - dQ64).max() / np.abs(dQ64).max())
-    out["mixed_conservation_residual"] = float(np.abs(dQmx.sum(1)).max() / np.abs(dQ64).max())
+    out["mixed_vs_fp64"] = fluxpass_errors(b, 5.0 / 3.0, res["fp64"][0], res["fp64"][1],
+                                           res["mixed"][0], res["mixed"][1], once_per_pair=True)
+    out["fp64_conservation"] = float(np.abs(res["fp64"][1].sum(1)).max() /
+                                     np.abs(res["fp64"][0]).max())
+    out["alg_bytes_fp64"] = alg_bytes
+    out["fp64"]["achieved_GBs"] = alg_bytes / (out["fp64"]["ms_mean"] * 1e-3) / 1e9
+    return out
+
+
+def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
+    """Config 2: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC."""
+    from shadowfax_amd import mesh as M
+    t0 = time.perf_counter()
+    # generators in Hilbert order, as the reference keeps its particles
+    m = M.voronoi_periodic(M.hilbert_sorted(M.jittered_grid(n, 0.25, seed=20240603)),
+                           nthreads=host_threads())
+    b = M.prepare(m, M.kh_ics(m), 5.0 / 3.0, kind=kind)
+    t_mesh = time.perf_counter() - t0
     # algorithmic bytes of the FP64 pass: faces {iL, iR, A, midpoint, shift}
     # 48 B, cells {W, grad, centroid, pos, vel, m, A_tot, dt} 168 B read +
     # dQ 32 B written
-    alg = 48 * F + 200 * C
-    out["alg_bytes_fp64"] = alg
-    out["fp64"]["achieved_GBs"] = alg / (out["fp64"]["ms_mean"] * 1e-3) / 1e9
+    out = bench_fluxpass(b, kind, steps, warmup, dev, 48 * b["F"] + 200 * b["C"])
+    out["host_mesh_s"] = t_mesh
+    out["workload"] = ("2D periodic unit box, %dx%d jittered generators (seed 20240603) in "
+                       "Hilbert order, Kelvin-Helmholtz, HLLC" % (n, n))
     return out
+
+
+def bench_fluxpass3d(steps, warmup, dev, n=2097152, kind=0):
+    """Config 3: 2,097,152 Poisson generators in the 3D periodic unit box,
+    Sedov blast (E = 1 in the 32 cells nearest the centre), exact solver."""
+    from shadowfax_amd import mesh as M
+    t0 = time.perf_counter()
+    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(n, seed=20240604)),
+                           nthreads=host_threads())
+    t_mesh = time.perf_counter() - t0
+    b = M.prepare(m, M.sedov_ics(m), 5.0 / 3.0, kind=kind)
+    del m
+    # algorithmic bytes of the FP64 pass: faces {iL, iR 8, A 8, midpoint 24,
+    # shift 24} 64 B; cells {W 40, grad 120, centroid/pos/vel 72, m/A_tot/dt
+    # 24} 256 B read + dQ 40 B written
+    out = bench_fluxpass(b, kind, steps, warmup, dev, 64 * b["F"] + 296 * b["C"])
+    out["host_mesh_s"] = t_mesh
+    out["workload"] = ("3D periodic unit box, %d Poisson generators (seed 20240604) in "
+                       "Hilbert order, Sedov blast, exact solver" % n)
+    return out
+
+
+def measured_traffic(kernel, grid):
+    """HBM bytes per launch of a kernel from the committed hardware-counter
+    summary (profiles/hbm_traffic.json), if it was measured at this grid."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    if not k or int(k.get("grid", -1)) != grid:
+        return None
+    return float(k["traffic_bytes"])
+
+
+def host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def copy_probe(mib, dev):
+    """A plain elementwise copy of known size (reads and writes mib MiB per
+    launch): under a hardware-counter run its FETCH_SIZE / WRITE_SIZE per
+    dispatch give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    log("copy probe: %d MiB read + %d MiB written per launch" % (mib, mib))
 
 
 def cpu_baseline(b, N):
@@ -131,6 +199,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--no-fluxpass", action="store_true")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D flux pass (config 3)")
+    ap.add_argument("--only", default=None,
+                    help="comma list of solver variants to time (exact_fp64, exact_mixed, "
+                         "hllc_fp64, hllc_mixed); skips the headline line")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain FP32 copy kernel over this many MiB (5 launches), "
+                         "the scale reference for hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the face-flux path on a single "
@@ -157,8 +232,11 @@ def main():
 
     variants = {}
     outs = {}
+    only = set(args.only.split(",")) if args.only else None
     for kname, kind in (("exact", 0), ("hllc", 1)):
         for prec in ("fp64", "mixed"):
+            if only is not None and "%s_%s" % (kname, prec) not in only:
+                continue
             s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"], prec)
             s.enable_stats(False)
             src = d64 if prec == "fp64" else d32
@@ -180,6 +258,8 @@ def main():
             variants["%s_%s" % (kname, prec)] = r
     errors = {}
     for kname in ("exact", "hllc"):
+        if (kname, "fp64") not in outs or (kname, "mixed") not in outs:
+            continue
         F64, st64 = outs[(kname, "fp64")]
         F32, st32 = outs[(kname, "mixed")]
         ok = (st64 == 0) & (st32 == 0)
@@ -188,13 +268,21 @@ def main():
         variants[kname + "_mixed"]["speedup_vs_fp64"] = (
             variants[kname + "_fp64"]["ms_mean"] / variants[kname + "_mixed"]["ms_mean"])
 
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb, dev)
+    if "exact_mixed" not in variants:
+        print(json.dumps({"variants": variants}))
+        return
     head = variants["exact_mixed"]
     ms = head["ms_mean"]
     value = N / (ms * 1e-3)
     bytes_per = (2 * 5 + 3) * 4 + 5 * 4  # WL, WR, n in + F out, FP32
     achieved = bytes_per * N / (ms * 1e-3) / 1e9
+    traffic = measured_traffic("sfx::mixed::solve_for_flux_kernel<3, 0, float>", N)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": "profiles/hbm_traffic_pmc.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "scaled by a copy kernel of known size)" if traffic else None,
             "kernel": "sfx::mixed::solve_for_flux_kernel<3,0,float>",
             "bytes_per_problem": bytes_per,
             "note": "exact solver is VALU-bound (Newton on FP32 transcendentals + one FP64 "
@@ -211,6 +299,9 @@ def main():
     if not args.no_fluxpass:
         log("flux pass 2D (config 2)")
         out["fluxpass_2d_kh"] = bench_fluxpass2d(args.steps, args.warmup, dev)
+        if not args.no_3d:
+            log("flux pass 3D (config 3)")
+            out["fluxpass_3d_sedov"] = bench_fluxpass3d(args.steps, args.warmup, dev)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(b, N)

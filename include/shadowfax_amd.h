@@ -163,7 +163,19 @@ typedef struct {
     int32_t* twin;      /* [F] other copy of a periodic face, or -1 */
     int64_t* cf_off;    /* [C+1] */
     int32_t* cf_face;   /* [cf_off[C]] faces of each cell, walk order */
+    int64_t* fv_off;    /* [F+1] each face's vertex list, as stored by its
+                           creating cell (VorFace::get_vertices) */
+    double* fverts;     /* [ndim][fv_off[F]] */
 } sfx_host_mesh;
+
+/* ParticleVector::sort (src/utilities/ParticleVector.cpp:205-283): the
+ * reference keeps its particles, and so its cells, in space-filling Hilbert
+ * order.  order[k] = index of the k-th generator along a Hilbert curve over
+ * the box (31 bits per axis in 2D, 21 in 3D); generators must lie in
+ * [0, box[d]).  Cells in this order keep each cell's neighbours close in
+ * memory, which the flux-pass gathers rely on. */
+int sfx_hilbert_order(int ndim, int64_t C, const double* pos, const double* box,
+                      int64_t* order);
 
 /* Periodic Voronoi tessellation of C generators in [0, box[d]) (ndim 2 or
  * 3), built on the host with nthreads threads (0: all cores).  Stands in for
@@ -206,13 +218,19 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left,
 /* ---- face-flux pass (device pointers) -------------------------------- */
 
 /* Device mesh for the flux pass.  "T" arrays are double for an
- * SFX_PREC_FP64 handle and float for an SFX_PREC_MIXED handle. */
+ * SFX_PREC_FP64 handle and float for an SFX_PREC_MIXED handle; the
+ * primitive states W are double for both.
+ * Layout: arrays the kernels read through a neighbour index are cell- or
+ * face-major (W [C][ndim+2], vel [C][ndim], grad [C][ndim][ndim+2],
+ * dQ and Q [C][ndim+2], dQL [F][ndim+2]) so that one gather is one cache
+ * line; per-face geometry, read one face per lane, is component-major
+ * ([ndim][F]). */
 typedef struct {
     int ndim;
     int64_t C, F;
     /* cells */
-    const void* W;             /* T [ndim+2][C] primitives */
-    const void* vel;           /* T [ndim][C] generator velocities */
+    const double* W;           /* [C][ndim+2] primitives (FP64 in both passes) */
+    const void* vel;           /* T [C][ndim] generator velocities */
     const double* mass;        /* [C] Q.m */
     const double* total_area;  /* [C] sum of the cell's face areas */
     const double* dt;          /* [C] real time of the cell's time step */
@@ -233,42 +251,145 @@ typedef struct {
     /* topology */
     const int64_t* cf_off;     /* [C+1] */
     const int32_t* cf_entry;   /* sfx_mesh_cell_entries */
+    const void* egeo;          /* T [cf_off[C]][ndim == 2 ? 8 : 12] per-entry
+                                  face records (sfx_mesh_entry_geometry) */
     const int64_t* gather_off; /* [C+1] */
     const int32_t* gather_entry; /* sfx_mesh_gather_index */
 } sfx_mesh;
 
+/* Per-entry face records for the gradient pass, built once per
+ * tessellation from the face arrays and cf_entry: for entry j of cell i
+ * (face f seen from side s) [area, mc (ndim), rRL seen from i (ndim),
+ * dL or dR (ndim), the neighbour's index bits (-1: mirror), (3D) pad].
+ * The same values the gradient loop of VorCell::estimate_gradient reads per
+ * face (src/VorCell.cpp:471-500), stored in the cell's walk order so that
+ * the pass streams them.  E = cf_off[C]; egeo: T [E][ndim == 2 ? 8 : 12]. */
+int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* egeo,
+                            void* stream);
+
 /* VorCell::estimate_gradient for every active cell (src/VorCell.cpp:464-530,
- * loop src/Simulation.cpp:414-420).  grad: T [ndim][ndim+2][C]
- * (grad[(d*(ndim+2)+k)*C + i] = dW_k/dx_d); inactive cells are untouched. */
+ * loop src/Simulation.cpp:414-420).  grad: T [C][ndim][ndim+2]
+ * (grad[(i*ndim + d)*(ndim+2) + k] = dW_k/dx_d); inactive cells are
+ * untouched. */
 int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream);
 
 /* VorFace::set_v + calculate_flux for every face (src/VorFace.cpp:586-866):
- * dQL: [ndim+2][F] the flux exchanged through each face, dt*A*F after the
+ * dQL: [F][ndim+2] the flux exchanged through each face, dt*A*F after the
  * mass-flux limiter, seen from the left cell (the right cell receives -dQL);
  * face_status: [F] or NULL. */
 int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad,
                     double* dQL, uint8_t* face_status, void* stream);
 
-/* Deterministic scatter: dQ[(k)*C + i] = sum of the cell's entries in
+/* Deterministic scatter: dQ[i*(ndim+2) + k] = sum of the cell's entries in
  * ascending face order (overwrites dQ). */
 int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL,
                   double* dQ, void* stream);
 
 /* VorTessManager::hydro (src/VorTessManager.cpp:198-212) preceded by the
  * gradient estimation: sfx_gradients + sfx_face_fluxes + sfx_gather_dQ.
- * grad (T [ndim][ndim+2][C]) and dQL ([ndim+2][F]) are caller-owned work
- * buffers; dQ receives the per-cell change [ndim+2][C]. */
+ * grad (T [C][ndim][ndim+2]) and dQL ([F][ndim+2]) are caller-owned work
+ * buffers; dQ receives the per-cell change [C][ndim+2]. */
 int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad,
                        double* dQL, double* dQ, uint8_t* face_status,
                        void* stream);
 
 /* GasParticle::update_Q then get_W (src/utilities/GasParticle.cpp:617-624,
  * src/riemann/ExactRiemannSolver.cpp:716-766): Q -= dQ, dQ = 0,
- * W = get_W(volume, Q) (W is T [ndim+2][C]); cell_status bit
- * SFX_ST_UPDATEQ where m or e went negative. */
+ * W = get_W(volume, Q) (Q, dQ, W [C][ndim+2]);
+ * cell_status bit SFX_ST_UPDATEQ where m or e went negative. */
 int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ,
-                    const double* volume, void* W, uint8_t* cell_status,
+                    const double* volume, double* W, uint8_t* cell_status,
                     void* stream);
+
+
+/* ---- serving: an ensemble of meshes resident in HBM ------------------ */
+
+/* Many independent meshes held on the device together and stepped in
+ * batches (no reference counterpart: the reference runs one mesh per MPI
+ * job; the per-mesh step is the same VorTessManager::hydro + update_Q +
+ * get_W).  All meshes share one set of arrays with global indices; the
+ * per-step scratch (gradients, face fluxes) is an arena of arena_cells
+ * cells and arena_faces faces that each batch shares out, so a batch is
+ * admitted by its scratch footprint.  A batched step is one replay of a
+ * captured HIP graph: the batch list (host to device), a plan kernel and
+ * four persistent kernels draining work tiles of all admitted meshes. */
+typedef struct sfx_ensemble sfx_ensemble;
+
+typedef struct {
+    int64_t nreq, nbatch, max_batch;
+    double wall_s;     /* first arrival window to last completion */
+    double busy_s;     /* time with a step in flight */
+    double p50_s, p99_s, mean_s, max_s;  /* request latency (arrival to done) */
+    double mean_batch; /* meshes per step */
+} sfx_serve_stats;
+
+#define SFX_ENS_W 0
+#define SFX_ENS_Q 1
+#define SFX_ENS_DQ 2
+#define SFX_SERVE_BATCHED 0 /* batches through the captured graph */
+#define SFX_SERVE_PLAIN 1   /* one mesh per step, launches sized to the mesh */
+
+/* Capacities are totals over all meshes (entries: cf_off[C] summed, the
+ * same bound is used for the gather index).  The handle's precision selects
+ * the storage (SFX_PREC_MIXED: FP32 velocities, gradients and face
+ * geometry). */
+int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells,
+                        int64_t max_faces, int64_t max_entries,
+                        int64_t arena_cells, int64_t arena_faces,
+                        sfx_ensemble** out);
+int sfx_ensemble_destroy(sfx_ensemble* e);
+
+/* Copies one mesh (device arrays with local indices, the sfx_mesh layout of
+ * the handle's precision; egeo, if set, is ignored) and its conserved
+ * variables Q ([C][ndim+2]) into the ensemble.  E = cf_off[C],
+ * G = gather_off[C].  Synchronous.  SFX_ERR_NOMEM when the ensemble is
+ * full. */
+int sfx_ensemble_add_mesh(sfx_ensemble* e, const sfx_mesh* m, const double* Q,
+                          int64_t E, int64_t G, int32_t* mesh_id);
+
+/* mesh_id >= 0: its cells and faces; mesh_id < 0: totals, returns the
+ * number of meshes. */
+int sfx_ensemble_info(const sfx_ensemble* e, int32_t mesh_id, int64_t* cells,
+                      int64_t* faces);
+
+/* One step (gradients, face fluxes, gather, update_Q + get_W) of each mesh
+ * in ids (no mesh twice; the batch's scratch must fit the arena).
+ * Asynchronous on the ensemble's stream. */
+int sfx_ensemble_step(sfx_ensemble* e, const int32_t* ids, int32_t n);
+/* The same step for one mesh with plain launches sized to it. */
+int sfx_ensemble_step_plain(sfx_ensemble* e, int32_t id);
+int sfx_ensemble_sync(sfx_ensemble* e);
+/* Copies W, Q or dQ (SFX_ENS_*) of one mesh to host_out ([C][ndim+2]). */
+int sfx_ensemble_read(sfx_ensemble* e, int32_t id, int what, double* host_out);
+
+/* Launch shape of the persistent kernels: threads per block (= cells or
+ * faces per work tile; 64..256, a multiple of 64) and blocks per CU. */
+int sfx_ensemble_set_launch(sfx_ensemble* e, int block, int blocks_per_cu);
+int sfx_ensemble_get_launch(const sfx_ensemble* e, int* block, int* blocks_per_cu);
+/* Times reps batched steps of ids for each block size in {64, 128, 256}
+ * and blocks per CU in {1, 2, 4, 8}, keeps the fastest (best_ms: its time
+ * per step) and restores the meshes' states afterwards. */
+int sfx_ensemble_autotune(sfx_ensemble* e, const int32_t* ids, int32_t n, int reps,
+                          double* best_ms);
+
+/* FIFO admission: the longest prefix of the pending requests (meshes
+ * pending[k]) whose scratch fits the arena, skipping (and keeping queued)
+ * a second request for a mesh already admitted; at most max_batch.
+ * admitted[0..*nadmitted) are indices into pending.  Host only. */
+int sfx_serve_admit(int64_t npending, const int32_t* pending,
+                    const int64_t* mesh_cells, const int64_t* mesh_faces,
+                    int64_t arena_cells, int64_t arena_faces, int32_t max_batch,
+                    int32_t* admitted, int32_t* nadmitted);
+
+/* Serves an arrival trace in real time, open loop: request r for mesh
+ * mesh_id[r] arrives arrival_s[r] seconds after the call (ascending); while
+ * the device is busy arrivals queue; when it is free the admitted prefix of
+ * the queue is stepped (SFX_SERVE_BATCHED) or the head alone
+ * (SFX_SERVE_PLAIN).  latency_s[r] = completion - arrival; batch_of[r] (or
+ * NULL) the step that served it. */
+int sfx_ensemble_serve(sfx_ensemble* e, int mode, int64_t nreq,
+                       const double* arrival_s, const int32_t* mesh_id,
+                       double* latency_s, int32_t* batch_of, sfx_serve_stats* st);
 
 #ifdef __cplusplus
 }

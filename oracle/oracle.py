@@ -393,3 +393,23 @@ def ref_mesh_gradients(mesh):
         _p(right, _i32p), _p(area), _p(mid), _p(shift), _p(ghost, _u8p),
         _p(cf_off, _i64p), _p(cf_face, _i32p), _p(out), _p(vface), _p(normal))
     return out, vface, normal
+
+
+def ref_mesh_geometry(mesh):
+    """Face area/midpoint and cell volume/centroid computed by the reference
+    VorFace/VorCell classes from the mesh's stored face vertex lists."""
+    ndim = int(mesh["ndim"])
+    C, F = int(mesh["C"]), int(mesh["F"])
+    pos = np.ascontiguousarray(mesh["pos"], np.float64)
+    fv_off = np.ascontiguousarray(mesh["fv_off"], np.int64)
+    fverts = np.ascontiguousarray(mesh["fverts"], np.float64)
+    cf_off = np.ascontiguousarray(mesh["cf_off"], np.int64)
+    cf_face = np.ascontiguousarray(mesh["cf_face"], np.int32)
+    area = np.zeros(F)
+    mid = np.zeros((ndim, F))
+    vol = np.zeros(C)
+    cen = np.zeros((ndim, C))
+    ref(ndim).sfxr_mesh_geometry(C, F, _p(pos), _p(fv_off, _i64p), _p(fverts),
+                                 _p(cf_off, _i64p), _p(cf_face, _i32p), _p(area), _p(mid),
+                                 _p(vol), _p(cen))
+    return area, mid, vol, cen

@@ -22,6 +22,7 @@
 #include "riemann/HLLCRiemannSolver.hpp"
 #include "utilities/GasParticle.hpp"
 
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -248,6 +249,71 @@ void sfxr_mesh_gradients(int64_t C, int64_t F, const double* W,
     }
     // objects are leaked on purpose: destructors of the reference mesh
     // classes walk global tessellation state this harness does not build
+}
+
+// Face and cell geometry with the reference classes: for every face, the
+// stored vertex list (fverts [ndim][fv_off[F]]) goes through
+// VorFace::add_vertex, then calculate_midpoint and calculate_area in the
+// order VorTess::construct calls them (3D: midpoint first, src/VorTess.cpp:
+// 266-268; 2D: area first, :336-343, a face with fewer than 3 vertices or a
+// 2D face shorter than 1e-13 gets area 0); every cell collects its faces
+// (cf_off/cf_face) and runs VorCell::calculate_centroid and
+// calculate_volume (src/VorCell.cpp:257-411).
+void sfxr_mesh_geometry(int64_t C, int64_t F, const double* pos,
+                        const int64_t* fv_off, const double* fverts,
+                        const int64_t* cf_off, const int32_t* cf_face,
+                        double* area, double* midpoint, double* volume,
+                        double* centroid) {
+    const int64_t NV = fv_off[F];
+    std::vector<VorGen*> gens(C);
+    std::vector<VorCell*> cells(C);
+    for(int64_t i = 0; i < C; i++) {
+#if ndim_ == 3
+        gens[i] = new VorGen(pos[i], pos[C + i], pos[2 * C + i]);
+#else
+        gens[i] = new VorGen(pos[i], pos[C + i]);
+#endif
+        cells[i] = new VorCell(gens[i]);
+    }
+    std::vector<VorFace*> faces(F);
+    for(int64_t f = 0; f < F; f++) {
+        faces[f] = new VorFace(0, gens);
+        const int64_t n = fv_off[f + 1] - fv_off[f];
+        for(int64_t t = fv_off[f]; t < fv_off[f + 1]; t++) {
+#if ndim_ == 3
+            faces[f]->add_vertex(new VorGen(fverts[t], fverts[NV + t], fverts[2 * NV + t]));
+#else
+            faces[f]->add_vertex(new VorGen(fverts[t], fverts[NV + t]));
+#endif
+        }
+#if ndim_ == 3
+        if(n < 3) {
+            faces[f]->set_area(0.);
+        } else {
+            faces[f]->calculate_midpoint();
+            faces[f]->calculate_area();
+        }
+#else
+        (void)n;
+        faces[f]->calculate_area();
+        const int64_t t = fv_off[f];
+        double dx = fverts[t] - fverts[t + 1], dy = fverts[NV + t] - fverts[NV + t + 1];
+        if(sqrt(dx * dx + dy * dy) < 1.e-13) faces[f]->set_area(0.);
+        faces[f]->calculate_midpoint();
+#endif
+        area[f] = faces[f]->get_area();
+        Vec& m = faces[f]->get_midpoint();
+        for(int d = 0; d < ndim_; d++) midpoint[d * F + f] = m[d];
+    }
+    for(int64_t i = 0; i < C; i++) {
+        for(int64_t j = cf_off[i]; j < cf_off[i + 1]; j++) cells[i]->add_face(faces[cf_face[j]]);
+        cells[i]->calculate_centroid();
+        cells[i]->calculate_volume();
+        volume[i] = cells[i]->get_volume();
+        Vec& c = cells[i]->get_centroid();
+        for(int d = 0; d < ndim_; d++) centroid[d * C + i] = c[d];
+    }
+    // leaked on purpose, as above
 }
 
 }  // extern "C"

@@ -36,7 +36,8 @@ class MeshDesc(ct.Structure):
                 ("volume", vp), ("active", vp),
                 ("left", vp), ("right", vp), ("ghost", vp), ("twin", vp),
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
-                ("cf_off", vp), ("cf_entry", vp), ("gather_off", vp), ("gather_entry", vp)]
+                ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
+                ("gather_entry", vp)]
 
 
 _lib = None
@@ -91,6 +92,7 @@ def lib():
 def _mesh_sigs():
     md = ct.POINTER(MeshDesc)
     return {
+        "sfx_mesh_entry_geometry": [vp, md, i64, vp, vp],
         "sfx_gradients": [vp, md, vp, vp],
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],

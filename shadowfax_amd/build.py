@@ -38,6 +38,10 @@ SOURCES = [
     ("sfx_capi.cpp", []),
     ("sfx_mesh_capi.cpp", ["-ffp-contract=off"]),
     ("mesh_voronoi.cpp", ["-ffp-contract=off"]),
+    ("mesh_voronoi3d.cpp", ["-ffp-contract=off"]),
+    ("ensemble_fp64.hip", ["-ffp-contract=off"]),
+    ("ensemble_mixed.hip", ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("sfx_ensemble.cpp", []),
 ]
 
 

@@ -1,0 +1,240 @@
+// ensemble_impl.h -- one hydro step for a batch of independent meshes held
+// in HBM together (the serving path).
+//
+// All meshes of an ensemble live in one set of arrays with global cell, face
+// and entry indices (mesh k owns the ranges [cell0, cell0 + ncell) etc.).
+// Per-step scratch -- gradients [cell][ND][NC] and face fluxes dQL
+// [face][NC] -- lives in an arena that each batch shares out: a mesh admitted
+// to a batch gets a contiguous arena range, and its kernels address it with
+// a constant shift (arena index = global index + shift).
+//
+// A step for a batch is: plan_kernel (ensemble_fp64.hip; one block: prefix sums over the
+// batch give every mesh its arena offsets and its range of work tiles, and
+// it zeroes the tile counters) followed by four persistent drains --
+// gradients, face fluxes, gather, update -- each a fixed grid whose blocks
+// claim tiles of `tile` consecutive cells or faces until none are left.
+// The tile range of a phase is cut into one contiguous chunk per XCD; a
+// block drains the chunk of the XCD it runs on (dispatch index % 8) first,
+// then the others, so neighbouring cells mostly meet in one L2.  Every block
+// leaves a chunk once its counter passes the chunk's end, so every drain
+// ends after at most one pass over the eight counters.
+//
+// The per-item work is exactly the standalone kernels' (grad_cell,
+// face_item, gather_cell, update_cell in flux_pass_impl.h), so a mesh
+// stepped inside an ensemble gets bit-identical results.
+#pragma once
+
+#include "flux_pass_impl.h"
+
+namespace sfx {
+namespace ens {
+
+constexpr int kMaxBatch = 1024;
+constexpr int kXcd = 8;
+constexpr int kPhases = 4;  // gradients, faces, gather, update
+
+// One admitted mesh.
+struct Slot {
+    int64_t cell0, ncell, face0, nface;  // global ranges of the mesh
+    int64_t gshift, qshift;              // arena index - global index
+    int64_t ctile0, ftile0;              // first cell tile, first face tile
+};
+
+struct Plan {
+    int32_t n;       // meshes in the batch
+    int32_t tile;    // items per tile
+    int64_t ctiles;  // cell tiles
+    int64_t ftiles;  // face tiles
+    unsigned ctr[kPhases][kXcd];
+    Slot slot[kMaxBatch];
+};
+
+// the slot whose tile range holds tile t (tile0 ascending in the slot order)
+template <bool FACES>
+__device__ __forceinline__ int find_slot(const Plan& p, int64_t t) {
+    int lo = 0, hi = p.n - 1;
+    while(lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t t0 = FACES ? p.slot[mid].ftile0 : p.slot[mid].ctile0;
+        if(t0 <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Drains the tiles of one phase: fn(slot, item) for every item of every
+// admitted mesh.  Called by every thread of the block.
+template <bool FACES, class Fn>
+__device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
+    __shared__ int64_t s_tile;
+    const int64_t T = FACES ? p->ftiles : p->ctiles;
+    const int tile = p->tile;
+    const int x0 = blockIdx.x % kXcd;
+    for(int c = 0; c < kXcd; c++) {
+        const int x = (x0 + c) % kXcd;
+        const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
+        for(;;) {
+            if(threadIdx.x == 0) s_tile = lo + (int64_t)atomicAdd(&p->ctr[phase][x], 1u);
+            __syncthreads();
+            const int64_t t = s_tile;
+            __syncthreads();
+            if(t >= hi) break;
+            const int s = find_slot<FACES>(*p, t);
+            const Slot& sl = p->slot[s];
+            const int64_t base = FACES ? sl.face0 + (t - sl.ftile0) * tile
+                                       : sl.cell0 + (t - sl.ctile0) * tile;
+            const int64_t end = FACES ? sl.face0 + sl.nface : sl.cell0 + sl.ncell;
+            const int64_t item = base + threadIdx.x;
+            if(threadIdx.x < tile && item < end) fn(sl, item);
+        }
+    }
+}
+
+template <int ND, typename T, typename R, bool DIFF>
+__global__ __launch_bounds__(256) void drain_gradients(flux::MeshView<T> m, T* __restrict__ grad,
+                                                       Plan* __restrict__ p) {
+    drain<false>(p, 0, [&](const Slot& sl, int64_t i) {
+        flux::grad_cell<ND, T, R, DIFF>(m, i, grad, sl.gshift);
+    });
+}
+
+template <int ND, int KIND, int MIXED, typename T, typename R>
+__global__ __launch_bounds__(256) void drain_faces(flux::MeshView<T> m, const T* __restrict__ grad,
+                                                   Consts cd, mixed::CF cf,
+                                                   double* __restrict__ dQL,
+                                                   uint8_t* __restrict__ fstatus,
+                                                   unsigned long long* __restrict__ ctr,
+                                                   Plan* __restrict__ p) {
+    unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
+    drain<true>(p, 1, [&](const Slot& sl, int64_t f) {
+        flux::face_item<ND, KIND, MIXED, T, R>(m, f, grad, sl.gshift, cd, cf, dQL, sl.qshift,
+                                               fstatus, cnt);
+    });
+    block_add_counters(ctr, cnt);
+}
+
+template <int ND, int MIXED>
+__global__ __launch_bounds__(256) void drain_gather(const int64_t* __restrict__ goff,
+                                                    const int32_t* __restrict__ gent,
+                                                    const uint8_t* __restrict__ ghost,
+                                                    const int32_t* __restrict__ twin,
+                                                    const double* __restrict__ dQL,
+                                                    double* __restrict__ dQ,
+                                                    Plan* __restrict__ p) {
+    drain<false>(p, 2, [&](const Slot& sl, int64_t i) {
+        flux::gather_cell<ND, MIXED>(i, goff, gent, ghost, twin, dQL, sl.qshift, dQ);
+    });
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void drain_update(Consts cd, double* __restrict__ Q,
+                                                    double* __restrict__ dQ,
+                                                    const double* __restrict__ volume,
+                                                    double* __restrict__ W,
+                                                    uint8_t* __restrict__ cstatus,
+                                                    unsigned long long* __restrict__ ctr,
+                                                    Plan* __restrict__ p) {
+    unsigned fail = 0;
+    drain<false>(p, 3, [&](const Slot&, int64_t i) {
+        fail += flux::update_cell<ND>(cd, i, Q, dQ, volume, W, cstatus);
+    });
+    const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
+    block_add_counters(ctr, cv);
+}
+
+// ---- one mesh, plain launches sized to the mesh (the unbatched path) ----
+
+template <int ND, typename T, typename R, bool DIFF>
+__global__ __launch_bounds__(256) void range_gradients(flux::MeshView<T> m, T* __restrict__ grad,
+                                                       int64_t cell0, int64_t ncell,
+                                                       int64_t gshift) {
+    const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
+    if(k < ncell) flux::grad_cell<ND, T, R, DIFF>(m, cell0 + k, grad, gshift);
+}
+
+template <int ND, int KIND, int MIXED, typename T, typename R>
+__global__ __launch_bounds__(256) void range_faces(flux::MeshView<T> m, const T* __restrict__ grad,
+                                                   Consts cd, mixed::CF cf,
+                                                   double* __restrict__ dQL,
+                                                   uint8_t* __restrict__ fstatus,
+                                                   unsigned long long* __restrict__ ctr,
+                                                   int64_t face0, int64_t nface, int64_t gshift,
+                                                   int64_t qshift) {
+    const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
+    if(k < nface)
+        flux::face_item<ND, KIND, MIXED, T, R>(m, face0 + k, grad, gshift, cd, cf, dQL, qshift,
+                                               fstatus, cnt);
+    block_add_counters(ctr, cnt);
+}
+
+template <int ND, int MIXED>
+__global__ __launch_bounds__(256) void range_gather(const int64_t* __restrict__ goff,
+                                                    const int32_t* __restrict__ gent,
+                                                    const uint8_t* __restrict__ ghost,
+                                                    const int32_t* __restrict__ twin,
+                                                    const double* __restrict__ dQL,
+                                                    double* __restrict__ dQ, int64_t cell0,
+                                                    int64_t ncell, int64_t qshift) {
+    const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
+    if(k < ncell) flux::gather_cell<ND, MIXED>(cell0 + k, goff, gent, ghost, twin, dQL, qshift, dQ);
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void range_update(Consts cd, double* __restrict__ Q,
+                                                    double* __restrict__ dQ,
+                                                    const double* __restrict__ volume,
+                                                    double* __restrict__ W,
+                                                    uint8_t* __restrict__ cstatus,
+                                                    unsigned long long* __restrict__ ctr,
+                                                    int64_t cell0, int64_t ncell) {
+    const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned fail = 0;
+    if(k < ncell) fail = flux::update_cell<ND>(cd, cell0 + k, Q, dQ, volume, W, cstatus);
+    const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
+    block_add_counters(ctr, cv);
+}
+
+}  // namespace ens
+
+// Everything one ensemble step touches.  m spans the whole ensemble (C, F
+// are the capacities, the strides of the component-major face arrays).
+template <typename T>
+struct EnsArgs {
+    flux::MeshView<T> m;
+    T* grad;       // arena [cells][ND][NC]
+    double* dQL;   // arena [faces][NC]
+    double* dQ;    // [C][NC]
+    double* Q;     // [C][NC]
+    double* W;     // [C][NC] (the same array as m.W)
+    uint8_t* fstatus;
+    uint8_t* cstatus;
+    Consts c;
+    unsigned long long* ctr;
+    ens::Plan* plan;
+    int block;    // threads per block = items per tile
+    int nblocks;  // persistent grid
+};
+
+// ensemble_fp64.hip (T = double) and ensemble_mixed.hip (T = float)
+template <typename T>
+hipError_t ens_launch_step(int ndim, int kind, const EnsArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t ens_launch_range(int ndim, int kind, const EnsArgs<T>& a, int64_t cell0, int64_t ncell,
+                            int64_t face0, int64_t nface, int64_t gshift, int64_t qshift,
+                            hipStream_t s);
+template <typename T>
+hipError_t ens_launch_entries(int ndim, const flux::MeshView<T>& m, int64_t E, T* egeo,
+                              hipStream_t s);
+// ensemble_fp64.hip
+hipError_t ens_launch_plan(const int32_t* batch, const int64_t* mcell0, const int64_t* mncell,
+                           const int64_t* mface0, const int64_t* mnface, int32_t tile,
+                           ens::Plan* plan, hipStream_t s);
+hipError_t ens_launch_offset_i32(const int32_t* src, int32_t* dst, int64_t n, int64_t off,
+                                 hipStream_t s);
+hipError_t ens_launch_offset_i64(const int64_t* src, int64_t* dst, int64_t n, int64_t off,
+                                 hipStream_t s);
+
+}  // namespace sfx

@@ -1,0 +1,74 @@
+// ensemble_launch.h -- launch sequences of the ensemble kernels, shared by
+// the FP64 (ensemble_fp64.hip) and mixed (ensemble_mixed.hip) translation
+// units, which instantiate them with the same compile flags as the
+// standalone flux pass of that precision.
+#pragma once
+
+#include "ensemble_impl.h"
+
+namespace sfx {
+
+template <typename T, int MIXED>
+hipError_t ens_step_impl(int ndim, int kind, const EnsArgs<T>& a, hipStream_t s) {
+    const mixed::CF cf = mixed::make_cf(a.c);
+    const dim3 g((unsigned)a.nblocks), b((unsigned)a.block);
+#define SFX_ENS(ND)                                                                            \
+    do {                                                                                       \
+        hipLaunchKernelGGL((ens::drain_gradients<ND, T, double, MIXED != 0>), g, b, 0, s, a.m,  \
+                           a.grad, a.plan);                                                     \
+        if(kind == 0)                                                                          \
+            hipLaunchKernelGGL((ens::drain_faces<ND, 0, MIXED, T, double>), g, b, 0, s, a.m,    \
+                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan);               \
+        else                                                                                   \
+            hipLaunchKernelGGL((ens::drain_faces<ND, 1, MIXED, T, double>), g, b, 0, s, a.m,    \
+                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan);               \
+        hipLaunchKernelGGL((ens::drain_gather<ND, MIXED>), g, b, 0, s, a.m.gather_off,          \
+                           a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, a.plan);         \
+        hipLaunchKernelGGL((ens::drain_update<ND>), g, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,     \
+                           a.W, a.cstatus, a.ctr, a.plan);                                      \
+    } while(0)
+    if(ndim == 3)
+        SFX_ENS(3);
+    else
+        SFX_ENS(2);
+#undef SFX_ENS
+    return hipGetLastError();
+}
+
+template <typename T, int MIXED>
+hipError_t ens_range_impl(int ndim, int kind, const EnsArgs<T>& a, int64_t cell0, int64_t ncell,
+                          int64_t face0, int64_t nface, int64_t gshift, int64_t qshift,
+                          hipStream_t s) {
+    const mixed::CF cf = mixed::make_cf(a.c);
+    const dim3 b(256);
+    const dim3 gc((unsigned)((ncell + 255) / 256)), gf((unsigned)((nface + 255) / 256));
+    if(ncell <= 0) return hipSuccess;
+#define SFX_RNG(ND)                                                                            \
+    do {                                                                                       \
+        hipLaunchKernelGGL((ens::range_gradients<ND, T, double, MIXED != 0>), gc, b, 0, s, a.m, \
+                           a.grad, cell0, ncell, gshift);                                       \
+        if(nface > 0) {                                                                        \
+            if(kind == 0)                                                                      \
+                hipLaunchKernelGGL((ens::range_faces<ND, 0, MIXED, T, double>), gf, b, 0, s,    \
+                                   a.m, a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, face0,        \
+                                   nface, gshift, qshift);                                      \
+            else                                                                               \
+                hipLaunchKernelGGL((ens::range_faces<ND, 1, MIXED, T, double>), gf, b, 0, s,    \
+                                   a.m, a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, face0,        \
+                                   nface, gshift, qshift);                                      \
+        }                                                                                      \
+        hipLaunchKernelGGL((ens::range_gather<ND, MIXED>), gc, b, 0, s, a.m.gather_off,         \
+                           a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, cell0, ncell,    \
+                           qshift);                                                             \
+        hipLaunchKernelGGL((ens::range_update<ND>), gc, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,    \
+                           a.W, a.cstatus, a.ctr, cell0, ncell);                                \
+    } while(0)
+    if(ndim == 3)
+        SFX_RNG(3);
+    else
+        SFX_RNG(2);
+#undef SFX_RNG
+    return hipGetLastError();
+}
+
+}  // namespace sfx

@@ -9,14 +9,27 @@ namespace sfx {
 static inline dim3 grid_for(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
 template <>
+hipError_t launch_entry_geometry<double>(int ndim, const flux::MeshView<double>& m, int64_t E,
+                                     double* egeo, hipStream_t s) {
+    if(E <= 0) return hipSuccess;
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::entry_geometry_kernel<3, double>), grid_for(E, 256), dim3(256), 0,
+                           s, m, E, egeo);
+    else
+        hipLaunchKernelGGL((flux::entry_geometry_kernel<2, double>), grid_for(E, 256), dim3(256), 0,
+                           s, m, E, egeo);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t launch_gradients<double>(int ndim, const flux::MeshView<double>& m, double* grad,
                                     hipStream_t s) {
     if(m.C <= 0) return hipSuccess;
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gradient_kernel<3, double, double>), grid_for(m.C, 256),
+        hipLaunchKernelGGL((flux::gradient_kernel<3, double, double, false>), grid_for(m.C, 256),
                            dim3(256), 0, s, m, grad);
     else
-        hipLaunchKernelGGL((flux::gradient_kernel<2, double, double>), grid_for(m.C, 256),
+        hipLaunchKernelGGL((flux::gradient_kernel<2, double, double, false>), grid_for(m.C, 256),
                            dim3(256), 0, s, m, grad);
     return hipGetLastError();
 }
@@ -52,17 +65,16 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
     return hipGetLastError();
 }
 
-template <>
-hipError_t launch_update<double>(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
-                                 const double* volume, double* W, uint8_t* cstatus,
-                                 unsigned long long* ctr, hipStream_t s) {
+hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
+                         const double* volume, double* W, uint8_t* cstatus,
+                         unsigned long long* ctr, hipStream_t s) {
     if(C <= 0) return hipSuccess;
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::update_kernel<3, double>), grid_for(C, 256), dim3(256), 0, s,
-                           c, C, Q, dQ, volume, W, cstatus, ctr);
+        hipLaunchKernelGGL((flux::update_kernel<3>), grid_for(C, 256), dim3(256), 0, s, c, C, Q,
+                           dQ, volume, W, cstatus, ctr);
     else
-        hipLaunchKernelGGL((flux::update_kernel<2, double>), grid_for(C, 256), dim3(256), 0, s,
-                           c, C, Q, dQ, volume, W, cstatus, ctr);
+        hipLaunchKernelGGL((flux::update_kernel<2>), grid_for(C, 256), dim3(256), 0, s, c, C, Q,
+                           dQ, volume, W, cstatus, ctr);
     return hipGetLastError();
 }
 

@@ -17,8 +17,16 @@
 // cell states, gradients and face geometry) and R (arithmetic type of the
 // reconstruction).  The FP64 parity pass is T = R = double in a TU compiled
 // with -ffp-contract=off, every expression in the reference's association;
-// the mixed pass is T = R = float with the mixed Riemann solvers and FP64
-// dQ.  Faces carry geometry precomputed on the host from the tessellation
+// the mixed pass is T = float, R = double with the mixed Riemann solvers and
+// FP64 dQ.
+//
+// Layout: everything a kernel gathers through a neighbour index is
+// item-major (one cache line per gather): W [C][NC], vel [C][ND],
+// grad [C][ND][NC], dQ [C][NC], Q [C][NC], dQL [F][NC]; per-face geometry,
+// read one face per lane, is component-major [ND][F]; K2 reads each cell's
+// faces through per-entry records (entry_geometry_kernel) stored in the
+// cell's walk order, so its loads stream instead of chasing face indices.
+// Faces carry geometry precomputed on the host from the tessellation
 // with the reference's FP64 expressions (see sfx_mesh_capi.cpp):
 //   dL = midpoint - centroid_L,  dR = midpoint - centroid_R + (pos_R - x_Rgen)
 //   rRL = x_Rgen - x_L,  mc = midpoint - (x_L + x_Rgen)/2,
@@ -39,8 +47,8 @@ __device__ __forceinline__ bool ent_right(int32_t e) { return e & 1; }
 template <typename T>
 struct MeshView {
     int64_t C, F;
-    const T* W;           // [NC][C]
-    const T* vel;         // [ND][C]
+    const double* W;      // [C][NC], FP64 in both passes
+    const T* vel;         // [C][ND]
     const double* mass;   // [C]
     const double* total_area;
     const double* dt;
@@ -58,9 +66,25 @@ struct MeshView {
     const T* rcent;  // [F]
     const int64_t* cf_off;
     const int32_t* cf_entry;
+    const T* egeo;   // [cf_off[C]][entry_rec(ND)]
     const int64_t* gather_off;
     const int32_t* gather_entry;
 };
+
+// Per-entry record of a cell's face, in the cell's walk order:
+//   [A, c (ND), r_ij (ND), d (ND), neighbour index bits, (3D: pad)]
+// with c = mc, r_ij = rRL seen from the cell (-rRL for the left cell),
+// d = dL or dR, neighbour = the other cell or -1 (mirror).
+__host__ __device__ constexpr int entry_rec(int ND) { return ND == 2 ? 8 : 12; }
+
+__device__ __forceinline__ double index_bits(int32_t i, double) {
+    return __longlong_as_double((long long)i);
+}
+__device__ __forceinline__ float index_bits(int32_t i, float) { return __int_as_float(i); }
+__device__ __forceinline__ int32_t bits_index(double v) {
+    return (int32_t)__double_as_longlong(v);
+}
+__device__ __forceinline__ int32_t bits_index(float v) { return __float_as_int(v); }
 
 template <typename R>
 __device__ __forceinline__ R rmin(R a, R b) { return fmin(a, b); }
@@ -207,59 +231,117 @@ __device__ __forceinline__ void predict_side(R gamma, const R* Wraw, const R* Wo
 
 // ------------------------------------------------------------- K2 grads --
 
-template <int ND, typename T, typename R>
-__global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restrict__ grad) {
-    const int NC = ND + 2;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= m.C) return;
+// One lane per cell entry: the record of entry j (face f = e >> 1 seen from
+// side e & 1) from the face arrays; run once per tessellation.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int64_t E,
+                                                             T* __restrict__ egeo) {
+    const int ER = entry_rec(ND);
+    const int64_t j = xcd_block() * blockDim.x + threadIdx.x;
+    if(j >= E) return;
+    const int64_t F = m.F;
+    const int32_t e = m.cf_entry[j];
+    const int32_t f = ent_face(e);
+    const bool isR = ent_right(e);
+    T rec[ER];
+    rec[0] = m.area[f];
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        const T rr = m.rRL[l * F + f];
+        rec[1 + l] = m.mc[l * F + f];
+        rec[1 + ND + l] = isR ? rr : -rr;
+        rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+    }
+    rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], T(0));
+    if(ER > 2 + 3 * ND) rec[ER - 1] = T(0);
+#pragma unroll
+    for(int k = 0; k < ER; k++) egeo[j * ER + k] = rec[k];
+}
+
+// DIFF selects the form of the Green-Gauss sum.  The reference's
+//   sum_f A [(Wj - Wi) c/|r| - (Wi + Wj)/2 r/|r|]
+// (DIFF = false, the FP64 parity pass) cancels the Wi sum_f A r/|r| part
+// only through the cell's closure sum_f A n_f = 0, which FP32 face areas
+// hold to 6e-8 only: the residual is a spurious gradient proportional to W
+// itself.  DIFF = true (the mixed pass) sums the algebraically equal
+//   sum_f A (Wj - Wi) (c - r/2)/|r|
+// that needs no cancellation.
+// Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
+// for a whole mesh, the arena offset of the cell's mesh in an ensemble).
+template <int ND, typename T, typename R, bool DIFF>
+__device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __restrict__ grad,
+                                          int64_t gshift) {
+    const int NC = ND + 2, ER = entry_rec(ND);
     if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
-    const int64_t C = m.C, F = m.F;
     R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) {
-        W[k] = Wmax[k] = Wmin[k] = (R)m.W[k * C + i];
+        W[k] = Wmax[k] = Wmin[k] = (R)m.W[i * NC + k];
 #pragma unroll
         for(int l = 0; l < ND; l++) delta[l][k] = R(0);
     }
     const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
-    for(int64_t j = j1; j-- > j0;) {
-        const int32_t e = m.cf_entry[j];
-        const int32_t f = ent_face(e);
-        const bool isR = ent_right(e);
-        const R A = (R)m.area[f];
-        if(!(A != R(0))) continue;
-        R c[ND], rij[ND], rr[ND];
+    // faces in reverse (src/VorCell.cpp:471), KB at a time: the records and
+    // neighbour states of a chunk are loaded before any is used, so a lane
+    // has KB independent loads in flight instead of one dependent chain
+    constexpr int KB = 4;
+    for(int64_t jb = j1; jb > j0; jb -= KB) {
+        R A[KB], c[KB][ND], rij[KB][ND], Wj[KB][NC];
+        int32_t o[KB];
 #pragma unroll
-        for(int l = 0; l < ND; l++) {
-            c[l] = (R)m.mc[l * F + f];
-            rr[l] = (R)m.rRL[l * F + f];
-            rij[l] = isR ? rr[l] : -rr[l];
-        }
-        R rnorm = (ND == 3) ? sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[ND - 1] * rij[ND - 1])
-                            : sqrt(rij[0] * rij[0] + rij[1] * rij[1]);
-        R Wj[NC];
-        const int32_t Rc = m.right[f];
-        if(!isR && Rc < 0) {
+        for(int u = 0; u < KB; u++) {
+            const int64_t j = jb - 1 - u;
+            const bool ok = j >= j0;
+            const T* rec = m.egeo + (ok ? j : j0) * ER;
+            A[u] = ok ? (R)rec[0] : R(0);
 #pragma unroll
-            for(int k = 0; k < NC; k++) Wj[k] = W[k];
-            mirror_velocity<ND, R>(Wj + 1, rr);
-        } else {
-            const int64_t o = isR ? m.left[f] : Rc;
-#pragma unroll
-            for(int k = 0; k < NC; k++) Wj[k] = (R)m.W[k * C + o];
+            for(int l = 0; l < ND; l++) {
+                c[u][l] = (R)rec[1 + l];
+                rij[u][l] = (R)rec[1 + ND + l];
+            }
+            o[u] = bits_index(rec[1 + 3 * ND]);
         }
 #pragma unroll
-        for(int l = ND; l--;) {
+        for(int u = 0; u < KB; u++) {
+            if(o[u] >= 0) {
 #pragma unroll
-            for(int k = 0; k < NC; k++) {
-                R t = (Wj[k] - W[k]) * c[l] / rnorm - R(0.5) * (W[k] + Wj[k]) * rij[l] / rnorm;
-                delta[l][k] += A * t;
+                for(int k = 0; k < NC; k++) Wj[u][k] = (R)m.W[(int64_t)o[u] * NC + k];
+            } else {
+                // mirror neighbour (:482-489); only a left cell has one, so
+                // rRL = -r_ij
+                R rr[ND];
+#pragma unroll
+                for(int l = 0; l < ND; l++) rr[l] = -rij[u][l];
+#pragma unroll
+                for(int k = 0; k < NC; k++) Wj[u][k] = W[k];
+                mirror_velocity<ND, R>(Wj[u] + 1, rr);
             }
         }
 #pragma unroll
-        for(int k = 0; k < NC; k++) {
-            Wmax[k] = smaxr<R>(Wmax[k], Wj[k]);
-            Wmin[k] = sminr<R>(Wmin[k], Wj[k]);
+        for(int u = 0; u < KB; u++) {
+            if(!(A[u] != R(0))) continue;  // also the padding past j0
+            const R rnorm =
+                    (ND == 3) ? sqrt(rij[u][0] * rij[u][0] + rij[u][1] * rij[u][1] +
+                                     rij[u][ND - 1] * rij[u][ND - 1])
+                              : sqrt(rij[u][0] * rij[u][0] + rij[u][1] * rij[u][1]);
+#pragma unroll
+            for(int l = ND; l--;) {
+#pragma unroll
+                for(int k = 0; k < NC; k++) {
+                    R t;
+                    if(DIFF)
+                        t = (Wj[u][k] - W[k]) * ((c[u][l] - R(0.5) * rij[u][l]) / rnorm);
+                    else
+                        t = (Wj[u][k] - W[k]) * c[u][l] / rnorm -
+                            R(0.5) * (W[k] + Wj[u][k]) * rij[u][l] / rnorm;
+                    delta[l][k] += A[u] * t;
+                }
+            }
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                Wmax[k] = smaxr<R>(Wmax[k], Wj[u][k]);
+                Wmin[k] = sminr<R>(Wmin[k], Wj[u][k]);
+            }
         }
     }
     const R V = (R)m.volume[i];
@@ -271,13 +353,11 @@ __global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restr
 #pragma unroll
     for(int k = 0; k < NC; k++) alpha[k] = R(1);
     for(int64_t j = j1; j-- > j0;) {
-        const int32_t e = m.cf_entry[j];
-        const int32_t f = ent_face(e);
-        if(!((R)m.area[f] != R(0))) continue;
-        const T* dsrc = ent_right(e) ? m.dR : m.dL;
+        const T* rec = m.egeo + j * ER;
+        if(!((R)rec[0] != R(0))) continue;
         R dd[ND];
 #pragma unroll
-        for(int l = 0; l < ND; l++) dd[l] = (R)dsrc[l * F + f];
+        for(int l = 0; l < ND; l++) dd[l] = (R)rec[1 + 2 * ND + l];
 #pragma unroll
         for(int k = 0; k < NC; k++) {
             R dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
@@ -288,9 +368,16 @@ __global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restr
         }
     }
 #pragma unroll
-    for(int l = ND; l--;)
+    for(int l = 0; l < ND; l++)
 #pragma unroll
-        for(int k = 0; k < NC; k++) grad[(int64_t)(l * NC + k) * C + i] = (T)(delta[l][k] * alpha[k]);
+        for(int k = 0; k < NC; k++)
+            grad[((i + gshift) * ND + l) * NC + k] = (T)(delta[l][k] * alpha[k]);
+}
+
+template <int ND, typename T, typename R, bool DIFF>
+__global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restrict__ grad) {
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if(i < m.C) grad_cell<ND, T, R, DIFF>(m, i, grad, 0);
 }
 
 // ------------------------------------------------------- K3 face fluxes --
@@ -299,25 +386,29 @@ template <int ND, typename T, typename R>
 __device__ __forceinline__ void load_grad(const T* grad, int64_t C, int64_t cell,
                                           R (*g)[ND + 2]) {
     const int NC = ND + 2;
+    (void)C;
 #pragma unroll
     for(int l = 0; l < ND; l++)
 #pragma unroll
-        for(int k = 0; k < NC; k++) g[l][k] = (R)grad[(int64_t)(l * NC + k) * C + cell];
+        for(int k = 0; k < NC; k++) g[l][k] = (R)grad[(cell * ND + l) * NC + k];
 }
 
 // MIXED: 0 = FP64 parity solvers, 1 = mixed solvers (and, for a periodic
 // face whose other copy has a lower index, no evaluation: the gather applies
 // that copy's flux with the opposite sign).
+// Face f: its flux goes to dQL[(f + qshift) * NC ...]; the gradients of its
+// cells are read at grad[(cell + gshift) * ND * NC ...]; the solver counters
+// are added to cnt.
 template <int ND, int KIND, int MIXED, typename T, typename R>
-__global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
-                                                        Consts cd, mixed::CF cf,
-                                                        double* __restrict__ dQL,
-                                                        uint8_t* __restrict__ fstatus,
-                                                        unsigned long long* __restrict__ ctr) {
+__device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
+                                          const T* __restrict__ grad, int64_t gshift,
+                                          const Consts& cd, const mixed::CF& cf,
+                                          double* __restrict__ dQL, int64_t qshift,
+                                          uint8_t* __restrict__ fstatus,
+                                          unsigned (&cnt)[CNT_N]) {
     const int NC = ND + 2, P = ND + 1;
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned iters = 0, brent = 0, vac = 0, fail = 0, solved = 0;
-    if(f < m.F) {
+    {
         const int64_t C = m.C, F = m.F;
         const int32_t L = m.left[f], Rc = m.right[f];
         const bool has_r = Rc >= 0;
@@ -337,8 +428,8 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
             for(int l = 0; l < ND; l++) {
                 rr[l] = (R)m.rRL[l * F + f];
                 mcv[l] = (R)m.mc[l * F + f];
-                vl[l] = (R)m.vel[l * C + L];
-                vr[l] = has_r ? (R)m.vel[l * C + Rc] : R(0);
+                vl[l] = (R)m.vel[(int64_t)L * ND + l];
+                vr[l] = has_r ? (R)m.vel[(int64_t)Rc * ND + l] : R(0);
             }
             // VorFace::set_v (src/VorFace.cpp:586-660)
             if(has_r || ND == 2) {
@@ -365,8 +456,8 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
             R WL[NC], WR[NC];
 #pragma unroll
             for(int k = 0; k < NC; k++) {
-                WL[k] = (R)m.W[k * C + L];
-                WR[k] = has_r ? (R)m.W[k * C + Rc] : R(0);
+                WL[k] = (R)m.W[(int64_t)L * NC + k];
+                WR[k] = has_r ? (R)m.W[(int64_t)Rc * NC + k] : R(0);
             }
 #pragma unroll
             for(int l = 0; l < ND; l++) {
@@ -378,14 +469,14 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
             const R gam = (R)cd.gamma;
             if(has_r) {
                 R g[ND][NC], d[ND];
-                load_grad<ND, T, R>(grad, C, Rc, g);
+                load_grad<ND, T, R>(grad, C, Rc + gshift, g);
 #pragma unroll
                 for(int l = 0; l < ND; l++) d[l] = (R)m.dR[l * F + f];
                 predict_side<ND, R>(gam, WR, WL, g, d, rc, true, (R)dt, WRrec);
             }
             {
                 R g[ND][NC], d[ND];
-                load_grad<ND, T, R>(grad, C, L, g);
+                load_grad<ND, T, R>(grad, C, L + gshift, g);
 #pragma unroll
                 for(int l = 0; l < ND; l++) d[l] = (R)m.dL[l * F + f];
                 predict_side<ND, R>(gam, WL, WR, g, d, rc, has_r, (R)dt, WLrec);
@@ -485,26 +576,39 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
             fail = st != 0;
         }
 #pragma unroll
-        for(int k = 0; k < NC; k++) dQL[(int64_t)k * F + f] = out[k];
+        for(int k = 0; k < NC; k++) dQL[(f + qshift) * NC + k] = out[k];
         if(fstatus) fstatus[f] = st;
     }
-    const unsigned cv[CNT_N] = {solved, brent, vac, iters, fail};
-    block_add_counters(ctr, cv);
+    cnt[CNT_NEVAL] += solved;
+    cnt[CNT_NBRENT] += brent;
+    cnt[CNT_NVACUUM] += vac;
+    cnt[CNT_NITER] += iters;
+    cnt[CNT_NFAIL] += fail;
+}
+
+template <int ND, int KIND, int MIXED, typename T, typename R>
+__global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
+                                                        Consts cd, mixed::CF cf,
+                                                        double* __restrict__ dQL,
+                                                        uint8_t* __restrict__ fstatus,
+                                                        unsigned long long* __restrict__ ctr) {
+    const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
+    if(f < m.F) face_item<ND, KIND, MIXED, T, R>(m, f, grad, 0, cd, cf, dQL, 0, fstatus, cnt);
+    block_add_counters(ctr, cnt);
 }
 
 // ------------------------------------------------------------ K4 gather --
 
+// Cell i: dQ[i] = its entries' dQL[(f + qshift)] in ascending face order.
 template <int ND, int MIXED>
-__global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
-                                                     const int64_t* __restrict__ goff,
-                                                     const int32_t* __restrict__ gent,
-                                                     const uint8_t* __restrict__ ghost,
-                                                     const int32_t* __restrict__ twin,
-                                                     const double* __restrict__ dQL,
-                                                     double* __restrict__ dQ) {
+__device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict__ goff,
+                                            const int32_t* __restrict__ gent,
+                                            const uint8_t* __restrict__ ghost,
+                                            const int32_t* __restrict__ twin,
+                                            const double* __restrict__ dQL, int64_t qshift,
+                                            double* __restrict__ dQ) {
     const int NC = ND + 2;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= C) return;
     double acc[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) acc[k] = 0.0;
@@ -522,33 +626,45 @@ __global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
         }
 #pragma unroll
         for(int k = 0; k < NC; k++) {
-            const double x = dQL[(int64_t)k * F + f];
+            const double x = dQL[((int64_t)f + qshift) * NC + k];
             acc[k] += neg ? -x : x;
         }
     }
 #pragma unroll
-    for(int k = 0; k < NC; k++) dQ[(int64_t)k * C + i] = acc[k];
+    for(int k = 0; k < NC; k++) dQ[i * NC + k] = acc[k];
+}
+
+template <int ND, int MIXED>
+__global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
+                                                     const int64_t* __restrict__ goff,
+                                                     const int32_t* __restrict__ gent,
+                                                     const uint8_t* __restrict__ ghost,
+                                                     const int32_t* __restrict__ twin,
+                                                     const double* __restrict__ dQL,
+                                                     double* __restrict__ dQ) {
+    (void)F;
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if(i < C) gather_cell<ND, MIXED>(i, goff, gent, ghost, twin, dQL, 0, dQ);
 }
 
 // ------------------------------------------------------------ K5 update --
 
-// GasParticle::update_Q then get_W (use_energy = true); T: storage of W.
-template <int ND, typename T>
-__global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, double* __restrict__ Q,
-                                                     double* __restrict__ dQ,
-                                                     const double* __restrict__ volume,
-                                                     T* __restrict__ Wout,
-                                                     uint8_t* __restrict__ cstatus,
-                                                     unsigned long long* __restrict__ ctr) {
+// GasParticle::update_Q then get_W (use_energy = true).
+// Cell i; returns 1 when m or e went negative.
+template <int ND>
+__device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
+                                                double* __restrict__ Q, double* __restrict__ dQ,
+                                                const double* __restrict__ volume,
+                                                double* __restrict__ Wout,
+                                                uint8_t* __restrict__ cstatus) {
     const int NC = ND + 2, P = ND + 1;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned fail = 0;
-    if(i < C) {
+    {
         double q[NC], w[NC];
 #pragma unroll
         for(int k = 0; k < NC; k++) {
-            q[k] = Q[k * C + i] - dQ[k * C + i];
-            dQ[k * C + i] = 0.0;
+            q[k] = Q[i * NC + k] - dQ[i * NC + k];
+            dQ[i * NC + k] = 0.0;
         }
         uint8_t st = (q[0] < 0. || q[P] < 0.) ? ST_UPDATEQ : 0;
         const double g = cd.gamma, V = volume[i];
@@ -572,12 +688,25 @@ __global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, doubl
         }
 #pragma unroll
         for(int k = 0; k < NC; k++) {
-            Q[k * C + i] = q[k];
-            Wout[k * C + i] = (T)w[k];
+            Q[i * NC + k] = q[k];
+            Wout[i * NC + k] = w[k];
         }
         if(cstatus) cstatus[i] = st;
         fail = st != 0;
     }
+    return fail;
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, double* __restrict__ Q,
+                                                     double* __restrict__ dQ,
+                                                     const double* __restrict__ volume,
+                                                     double* __restrict__ Wout,
+                                                     uint8_t* __restrict__ cstatus,
+                                                     unsigned long long* __restrict__ ctr) {
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned fail = 0;
+    if(i < C) fail = update_cell<ND>(cd, i, Q, dQ, volume, Wout, cstatus);
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
     block_add_counters(ctr, cv);
 }

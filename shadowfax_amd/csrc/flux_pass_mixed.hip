@@ -1,10 +1,15 @@
 // flux_pass_mixed.hip -- mixed instantiation of the flux-pass kernels
-// (flux_pass_impl.h): FP32 storage of cell states, gradients and face
-// geometry, FP32 reconstruction and limiting, the mixed Riemann solvers
-// (riemann_mixed_impl.h), FP64 face fluxes and FP64 accumulation.  Each
-// periodic boundary face is evaluated once and applied with opposite signs
-// to its two cells, so mass, momentum and energy are conserved to FP64
-// round-off (the reference evaluates both copies, src/VorFace.cpp:913-948).
+// (flux_pass_impl.h): FP32 storage of generator velocities, gradients and
+// face geometry (half the bytes of those gathers); the primitive states stay
+// FP64 (a face flux driven by the small difference of two large pressures,
+// as between the hot cells of a Sedov blast, cannot be recovered from
+// FP32-rounded states: rounding the states alone gave 6e-6 face errors);
+// gradient estimation (in the closure-free difference form), limiting and
+// the MUSCL-Hancock predictor evaluated in FP64 registers; the mixed Riemann
+// solvers (riemann_mixed_impl.h); FP64 face fluxes and FP64 accumulation.  Each periodic boundary face is evaluated
+// once and applied with opposite signs to its two cells, so mass, momentum
+// and energy are conserved to FP64 round-off (the reference evaluates both
+// copies, src/VorFace.cpp:913-948).
 #include "flux_pass_impl.h"
 #include "sfx_launch.h"
 
@@ -13,14 +18,27 @@ namespace sfx {
 static inline dim3 grid_for(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
 template <>
+hipError_t launch_entry_geometry<float>(int ndim, const flux::MeshView<float>& m, int64_t E,
+                                     float* egeo, hipStream_t s) {
+    if(E <= 0) return hipSuccess;
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::entry_geometry_kernel<3, float>), grid_for(E, 256), dim3(256), 0,
+                           s, m, E, egeo);
+    else
+        hipLaunchKernelGGL((flux::entry_geometry_kernel<2, float>), grid_for(E, 256), dim3(256), 0,
+                           s, m, E, egeo);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t launch_gradients<float>(int ndim, const flux::MeshView<float>& m, float* grad,
                                    hipStream_t s) {
     if(m.C <= 0) return hipSuccess;
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gradient_kernel<3, float, float>), grid_for(m.C, 256),
+        hipLaunchKernelGGL((flux::gradient_kernel<3, float, double, true>), grid_for(m.C, 256),
                            dim3(256), 0, s, m, grad);
     else
-        hipLaunchKernelGGL((flux::gradient_kernel<2, float, float>), grid_for(m.C, 256),
+        hipLaunchKernelGGL((flux::gradient_kernel<2, float, double, true>), grid_for(m.C, 256),
                            dim3(256), 0, s, m, grad);
     return hipGetLastError();
 }
@@ -32,7 +50,7 @@ hipError_t launch_face_fluxes<float>(int ndim, int kind, const flux::MeshView<fl
     if(m.F <= 0) return hipSuccess;
     const mixed::CF cf = mixed::make_cf(c);
 #define SFX_FACE(ND, K)                                                                   \
-    hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, float>), grid_for(m.F, 256), \
+    hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, double>), grid_for(m.F, 256), \
                        dim3(256), 0, s, m, grad, c, cf, dQL, fstatus, ctr)
     if(ndim == 3 && kind == 0) SFX_FACE(3, 0);
     else if(ndim == 3) SFX_FACE(3, 1);
@@ -55,18 +73,5 @@ hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* go
     return hipGetLastError();
 }
 
-template <>
-hipError_t launch_update<float>(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
-                                const double* volume, float* W, uint8_t* cstatus,
-                                unsigned long long* ctr, hipStream_t s) {
-    if(C <= 0) return hipSuccess;
-    if(ndim == 3)
-        hipLaunchKernelGGL((flux::update_kernel<3, float>), grid_for(C, 256), dim3(256), 0, s, c,
-                           C, Q, dQ, volume, W, cstatus, ctr);
-    else
-        hipLaunchKernelGGL((flux::update_kernel<2, float>), grid_for(C, 256), dim3(256), 0, s, c,
-                           C, Q, dQ, volume, W, cstatus, ctr);
-    return hipGetLastError();
-}
 
 }  // namespace sfx

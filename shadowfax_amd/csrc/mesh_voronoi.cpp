@@ -1,12 +1,11 @@
-// mesh_voronoi.cpp -- host construction of periodic Voronoi meshes and of the
-// per-face geometry the flux-pass kernels read.
+// mesh_voronoi.cpp -- host construction of 2D periodic Voronoi meshes.
 //
 // The reference builds its faces in VorTess::construct (Shadowfax
 // src/VorTess.cpp:285-396) from a Delaunay tessellation that needs
 // Boost.Multiprecision.  Here each cell is built independently by clipping a
-// box with the bisector half-planes of its neighbours (found through a grid
-// hash over the periodic box); the face list then follows the reference's
-// contract:
+// box with the bisector half-planes of its neighbours, closest first
+// (candidates from a grid hash over the periodic box); the face list then
+// follows the reference's contract (mesh_assemble.h):
 //   * cells in generator order; each cell's faces in counter-clockwise order;
 //   * a face is created by the first of its two cells to be visited (its
 //     "left"); a neighbour that is a periodic copy (ghost, reference id 1,
@@ -21,93 +20,100 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
-#include <cstring>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
+#include "mesh_assemble.h"
 #include "sfx_mesh_host.h"
 
 namespace sfx {
 namespace {
 
-struct Cand {
-    double x, y;      // position of the (possibly shifted) generator
-    int32_t j;        // generator index
-    double sx, sy;    // periodic shift applied
+struct Cand2 {
+    double x, y;  // position of the (possibly shifted) generator
+    double d2;    // squared distance to the cell's generator
+    int32_t j;    // generator index
+    int8_t s[2];  // periodic shift in box lengths
 };
 
-struct Edge {  // polygon edge from vertex k to k+1, created by candidate c
-    int32_t c;
-};
-
-struct Cell2 {
-    std::vector<double> vx, vy;  // CCW vertices
-    std::vector<int32_t> lab;    // lab[k]: candidate creating edge k -> k+1
-    std::vector<Cand> cand;
-};
-
-// clip convex polygon (CCW) by {x : (x - m).u <= 0}
-void clip(Cell2& P, const Cand& q, double xi, double yi, int32_t label) {
-    const double ux = q.x - xi, uy = q.y - yi;
-    const double mx = 0.5 * (q.x + xi), my = 0.5 * (q.y + yi);
-    const size_t n = P.vx.size();
-    std::vector<double> s(n);
-    bool any_out = false;
-    for(size_t k = 0; k < n; k++) {
-        s[k] = (P.vx[k] - mx) * ux + (P.vy[k] - my) * uy;
-        any_out |= s[k] > 0.0;
-    }
-    if(!any_out) return;
-    std::vector<double> nx, ny;
+// Convex polygon, counter-clockwise; lab[k]: candidate whose bisector made
+// the edge from vertex k to k+1 (-1: the initial box).
+struct Poly2 {
+    std::vector<double> vx, vy;
+    std::vector<int32_t> lab;
+    std::vector<double> s, nx, ny;  // scratch
     std::vector<int32_t> nl;
-    nx.reserve(n + 2);
-    ny.reserve(n + 2);
-    nl.reserve(n + 2);
-    for(size_t k = 0; k < n; k++) {
-        const size_t k1 = (k + 1) % n;
-        const bool in0 = s[k] <= 0.0, in1 = s[k1] <= 0.0;
-        if(in0) {
-            nx.push_back(P.vx[k]);
-            ny.push_back(P.vy[k]);
-            nl.push_back(P.lab[k]);
+
+    double rmax2(double xi, double yi) const {
+        double r = 0.0;
+        for(size_t k = 0; k < vx.size(); k++) {
+            double dx = vx[k] - xi, dy = vy[k] - yi;
+            r = std::max(r, dx * dx + dy * dy);
         }
-        if(in0 != in1) {
-            const double t = s[k] / (s[k] - s[k1]);
-            nx.push_back(P.vx[k] + t * (P.vx[k1] - P.vx[k]));
-            ny.push_back(P.vy[k] + t * (P.vy[k1] - P.vy[k]));
-            // leaving the half-plane: the new edge runs along the bisector
-            nl.push_back(in0 ? label : P.lab[k]);
-        }
+        return r;
     }
-    P.vx.swap(nx);
-    P.vy.swap(ny);
-    P.lab.swap(nl);
-}
+
+    // clip by {x : (x - m).u <= 0}, m = (c + q)/2, u = q - c
+    bool clip(double xi, double yi, const Cand2& q, int32_t label) {
+        const double ux = q.x - xi, uy = q.y - yi;
+        const double mx = 0.5 * (q.x + xi), my = 0.5 * (q.y + yi);
+        const size_t n = vx.size();
+        s.resize(n);
+        bool any_out = false;
+        for(size_t k = 0; k < n; k++) {
+            s[k] = (vx[k] - mx) * ux + (vy[k] - my) * uy;
+            any_out |= s[k] > 0.0;
+        }
+        if(!any_out) return false;
+        nx.clear();
+        ny.clear();
+        nl.clear();
+        for(size_t k = 0; k < n; k++) {
+            const size_t k1 = (k + 1) % n;
+            const bool in0 = s[k] <= 0.0, in1 = s[k1] <= 0.0;
+            if(in0) {
+                nx.push_back(vx[k]);
+                ny.push_back(vy[k]);
+                nl.push_back(lab[k]);
+            }
+            if(in0 != in1) {
+                const double t = s[k] / (s[k] - s[k1]);
+                nx.push_back(vx[k] + t * (vx[k1] - vx[k]));
+                ny.push_back(vy[k] + t * (vy[k1] - vy[k]));
+                // leaving the half-plane: the new edge runs along the bisector
+                nl.push_back(in0 ? label : lab[k]);
+            }
+        }
+        vx.swap(nx);
+        vy.swap(ny);
+        lab.swap(nl);
+        return true;
+    }
+};
 
 struct Grid2 {
-    double Lx, Ly, hx, hy;
-    int nx, ny;
+    int n[2];
+    double L[2], h[2];
     std::vector<int64_t> off;
     std::vector<int32_t> idx;
 };
 
 Grid2 make_grid(int64_t C, const double* x, const double* y, double Lx, double Ly) {
     Grid2 g;
-    g.Lx = Lx;
-    g.Ly = Ly;
-    const double h = std::sqrt(Lx * Ly / (double)std::max<int64_t>(C, 1));
-    g.nx = std::max(1, (int)std::floor(Lx / h));
-    g.ny = std::max(1, (int)std::floor(Ly / h));
-    g.hx = Lx / g.nx;
-    g.hy = Ly / g.ny;
-    const int64_t nb = (int64_t)g.nx * g.ny;
+    g.L[0] = Lx;
+    g.L[1] = Ly;
+    const double hh = std::sqrt(Lx * Ly * 2.0 / (double)std::max<int64_t>(C, 1));
+    for(int d = 0; d < 2; d++) {
+        g.n[d] = std::max(1, (int)std::floor(g.L[d] / hh));
+        g.h[d] = g.L[d] / g.n[d];
+    }
+    const int64_t nb = (int64_t)g.n[0] * g.n[1];
     g.off.assign(nb + 1, 0);
-    std::vector<int32_t> b(C);
+    std::vector<int64_t> b(C);
     for(int64_t i = 0; i < C; i++) {
-        int bx = std::min(g.nx - 1, std::max(0, (int)std::floor(x[i] / g.hx)));
-        int by = std::min(g.ny - 1, std::max(0, (int)std::floor(y[i] / g.hy)));
-        b[i] = by * g.nx + bx;
+        int bx = std::min(g.n[0] - 1, std::max(0, (int)std::floor(x[i] / g.h[0])));
+        int by = std::min(g.n[1] - 1, std::max(0, (int)std::floor(y[i] / g.h[1])));
+        b[i] = (int64_t)by * g.n[0] + bx;
         g.off[b[i] + 1]++;
     }
     for(int64_t k = 0; k < nb; k++) g.off[k + 1] += g.off[k];
@@ -117,59 +123,97 @@ Grid2 make_grid(int64_t C, const double* x, const double* y, double Lx, double L
     return g;
 }
 
-// Build cell i: candidates ring by ring until the security radius holds.
-bool build_cell(const Grid2& g, int64_t C, const double* x, const double* y, int64_t i,
-                Cell2& P) {
+// Build cell i: candidates ring by ring (Chebyshev distance in grid cells),
+// closest first, until every vertex lies within half the distance covered
+// by the rings gathered so far (the security-radius test).
+bool build_cell(const Grid2& g, const double* x, const double* y, int64_t i, Poly2& P,
+                std::vector<Cand2>& cand, CellFaces<2>& out) {
     const double xi = x[i], yi = y[i];
-    const int bx = std::min(g.nx - 1, std::max(0, (int)std::floor(xi / g.hx)));
-    const int by = std::min(g.ny - 1, std::max(0, (int)std::floor(yi / g.hy)));
-    const double half = 0.5 * std::max(g.Lx, g.Ly) * 2.0;
+    const double c[2] = {xi, yi};
+    int gc[2];
+    for(int d = 0; d < 2; d++)
+        gc[d] = std::min(g.n[d] - 1, std::max(0, (int)std::floor(c[d] / g.h[d])));
+    const double half = std::max(g.L[0], g.L[1]);
     P.vx = {xi - half, xi + half, xi + half, xi - half};
     P.vy = {yi - half, yi - half, yi + half, yi + half};
     P.lab = {-1, -1, -1, -1};
-    P.cand.clear();
-    const int rmax = std::max(g.nx, g.ny) + 2;
-    auto visit = [&](int dx, int dy) {
-        int cx = bx + dx, cy = by + dy;
-        int wx = ((cx % g.nx) + g.nx) % g.nx, wy = ((cy % g.ny) + g.ny) % g.ny;
-        double sx = (double)((cx - wx) / g.nx) * g.Lx;
-        double sy = (double)((cy - wy) / g.ny) * g.Ly;
-        const int64_t b = (int64_t)wy * g.nx + wx;
-        for(int64_t k = g.off[b]; k < g.off[b + 1]; k++) {
-            const int32_t j = g.idx[k];
-            if(j == i && sx == 0.0 && sy == 0.0) continue;
-            Cand q{x[j] + sx, y[j] + sy, j, sx, sy};
-            P.cand.push_back(q);
-            clip(P, q, xi, yi, (int32_t)P.cand.size() - 1);
-        }
-    };
-    for(int r = 0; r <= rmax; r++) {
-        if(r == 0) {
-            visit(0, 0);
-        } else {
-            for(int d = -r; d <= r; d++) {
-                visit(d, -r);
-                visit(d, r);
+    cand.clear();
+    size_t next = 0;
+    double r2 = P.rmax2(xi, yi);
+    for(int k = 0; k < 64; k++) {
+        for(int oy = -k; oy <= k; oy++)
+            for(int ox = -k; ox <= k; ox++) {
+                if(std::max(std::abs(ox), std::abs(oy)) != k) continue;
+                const int o[2] = {ox, oy};
+                int cc[2];
+                int8_t sh[2];
+                for(int d = 0; d < 2; d++) {
+                    int t = gc[d] + o[d];
+                    int w = (int)std::floor((double)t / g.n[d]);
+                    cc[d] = t - w * g.n[d];
+                    sh[d] = (int8_t)w;
+                }
+                const int64_t cell = (int64_t)cc[1] * g.n[0] + cc[0];
+                for(int64_t p = g.off[cell]; p < g.off[cell + 1]; p++) {
+                    const int32_t jj = g.idx[p];
+                    if(jj == i && sh[0] == 0 && sh[1] == 0) continue;
+                    Cand2 q;
+                    q.x = x[jj] + sh[0] * g.L[0];
+                    q.y = y[jj] + sh[1] * g.L[1];
+                    const double dx = q.x - xi, dy = q.y - yi;
+                    q.d2 = dx * dx + dy * dy;
+                    q.j = jj;
+                    q.s[0] = sh[0];
+                    q.s[1] = sh[1];
+                    cand.push_back(q);
+                }
             }
-            for(int d = -r + 1; d <= r - 1; d++) {
-                visit(-r, d);
-                visit(r, d);
+        std::sort(cand.begin() + next, cand.end(), [](const Cand2& a, const Cand2& b) {
+            return a.d2 < b.d2 || (a.d2 == b.d2 && a.j < b.j);
+        });
+        double D = 1e300;
+        for(int d = 0; d < 2; d++) {
+            D = std::min(D, c[d] - (gc[d] - k) * g.h[d]);
+            D = std::min(D, (gc[d] + k + 1) * g.h[d] - c[d]);
+        }
+        const double D2 = D * D;
+        while(next < cand.size() && cand[next].d2 < D2 && cand[next].d2 < 4.0 * r2) {
+            if(P.clip(xi, yi, cand[next], (int32_t)next)) r2 = P.rmax2(xi, yi);
+            next++;
+        }
+        if(4.0 * r2 <= D2) {
+            const size_t n = P.vx.size();
+            for(size_t e = 0; e < n; e++)
+                if(P.lab[e] < 0) return false;  // box edge survived
+            // start at the edge whose neighbour direction has the smallest
+            // angle in [-pi, pi): a deterministic first face
+            size_t best = 0;
+            double ba = 1e300;
+            for(size_t e = 0; e < n; e++) {
+                const Cand2& q = cand[P.lab[e]];
+                double a = std::atan2(q.y - yi, q.x - xi);
+                if(a < ba) {
+                    ba = a;
+                    best = e;
+                }
             }
+            out.clear();
+            for(size_t t = 0; t < n; t++) {
+                const size_t e = (best + t) % n, e1 = (e + 1) % n;
+                const Cand2& q = cand[P.lab[e]];
+                out.j.push_back(q.j);
+                out.sh.push_back(q.s[0]);
+                out.sh.push_back(q.s[1]);
+                out.vx.push_back(P.vx[e]);
+                out.vx.push_back(P.vy[e]);
+                out.vx.push_back(P.vx[e1]);
+                out.vx.push_back(P.vy[e1]);
+                out.voff.push_back((int32_t)(out.vx.size() / 2));
+            }
+            return true;
         }
-        // security radius: generators outside ring r are at least r*h away
-        double R2 = 0.0;
-        for(size_t k = 0; k < P.vx.size(); k++) {
-            double dx = P.vx[k] - xi, dy = P.vy[k] - yi;
-            R2 = std::max(R2, dx * dx + dy * dy);
-        }
-        const double reach = r * std::min(g.hx, g.hy);
-        bool boxed = false;
-        for(int32_t l : P.lab) boxed |= l < 0;
-        if(!boxed && 4.0 * R2 <= reach * reach) return true;
     }
-    for(int32_t l : P.lab)
-        if(l < 0) return false;
-    return true;
+    return false;
 }
 
 }  // namespace
@@ -178,184 +222,32 @@ int voronoi2d_periodic(int64_t C, const double* pos, double Lx, double Ly, HostM
                        int nthreads) {
     const double* x = pos;
     const double* y = pos + C;
+    const double L[2] = {Lx, Ly};
+    for(int64_t i = 0; i < C; i++)
+        if(!(x[i] >= 0.0 && x[i] < Lx && y[i] >= 0.0 && y[i] < Ly)) return -3;
     m = HostMesh();
-    m.ndim = 2;
-    m.C = C;
-    Grid2 g = make_grid(C, x, y, Lx, Ly);
-    std::vector<Cell2> cells(C);
     if(nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-    std::atomic<int64_t> next(0);
+    Grid2 g = make_grid(C, x, y, Lx, Ly);
+    std::vector<CellFaces<2>> cells(C);
     std::atomic<bool> ok(true);
-    auto work = [&]() {
-        for(;;) {
-            int64_t i0 = next.fetch_add(256);
-            if(i0 >= C) break;
-            for(int64_t i = i0; i < std::min(C, i0 + 256); i++) {
-                if(!build_cell(g, C, x, y, i, cells[i])) ok = false;
-                // rotate so the first edge is the one whose neighbour direction
-                // has the smallest angle in [-pi, pi): a deterministic start
-                Cell2& P = cells[i];
-                const size_t n = P.vx.size();
-                size_t best = 0;
-                double ba = 1e300;
-                for(size_t k = 0; k < n; k++) {
-                    const Cand& q = P.cand[P.lab[k]];
-                    double a = std::atan2(q.y - y[i], q.x - x[i]);
-                    if(a < ba) {
-                        ba = a;
-                        best = k;
-                    }
-                }
-                std::rotate(P.vx.begin(), P.vx.begin() + best, P.vx.end());
-                std::rotate(P.vy.begin(), P.vy.begin() + best, P.vy.end());
-                std::rotate(P.lab.begin(), P.lab.begin() + best, P.lab.end());
-            }
-        }
-    };
-    std::vector<std::thread> th;
-    for(int t = 0; t < nthreads; t++) th.emplace_back(work);
-    for(auto& t : th) t.join();
-    if(!ok) return -1;
-
-    // faces, in cell order; a face belongs to the first real cell visiting it
-    m.pos.assign(pos, pos + 2 * C);
-    m.cf_off.assign(C + 1, 0);
-    std::vector<int64_t> first_face(C + 1, 0);
-    for(int64_t i = 0; i < C; i++) {
-        const Cell2& P = cells[i];
-        int64_t nnew = 0;
-        for(int32_t l : P.lab) {
-            const Cand& q = P.cand[l];
-            const bool ghost = q.sx != 0.0 || q.sy != 0.0;
-            if(ghost || q.j > i) nnew++;
-        }
-        first_face[i + 1] = first_face[i] + nnew;
-        m.cf_off[i + 1] = m.cf_off[i] + (int64_t)P.lab.size();
-    }
-    const int64_t F = first_face[C];
-    m.F = F;
-    m.left.resize(F);
-    m.right.resize(F);
-    m.area.resize(F);
-    m.midpoint.resize(2 * F);
-    m.shift.resize(2 * F);
-    m.ghost.resize(F);
-    m.twin.assign(F, -1);
-    m.verts.resize(4 * F);
-    m.cf_face.resize(m.cf_off[C]);
-    // face index of (creator i, neighbour j) for real-real faces
-    std::vector<std::unordered_map<int32_t, int64_t>> owned(C);
-    for(int64_t i = 0; i < C; i++) {
-        const Cell2& P = cells[i];
-        int64_t f = first_face[i];
-        const size_t n = P.vx.size();
-        for(size_t k = 0; k < n; k++) {
-            const Cand& q = P.cand[P.lab[k]];
-            const bool ghost = q.sx != 0.0 || q.sy != 0.0;
-            if(!(ghost || q.j > i)) continue;
-            const size_t k1 = (k + 1) % n;
-            m.left[f] = (int32_t)i;
-            m.right[f] = q.j;
-            m.ghost[f] = ghost ? 1 : 0;
-            m.shift[f] = q.sx;
-            m.shift[F + f] = q.sy;
-            const double v0x = P.vx[k], v0y = P.vy[k], v1x = P.vx[k1], v1y = P.vy[k1];
-            m.verts[4 * f + 0] = v0x;
-            m.verts[4 * f + 1] = v0y;
-            m.verts[4 * f + 2] = v1x;
-            m.verts[4 * f + 3] = v1y;
-            double d0 = v0x - v1x, d1 = v0y - v1y;
-            double area = std::sqrt(d0 * d0 + d1 * d1);
-            if(std::sqrt((v0x - v1x) * (v0x - v1x) + (v0y - v1y) * (v0y - v1y)) < 1.e-13)
-                area = 0.;
-            m.area[f] = area;
-            m.midpoint[f] = 0.5 * (v0x + v1x);
-            m.midpoint[F + f] = 0.5 * (v0y + v1y);
-            if(!ghost) owned[i][q.j] = f;
-            f++;
-        }
-    }
-    for(int64_t i = 0; i < C; i++) {
-        const Cell2& P = cells[i];
-        int64_t f = first_face[i];
-        for(size_t k = 0; k < P.lab.size(); k++) {
-            const Cand& q = P.cand[P.lab[k]];
-            const bool ghost = q.sx != 0.0 || q.sy != 0.0;
-            int64_t face;
-            if(ghost || q.j > i) {
-                face = f++;
-            } else {
-                auto it = owned[q.j].find((int32_t)i);
-                if(it == owned[q.j].end()) return -2;  // inconsistent tessellation
-                face = it->second;
-            }
-            m.cf_face[m.cf_off[i] + k] = (int32_t)face;
-        }
-    }
-    // twins of periodic faces: (i -> j, s) pairs with (j -> i, -s)
     {
-        std::unordered_map<uint64_t, std::vector<int64_t>> key;
-        for(int64_t f = 0; f < F; f++) {
-            if(!m.ghost[f]) continue;
-            uint64_t a = (uint64_t)std::min(m.left[f], m.right[f]);
-            uint64_t b = (uint64_t)std::max(m.left[f], m.right[f]);
-            key[(a << 32) | b].push_back(f);
-        }
-        for(auto& kv : key) {
-            auto& v = kv.second;
-            for(size_t p = 0; p < v.size(); p++)
-                for(size_t q = 0; q < v.size(); q++) {
-                    int64_t f = v[p], h = v[q];
-                    if(f == h || m.twin[f] >= 0 || m.twin[h] >= 0) continue;
-                    if(m.left[f] == m.right[h] && m.right[f] == m.left[h] &&
-                       m.shift[f] == -m.shift[h] && m.shift[F + f] == -m.shift[F + h]) {
-                        m.twin[f] = h;
-                        m.twin[h] = f;
-                    }
+        std::atomic<int64_t> next(0);
+        std::vector<std::thread> th;
+        for(int t = 0; t < nthreads; t++)
+            th.emplace_back([&]() {
+                Poly2 P;
+                std::vector<Cand2> cand;
+                for(;;) {
+                    int64_t i0 = next.fetch_add(256);
+                    if(i0 >= C) break;
+                    for(int64_t i = i0; i < std::min(C, i0 + 256); i++)
+                        if(!build_cell(g, x, y, i, P, cand, cells[i])) ok = false;
                 }
-        }
+            });
+        for(auto& t : th) t.join();
     }
-    // cell volume and centroid from the faces' stored vertices
-    m.volume.resize(C);
-    m.centroid.resize(2 * C);
-    for(int64_t i = 0; i < C; i++) {
-        const double cx = x[i], cy = y[i];
-        const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
-        // VorCell::calculate_volume, 2D (src/VorCell.cpp:375-385): faces in reverse
-        double V = 0.;
-        for(int64_t j = j1; j-- > j0;) {
-            const int64_t f = m.cf_face[j];
-            double ax = m.verts[4 * f + 0] - cx, ay = m.verts[4 * f + 1] - cy;
-            double bxx = m.verts[4 * f + 2] - cx, byy = m.verts[4 * f + 3] - cy;
-            // a face referenced from its right cell lies in the same frame
-            // (only real-real faces are shared)
-            V += 0.5 * std::fabs(ax * byy - ay * bxx);
-        }
-        m.volume[i] = V;
-        // VorCell::calculate_centroid, 2D (src/VorCell.cpp:297-315)
-        double Ct0 = 0, Ct1 = 0, At = 0;
-        for(int64_t j = j0; j < j1; j++) {
-            const int64_t f = m.cf_face[j];
-            const double p0x = m.verts[4 * f + 0], p0y = m.verts[4 * f + 1];
-            const double p1x = m.verts[4 * f + 2], p1y = m.verts[4 * f + 3];
-            double c0 = (p0x + p1x + cx) / 3.;
-            double c1 = (p0y + p1y + cy) / 3.;
-            double ax = p0x - cx, ay = p0y - cy, bxx = p1x - cx, byy = p1y - cy;
-            double A = 0.5 * std::fabs(ax * byy - ay * bxx);
-            Ct0 += A * c0;
-            Ct1 += A * c1;
-            At += A;
-        }
-        m.centroid[i] = Ct0 / At;
-        m.centroid[C + i] = Ct1 / At;
-    }
-    return 0;
+    if(!ok) return -1;
+    return assemble<2>(C, pos, L, cells, m, nthreads);
 }
 
-}  // namespace sfx
-
-namespace sfx {
-int voronoi3d_periodic(int64_t, const double*, double, double, double, HostMesh&, int) {
-    return -4;
-}
 }  // namespace sfx

@@ -63,4 +63,16 @@ __device__ __forceinline__ void block_add_counters(unsigned long long* ctr,
         atomicAdd(ctr + (blockIdx.x % kCtrSlots) * CNT_N + threadIdx.x, s_ctr[threadIdx.x]);
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin
+// (dispatch index b runs on XCD b % 8) and every XCD has its own L2.  This
+// maps the dispatch index to a logical block index such that each XCD works
+// through one contiguous range of blocks: kernels whose neighbouring items
+// share data (cells and the faces between them) then find it in their own
+// L2.  A bijection on [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t nb = gridDim.x, b = blockIdx.x;
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, t = b >> 3;
+    return x * q + (x < r ? x : r) + t;
+}
+
 }  // namespace sfx

@@ -33,15 +33,17 @@ template <typename T>
 struct MeshView;
 }
 template <typename T>
+hipError_t launch_entry_geometry(int ndim, const flux::MeshView<T>& m, int64_t E, T* egeo,
+                                 hipStream_t s);
+template <typename T>
 hipError_t launch_gradients(int ndim, const flux::MeshView<T>& m, T* grad, hipStream_t s);
 template <typename T>
 hipError_t launch_face_fluxes(int ndim, int kind, const flux::MeshView<T>& m, const T* grad,
                               const Consts& c, double* dQL, uint8_t* fstatus,
                               unsigned long long* ctr, hipStream_t s);
-template <typename T>
 hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
-                         const double* volume, T* W, uint8_t* cstatus, unsigned long long* ctr,
-                         hipStream_t s);
+                         const double* volume, double* W, uint8_t* cstatus,
+                         unsigned long long* ctr, hipStream_t s);
 hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* goff,
                               const int32_t* gent, const double* dQL, double* dQ,
                               hipStream_t s);

@@ -6,9 +6,11 @@
 // x86-64 baseline has no instruction for anyway.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "flux_pass_impl.h"
@@ -34,7 +36,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     flux::MeshView<T> v;
     v.C = m->C;
     v.F = m->F;
-    v.W = (const T*)m->W;
+    v.W = m->W;
     v.vel = (const T*)m->vel;
     v.mass = m->mass;
     v.total_area = m->total_area;
@@ -53,6 +55,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.rcent = (const T*)m->rcent;
     v.cf_off = m->cf_off;
     v.cf_entry = m->cf_entry;
+    v.egeo = (const T*)m->egeo;
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
     return v;
@@ -67,9 +70,9 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
     if(m->C < 0 || m->F < 0) return set_error(SFX_ERR_ARG, "%s: negative size", who);
     if(m->C > INT32_MAX || m->F > (INT32_MAX >> 1))
         return set_error(SFX_ERR_ARG, "%s: mesh too large for 32-bit indices", who);
-    if(gradients && (!m->W || !m->area || !m->rRL || !m->mc || !m->dL || !m->dR ||
-                     !m->volume || !m->cf_off || !m->cf_entry || !m->left || !m->right))
-        return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass", who);
+    if(gradients && (!m->W || !m->volume || !m->cf_off || !m->egeo))
+        return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass (W, volume, "
+                         "cf_off, egeo)", who);
     if(faces && (!m->W || !m->vel || !m->mass || !m->total_area || !m->dt || !m->left ||
                  !m->right || !m->ghost || !m->area || !m->dL || !m->dR || !m->rRL ||
                  !m->mc || !m->rcent))
@@ -98,7 +101,6 @@ int sfx_voronoi_periodic(int ndim, int64_t C, const double* pos, const double* b
         rc = voronoi3d_periodic(C, pos, box[0], box[1], box[2], m, nthreads);
     else
         return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: ndim must be 2 or 3");
-    if(rc == -4) return set_error(SFX_ERR_UNSUPPORTED, "3D tessellation not available");
     if(rc != 0)
         return set_error(SFX_ERR_ARG, "sfx_voronoi_periodic: tessellation failed (%d)", rc);
     out->ndim = ndim;
@@ -116,7 +118,9 @@ int sfx_voronoi_periodic(int ndim, int64_t C, const double* pos, const double* b
     out->twin = dup(m.twin);
     out->cf_off = dup(m.cf_off);
     out->cf_face = dup(m.cf_face);
-    if(!out->pos || !out->left || !out->cf_face) {
+    out->fv_off = dup(m.fv_off);
+    out->fverts = dup(m.fverts);
+    if(!out->pos || !out->left || !out->cf_face || !out->fv_off || !out->fverts) {
         sfx_host_mesh_free(out);
         return set_error(SFX_ERR_NOMEM, "sfx_voronoi_periodic: out of host memory");
     }
@@ -127,7 +131,7 @@ void sfx_host_mesh_free(sfx_host_mesh* m) {
     if(!m) return;
     void* ps[] = {m->pos,      m->volume, m->centroid, m->left,  m->right,
                   m->area,     m->midpoint, m->shift,  m->ghost, m->twin,
-                  m->cf_off,   m->cf_face};
+                  m->cf_off,   m->cf_face, m->fv_off, m->fverts};
     for(void* p : ps) std::free(p);
     std::memset(m, 0, sizeof *m);
 }
@@ -212,6 +216,27 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left, const int32
     return SFX_OK;
 }
 
+int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* egeo,
+                            void* stream) {
+    if(!h || !m) return set_error(SFX_ERR_ARG, "sfx_mesh_entry_geometry: NULL handle or mesh");
+    int rc = check_mesh(h, m, "sfx_mesh_entry_geometry", false, false, false);
+    if(rc) return rc;
+    if(E < 0 || E > INT32_MAX)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_entry_geometry: E = %lld", (long long)E);
+    if(E == 0) return SFX_OK;
+    if(!m->cf_entry || !m->area || !m->mc || !m->rRL || !m->dL || !m->dR || !m->left ||
+       !m->right || !egeo)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_entry_geometry: NULL array");
+    if((rc = bind_device(h))) return rc;
+    hipError_t e = (h->precision == SFX_PREC_FP64)
+                           ? launch_entry_geometry<double>(h->ndim, view<double>(m), E,
+                                                           (double*)egeo, (hipStream_t)stream)
+                           : launch_entry_geometry<float>(h->ndim, view<float>(m), E,
+                                                          (float*)egeo, (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "entry geometry kernel launch");
+    return SFX_OK;
+}
+
 int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream) {
     int rc = check_mesh(h, m, "sfx_gradients", false, true, false);
     if(rc) return rc;
@@ -270,7 +295,7 @@ int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, double* dQL
 }
 
 int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const double* volume,
-                    void* W, uint8_t* cell_status, void* stream) {
+                    double* W, uint8_t* cell_status, void* stream) {
     if(!h) return set_error(SFX_ERR_ARG, "sfx_update_step: NULL handle");
     if(C < 0) return set_error(SFX_ERR_ARG, "sfx_update_step: C < 0");
     if(C == 0) return SFX_OK;
@@ -278,13 +303,70 @@ int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const doubl
     int rc = bind_device(h);
     if(rc) return rc;
     unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
-    hipError_t e = (h->precision == SFX_PREC_FP64)
-                           ? launch_update<double>(h->ndim, h->c, C, Q, dQ, volume, (double*)W,
-                                                   cell_status, ctr, (hipStream_t)stream)
-                           : launch_update<float>(h->ndim, h->c, C, Q, dQ, volume, (float*)W,
-                                                  cell_status, ctr, (hipStream_t)stream);
+    hipError_t e = launch_update(h->ndim, h->c, C, Q, dQ, volume, W, cell_status, ctr,
+                                 (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "update kernel launch");
     return SFX_OK;
 }
 
 }  // extern "C"
+
+// ---- generator order -------------------------------------------------
+
+namespace {
+
+// Hilbert index of integer coordinates X[0..n) with b bits each: Skilling's
+// transpose method (J. Skilling, "Programming the Hilbert curve", AIP Conf.
+// Proc. 707, 381 (2004)): undo the excess work of the curve's inverse, Gray
+// encode, then interleave the transposed bits, most significant first.
+uint64_t hilbert_index(uint32_t* X, int b, int n) {
+    const uint32_t M = 1u << (b - 1);
+    for(uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for(int i = 0; i < n; i++) {
+            if(X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for(int i = 1; i < n; i++) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for(uint32_t Q = M; Q > 1; Q >>= 1)
+        if(X[n - 1] & Q) t ^= Q - 1;
+    for(int i = 0; i < n; i++) X[i] ^= t;
+    uint64_t key = 0;
+    for(int bit = b - 1; bit >= 0; bit--)
+        for(int i = 0; i < n; i++) key = (key << 1) | ((X[i] >> bit) & 1u);
+    return key;
+}
+
+}  // namespace
+
+extern "C" int sfx_hilbert_order(int ndim, int64_t C, const double* pos, const double* box,
+                                 int64_t* order) {
+    if(ndim != 2 && ndim != 3) return set_error(SFX_ERR_ARG, "sfx_hilbert_order: ndim");
+    if(C < 0 || (C > 0 && (!pos || !box || !order)))
+        return set_error(SFX_ERR_ARG, "sfx_hilbert_order: NULL array");
+    const int b = ndim == 2 ? 31 : 21;
+    const double scale = (double)(1u << (b - 1)) * 2.0;
+    std::vector<std::pair<uint64_t, int64_t>> key(C);
+    for(int64_t i = 0; i < C; i++) {
+        uint32_t X[3] = {0, 0, 0};
+        for(int d = 0; d < ndim; d++) {
+            double u = pos[d * C + i] / box[d];
+            if(!(u >= 0.0 && u < 1.0))
+                return set_error(SFX_ERR_ARG, "sfx_hilbert_order: point %lld outside the box",
+                                 (long long)i);
+            double s = std::floor(u * scale);
+            X[d] = (uint32_t)std::min(s, scale - 1.0);
+        }
+        key[i] = {hilbert_index(X, b, ndim), i};
+    }
+    std::sort(key.begin(), key.end());
+    for(int64_t i = 0; i < C; i++) order[i] = key[i].second;
+    return SFX_OK;
+}

@@ -18,6 +18,8 @@ struct HostMesh {
     std::vector<uint8_t> ghost;                         // right is a periodic copy
     std::vector<int32_t> twin;                          // other copy of a periodic face
     std::vector<double> verts;                          // 2D: v0x v0y v1x v1y per face
+    std::vector<int64_t> fv_off;                        // [F+1] face vertex lists
+    std::vector<double> fverts;                         // [ndim][fv_off[F]]
     std::vector<int64_t> cf_off;                        // per-cell faces (walk order)
     std::vector<int32_t> cf_face;
 };

@@ -24,6 +24,8 @@ class FluxPass:
         self.s = solver
         self.m = dmesh
         self._d = ct.byref(dmesh.desc)
+        if not dmesh.egeo_built:
+            dmesh.build_entries(solver.handle)
 
     def _v(self, t):
         return ct.c_void_p(t.data_ptr())
@@ -85,7 +87,7 @@ def smoke(dev):
     fp = FluxPass(s64, DeviceMesh(b, "fp64", dev))
     fp.run()
     torch.cuda.synchronize()
-    dQ = fp.m.dQ.cpu().numpy()
+    dQ = fp.m.dQ_soa()
     ob = dict(b)
     ob["grad"] = O.gradients(1, b)
     dQo, _, _, _ = O.face_flux_pass(1, ob)
@@ -97,6 +99,6 @@ def smoke(dev):
     fm = FluxPass(smx, DeviceMesh(b, "mixed", dev))
     fm.run()
     torch.cuda.synchronize()
-    errm = np.abs(fm.m.dQ.cpu().numpy() - dQ).max() / scale
+    errm = np.abs(fm.m.dQ_soa() - dQ).max() / scale
     print("smoke flux pass mixed vs fp64: max err %.3e" % errm, flush=True)
     assert errm <= 1e-6

@@ -26,7 +26,8 @@ class HostMesh(ct.Structure):
     _fields_ = [("ndim", ct.c_int), ("C", ct.c_int64), ("F", ct.c_int64),
                 ("pos", _dp), ("volume", _dp), ("centroid", _dp), ("left", _i32p),
                 ("right", _i32p), ("area", _dp), ("midpoint", _dp), ("shift", _dp),
-                ("ghost", _u8p), ("twin", _i32p), ("cf_off", _i64p), ("cf_face", _i32p)]
+                ("ghost", _u8p), ("twin", _i32p), ("cf_off", _i64p), ("cf_face", _i32p),
+                ("fv_off", _i64p), ("fverts", _dp)]
 
 
 def _lib():
@@ -42,6 +43,7 @@ def _lib():
                                             _i32p]
         L.sfx_mesh_gather_index.argtypes = [ct.c_int64, ct.c_int64, _i32p, _i32p, _u8p,
                                             _i64p, _i32p]
+        L.sfx_hilbert_order.argtypes = [ct.c_int, ct.c_int64, _dp, _dp, _i64p]
         L._mesh_sigs_set = True
     return L
 
@@ -68,6 +70,22 @@ def jittered_grid(n, jitter=0.25, seed=1, ndim=2, box=1.0):
 def poisson_points(N, seed=1, ndim=3, box=1.0):
     ctr = np.arange(N, dtype=np.uint64)
     return np.stack([uniform(seed, 200 + d, ctr) * box for d in range(ndim)])
+
+
+def hilbert_order(pos, box=1.0):
+    """Permutation putting the generators in Hilbert order (the order the
+    reference keeps its particles in, src/utilities/ParticleVector.cpp:222)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    ndim, C = pos.shape
+    boxv = np.ascontiguousarray(np.broadcast_to(np.asarray(box, np.float64), (ndim,)))
+    order = np.zeros(C, np.int64)
+    Cp.check(_lib().sfx_hilbert_order(ndim, C, _p(pos), _p(boxv), _p(order, _i64p)),
+             "sfx_hilbert_order")
+    return order
+
+
+def hilbert_sorted(pos, box=1.0):
+    return np.ascontiguousarray(pos[:, hilbert_order(pos, box)])
 
 
 # --------------------------------------------------------- tessellation --
@@ -102,6 +120,9 @@ def voronoi_periodic(pos, box=1.0, nthreads=0):
         m["cf_off"] = arr(hm.cf_off, C + 1, np.int64)
         nE = int(m["cf_off"][-1])
         m["cf_face"] = arr(hm.cf_face, nE, np.int32)
+        m["fv_off"] = arr(hm.fv_off, F + 1, np.int64)
+        nV = int(m["fv_off"][-1])
+        m["fverts"] = arr(hm.fverts, ndim * nV, np.float64).reshape(ndim, nV)
     finally:
         L.sfx_host_mesh_free(ct.byref(hm))
     return m
@@ -253,13 +274,20 @@ def prepare(mesh, W, gamma=5.0 / 3.0, kind=0, vel=None, dt=None, cfl=0.4):
 
 # ------------------------------------------------------------- device --
 
-T_ARRAYS = ("W", "vel", "area", "dL", "dR", "rRL", "mc", "rcent")
+T_ARRAYS = ("vel", "area", "dL", "dR", "rRL", "mc", "rcent")
 F64_ARRAYS = ("mass", "total_area", "dt", "volume")
 
 
 class DeviceMesh:
     """Device copy of a prepared mesh.  storage: 'fp64' or 'mixed' (FP32
-    cell states, velocities and face geometry)."""
+    generator velocities, gradients and face geometry; the states W are FP64
+    in both).
+
+    Arrays the kernels gather through a neighbour index are item-major:
+    W (C, ndim+2), vel (C, ndim), grad (C, ndim, ndim+2), Q and dQ
+    (C, ndim+2), dQL (F, ndim+2); per-face geometry is (ndim, F).  The
+    *_soa() accessors return host copies in the component-major layout of
+    the prepared bundle (and of the oracle)."""
 
     def __init__(self, b, storage="fp64", device=None):
         import torch
@@ -272,7 +300,9 @@ class DeviceMesh:
         t = lambda a, dt: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=self.dev)
         self.arr = {}
         for k in T_ARRAYS:
-            self.arr[k] = t(b[k], tdt)
+            a = b[k].T if k == "vel" else b[k]
+            self.arr[k] = t(a, tdt)
+        self.arr["W"] = t(b["W"].T, torch.float64)  # FP64 in both passes
         for k in F64_ARRAYS:
             self.arr[k] = t(b[k], torch.float64)
         self.arr["active"] = t(b["active"], torch.uint8)
@@ -281,11 +311,15 @@ class DeviceMesh:
             self.arr[k] = t(b[k], torch.int32)
         for k in ("cf_off", "gather_off"):
             self.arr[k] = t(b[k], torch.int64)
-        self.Q = t(b["Q"], torch.float64)
+        self.Q = t(b["Q"].T, torch.float64)
         nd, NC = self.ndim, self.ndim + 2
-        self.grad = torch.zeros((nd, NC, self.C), dtype=tdt, device=self.dev)
-        self.dQL = torch.zeros((NC, self.F), dtype=torch.float64, device=self.dev)
-        self.dQ = torch.zeros((NC, self.C), dtype=torch.float64, device=self.dev)
+        self.E = int(b["cf_off"][-1])
+        self.G = int(b["gather_off"][-1])
+        self.egeo = torch.zeros((self.E, 8 if nd == 2 else 12), dtype=tdt, device=self.dev)
+        self.egeo_built = False
+        self.grad = torch.zeros((self.C, nd, NC), dtype=tdt, device=self.dev)
+        self.dQL = torch.zeros((self.F, NC), dtype=torch.float64, device=self.dev)
+        self.dQ = torch.zeros((self.C, NC), dtype=torch.float64, device=self.dev)
         self.fstatus = torch.zeros(self.F, dtype=torch.uint8, device=self.dev)
         self.cstatus = torch.zeros(self.C, dtype=torch.uint8, device=self.dev)
         self.use_twin = storage == "mixed"
@@ -302,6 +336,30 @@ class DeviceMesh:
         for k in ("dL", "dR", "rRL", "mc", "rcent"):
             setattr(d, k, p(k))
         d.cf_entry = p("cf_entry")
+        d.egeo = ct.c_void_p(self.egeo.data_ptr())
         d.gather_entry = p("gather_entry")
         d.twin = p("twin") if self.use_twin else None
         return d
+
+    def build_entries(self, solver_handle, stream=None):
+        """Per-entry face records of the gradient pass (once per mesh)."""
+        Cp.check(Cp.lib().sfx_mesh_entry_geometry(solver_handle, ct.byref(self.desc), self.E,
+                                                  ct.c_void_p(self.egeo.data_ptr()), stream),
+                 "sfx_mesh_entry_geometry")
+        self.egeo_built = True
+
+    # host copies in the component-major layout of the bundle / oracle
+    def grad_soa(self):
+        return self.grad.double().cpu().numpy().transpose(1, 2, 0).copy()
+
+    def dQL_soa(self):
+        return self.dQL.cpu().numpy().T.copy()
+
+    def dQ_soa(self):
+        return self.dQ.cpu().numpy().T.copy()
+
+    def Q_soa(self):
+        return self.Q.cpu().numpy().T.copy()
+
+    def W_soa(self):
+        return self.arr["W"].double().cpu().numpy().T.copy()

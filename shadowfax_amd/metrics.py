@@ -47,3 +47,49 @@ def summarize(err):
     q = np.quantile(err, [0.5, 0.99])
     return {"max": float(err.max()), "p99": float(q[1]), "p50": float(q[0]),
             "n": int(err.size)}
+
+
+# ---------------------------------------------------------- flux pass --
+
+def face_flux_scale(b, gamma):
+    """dt A rho_bar a_bar max(1, a_bar) per face: the flux floor of the
+    normwise metric carried to dQL = dt A F units (b: a prepared mesh)."""
+    W, L, R = b["W"], b["left"], b["right"]
+    Rr = np.where(R >= 0, R, L)
+    a = np.sqrt(gamma * W[-1] / W[0])
+    rho = 0.5 * (W[0, L] + W[0, Rr])
+    ab = 0.5 * (a[L] + a[Rr])
+    return b["dt"][L] * b["area"] * rho * ab * np.maximum(ab, 1.0)
+
+
+def fluxpass_errors(b, gamma, dQL_ref, dQ_ref, dQL, dQ, once_per_pair=False):
+    """Errors of one flux pass (dQL [nc][F], dQ [nc][C]) against a reference
+    pass on the same mesh:
+      face_*: normwise per face, floor face_flux_scale;
+      cell_gross_*: per cell, |dQ - dQ_ref|_inf over the cell's gross flux
+        sum_f |dQL_ref,f|_inf (dQ is a sum of face fluxes that largely
+        cancel, so |dQ| itself is no scale);
+      cell_vs_maxdQ: max |dQ - dQ_ref| / max |dQ_ref|;
+      conservation: max_k |sum_i dQ_ki| / max |dQL_ref|.
+    once_per_pair: the pass evaluated each periodic face pair once and left
+    the second copy's dQL unused; it is compared through its twin."""
+    dQL_ref, dQ_ref, dQL, dQ = map(_np, (dQL_ref, dQ_ref, dQL, dQ))
+    F = dQL.shape[1]
+    if once_per_pair:
+        dQL = dQL.copy()
+        tw = b["twin"]
+        skip = (b["ghost"] == 1) & (tw >= 0) & (tw < np.arange(F))
+        dQL[:, skip] = -dQL[:, tw[skip]]
+    den = np.maximum(np.abs(dQL_ref).max(0), face_flux_scale(b, gamma))
+    den = np.where(den > 0, den, 1.0)
+    ef = np.abs(dQL - dQL_ref).max(0) / den
+    gross = np.zeros(dQ.shape[1])
+    np.add.at(gross, b["left"], np.abs(dQL_ref).max(0))
+    real = (b["right"] >= 0) & (b["ghost"] == 0)
+    np.add.at(gross, b["right"][real], np.abs(dQL_ref[:, real]).max(0))
+    ec = np.abs(dQ - dQ_ref).max(0) / np.where(gross > 0, gross, 1.0)
+    scale = np.abs(dQL_ref).max()
+    return {"face_max": float(ef.max()), "face_p99": float(np.quantile(ef, 0.99)),
+            "cell_gross_max": float(ec.max()), "cell_gross_p99": float(np.quantile(ec, 0.99)),
+            "cell_vs_maxdQ": float(np.abs(dQ - dQ_ref).max() / np.abs(dQ_ref).max()),
+            "conservation": float(np.abs(dQ.sum(1)).max() / scale)}

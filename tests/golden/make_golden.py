@@ -10,6 +10,12 @@ moving: random face velocity and 25% reflective faces):
   outputs F_exact, st_exact, F_hllc, st_hllc, Whalf, mach, st_solve
 plus time-derivative and get_W/get_Q vectors.  gamma = 5/3, tol = 1e-8,
 cutoff = -5 (the configuration the benchmarks use) and a gamma = 1.4 set.
+
+tests/golden/mesh_{2,3}d.npz: small periodic and mirror-walled meshes from
+the host tessellator with states, and the reference classes' gradients
+(VorCell::estimate_gradient), face velocities and normals (VorFace::set_v,
+get_normal), face area/midpoint and cell volume/centroid (VorFace /
+VorCell calculate_*) on them.
 """
 import os
 import sys
@@ -102,6 +108,24 @@ def mirrorize(m):
     return m
 
 
+def reference_outputs(out, name, m):
+    geo = m.pop("geo", None)
+    g, vf, nr = O.ref_mesh_gradients(m)
+    for k in MESH_KEYS + ("fv_off", "fverts"):
+        out[name + "_" + k] = m[k]
+    out[name + "_ref_grad"] = g
+    out[name + "_ref_vface"] = vf
+    out[name + "_ref_normal"] = nr
+    # face area/midpoint and cell volume/centroid by the reference classes
+    # from the same vertex lists (before mirrorize changes the topology)
+    if geo is not None:
+        a, mid, vol, cen = geo
+        out[name + "_ref_area"] = a
+        out[name + "_ref_midpoint"] = mid
+        out[name + "_ref_volume"] = vol
+        out[name + "_ref_centroid"] = cen
+
+
 def mesh_set():
     """Small 2D meshes with KH-like states plus noise; reference gradients
     (VorCell::estimate_gradient), face velocities (VorFace::set_v) and normals
@@ -111,6 +135,7 @@ def mesh_set():
     rng = np.random.default_rng(42)
     for name, n, mirror in (("per12", 12, False), ("mir10", 10, True)):
         m = M.voronoi_periodic(M.jittered_grid(n, 0.25, seed=11 + n))
+        m["geo"] = O.ref_mesh_geometry(m)
         if mirror:
             m = mirrorize(m)
         W = M.kh_ics(m)
@@ -118,12 +143,29 @@ def mesh_set():
         W[3] *= 1.0 + 0.3 * rng.random(m["C"])
         m["W"] = W
         m["vel"] = W[1:3] + 0.05 * rng.normal(size=(2, m["C"]))
-        g, vf, nr = O.ref_mesh_gradients(m)
-        for k in MESH_KEYS:
-            out[name + "_" + k] = m[k]
-        out[name + "_ref_grad"] = g
-        out[name + "_ref_vface"] = vf
-        out[name + "_ref_normal"] = nr
+        reference_outputs(out, name, m)
+    return out
+
+
+def mesh_set_3d():
+    """Small 3D Poisson meshes (periodic, and with the x faces turned into
+    mirrors) with Sedov-like states plus noise; reference gradients, face
+    velocities, normals and geometry."""
+    from shadowfax_amd import mesh as M
+    out = {}
+    rng = np.random.default_rng(43)
+    for name, n, mirror in (("p3d", 160, False), ("m3d", 120, True)):
+        m = M.voronoi_periodic(M.poisson_points(n, seed=31 + n, ndim=3))
+        m["geo"] = O.ref_mesh_geometry(m)
+        if mirror:
+            m = mirrorize(m)
+        W = M.sedov_ics(m, ncells=4)
+        W[0] *= 1.0 + 0.2 * rng.random(m["C"])
+        W[4] *= 1.0 + 0.3 * rng.random(m["C"])
+        W[1:4] = 0.3 * rng.normal(size=(3, m["C"]))
+        m["W"] = W
+        m["vel"] = W[1:4] + 0.05 * rng.normal(size=(3, m["C"]))
+        reference_outputs(out, name, m)
     return out
 
 
@@ -140,9 +182,10 @@ def main():
         path = os.path.join(HERE, "riemann_%dd.npz" % ndim)
         np.savez_compressed(path, **data)
         print(path, os.path.getsize(path))
-    path = os.path.join(HERE, "mesh_2d.npz")
-    np.savez_compressed(path, **mesh_set())
-    print(path, os.path.getsize(path))
+    for nd, fn in ((2, mesh_set), (3, mesh_set_3d)):
+        path = os.path.join(HERE, "mesh_%dd.npz" % nd)
+        np.savez_compressed(path, **fn())
+        print(path, os.path.getsize(path))
 
 
 if __name__ == "__main__":

@@ -56,3 +56,30 @@ def test_header_compiles_as_c():
                         os.path.join(root, "include"), "-x", "c", "-", "-o", "/dev/null"],
                        input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors of sfx_mesh and sfx_host_mesh have the C layout
+    (size and every field offset)."""
+    import ctypes as ct
+
+    from shadowfax_amd import mesh as M
+
+    root = os.path.dirname(os.path.dirname(C.LIBPATH))
+    lines = []
+    for cname, py in (("sfx_mesh", C.MeshDesc), ("sfx_host_mesh", M.HostMesh)):
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    src = ('#include <stddef.h>\n#include <stdio.h>\n#include "shadowfax_amd.h"\n'
+           "int main(void){%s return 0;}\n" % "\n".join(lines))
+    exe = str(tmp_path / "layout")
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(root, "include"), "-x", "c", "-",
+                        "-o", exe], input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in (("sfx_mesh", C.MeshDesc), ("sfx_host_mesh", M.HostMesh)):
+        assert got[(cname, "size")] == ct.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

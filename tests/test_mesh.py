@@ -1,0 +1,134 @@
+"""Host tessellator (the mesh exporter the flux pass reads), CPU only.
+
+  * face area/midpoint and cell volume/centroid are bit-identical to what the
+    reference classes (VorFace::calculate_midpoint/calculate_area,
+    VorCell::calculate_volume/calculate_centroid) compute from the same
+    stored vertex lists (golden vectors from the compiled reference);
+  * the tessellation itself is pinned independently of the reference: cell
+    volumes and face areas against Qhull (scipy.spatial) on a 3x3(x3) tiled
+    copy of the periodic box, volumes summing to the box, each cell closed
+    (sum_f A_f n_f = 0), and the face-list contract (each real-real face in
+    both cells' lists, each periodic face once per side with a twin).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from shadowfax_amd import mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def golden_points(nd, name):
+    if nd == 2:
+        n = {"per12": 12, "mir10": 10}[name]
+        return M.jittered_grid(n, 0.25, seed=11 + n)
+    n = {"p3d": 160, "m3d": 120}[name]
+    return M.poisson_points(n, seed=31 + n, ndim=3)
+
+
+@pytest.mark.parametrize("nd,name", [(2, "per12"), (2, "mir10"), (3, "p3d"), (3, "m3d")])
+def test_geometry_matches_reference_classes_bitwise(nd, name):
+    z = np.load(os.path.join(HERE, "golden", "mesh_%dd.npz" % nd))
+    m = M.voronoi_periodic(golden_points(nd, name))
+    for k, ref in (("area", "ref_area"), ("midpoint", "ref_midpoint"),
+                   ("volume", "ref_volume"), ("centroid", "ref_centroid")):
+        assert np.array_equal(m[k], z[name + "_" + ref]), k
+
+
+def closure(m):
+    pos, L, R, A = m["pos"], m["left"], m["right"], m["area"]
+    n = pos[:, R] + m["shift"] - pos[:, L]
+    n = n / np.linalg.norm(n, axis=0)
+    acc = np.zeros_like(pos)
+    np.add.at(acc.T, L, (A * n).T)
+    real = m["ghost"] == 0
+    np.add.at(acc.T, R[real], -(A * n)[:, real].T)
+    return np.abs(acc).max()
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_volumes_closure_and_face_contract(nd):
+    pos = M.jittered_grid(40, 0.25, seed=5) if nd == 2 else M.poisson_points(3000, seed=5)
+    m = M.voronoi_periodic(pos)
+    C, F = m["C"], m["F"]
+    assert abs(m["volume"].sum() - 1.0) <= 1e-13
+    assert closure(m) <= 1e-14
+    L, R, gh, tw = m["left"], m["right"], m["ghost"], m["twin"]
+    # each face listed by its left cell; a real-real face also by its right
+    # cell, a periodic face only once (its twin is the other side's copy)
+    cnt = np.zeros(F, np.int64)
+    off, cf = m["cf_off"], m["cf_face"]
+    owner = np.repeat(np.arange(C), np.diff(off))
+    np.add.at(cnt, cf, 1)
+    assert np.all(cnt == np.where(gh == 1, 1, 2))
+    assert np.all((owner == L[cf]) | ((owner == R[cf]) & (gh[cf] == 0)))
+    assert np.all(L[gh == 0] < R[gh == 0])
+    g = np.where(gh == 1)[0]
+    assert np.all(tw[g] >= 0)
+    assert np.array_equal(tw[tw[g]], g)
+    assert np.array_equal(L[tw[g]], R[g]) and np.array_equal(R[tw[g]], L[g])
+    assert np.array_equal(m["shift"][:, tw[g]], -m["shift"][:, g])
+    # the two copies of a periodic face agree to round-off
+    assert np.abs(m["area"][tw[g]] - m["area"][g]).max() <= 1e-15
+    # centroids inside the box around their generator
+    d = m["centroid"] - m["pos"]
+    assert np.abs(d).max() < 0.1
+
+
+@pytest.mark.parametrize("nd,C", [(2, 300), (3, 400)])
+def test_tessellation_matches_qhull(nd, C):
+    spatial = pytest.importorskip("scipy.spatial")
+    rng = np.random.default_rng(17 + nd)
+    pos = rng.random((nd, C))
+    m = M.voronoi_periodic(pos)
+    tiles = np.array(np.meshgrid(*[(-1, 0, 1)] * nd, indexing="ij")).reshape(nd, -1).T
+    center = int(np.where((tiles == 0).all(1))[0][0])
+    P = np.concatenate([pos.T + t for t in tiles])
+    vor = spatial.Voronoi(P)
+    vol = np.array([spatial.ConvexHull(vor.vertices[vor.regions[vor.point_region[center * C + i]]])
+                    .volume for i in range(C)])
+    assert np.abs(vol - m["volume"]).max() <= 1e-12 * m["volume"].min()
+    # faces: (cell, neighbour, shift) -> area
+    ref = {}
+    for (p, q), rv in zip(vor.ridge_points, vor.ridge_vertices):
+        for a, b in ((p, q), (q, p)):
+            if a // C != center:
+                continue
+            V = vor.vertices[rv]
+            if nd == 2:
+                area = np.linalg.norm(V[0] - V[1])
+            else:
+                c = V.mean(0)
+                area = sum(0.5 * np.linalg.norm(np.cross(V[k] - c, V[(k + 1) % len(V)] - c))
+                           for k in range(len(V)))
+            ref[(a % C, b % C, tuple(tiles[b // C]))] = area
+    mine = {}
+    for f in range(m["F"]):
+        sh = tuple(np.round(m["shift"][:, f]).astype(int))
+        mine[(m["left"][f], m["right"][f], sh)] = m["area"][f]
+        if not m["ghost"][f]:
+            mine[(m["right"][f], m["left"][f], sh)] = m["area"][f]
+    assert set(mine) == set(ref)
+    assert max(abs(mine[k] - ref[k]) for k in ref) <= 1e-13
+
+
+@pytest.mark.parametrize("nd,n", [(2, 16), (3, 8)])
+def test_hilbert_order_is_a_hilbert_curve(nd, n):
+    """Points at the centres of an n^nd grid, in Hilbert order, form a path
+    of unit steps visiting every grid cell once (the defining property of the
+    curve); random points come out as a permutation with short steps."""
+    g = np.stack(np.meshgrid(*[np.arange(n)] * nd, indexing="ij")).reshape(nd, -1)
+    rng = np.random.default_rng(3)
+    g = g[:, rng.permutation(g.shape[1])]
+    pos = (g + 0.5) / n
+    order = M.hilbert_order(pos)
+    assert np.array_equal(np.sort(order), np.arange(g.shape[1]))
+    steps = np.abs(np.diff(g[:, order], axis=1)).sum(0)
+    assert np.all(steps == 1)
+    p = M.poisson_points(20000, seed=1, ndim=nd)
+    o = M.hilbert_order(p)
+    assert np.array_equal(np.sort(o), np.arange(20000))
+    d = np.linalg.norm(np.diff(p[:, o], axis=1), axis=0)
+    assert np.median(d) < 2.0 * 20000 ** (-1.0 / nd)

@@ -28,13 +28,17 @@ KEYS = ("pos", "volume", "centroid", "left", "right", "area", "midpoint", "shift
         "twin", "cf_off", "cf_face", "W", "vel")
 
 
+MESHES = ["per12", "mir10", "p3d", "m3d"]
+
+
 def golden_mesh(name):
-    z = np.load(os.path.join(HERE, "golden", "mesh_2d.npz"))
+    nd = 3 if name.endswith("3d") else 2
+    z = np.load(os.path.join(HERE, "golden", "mesh_%dd.npz" % nd))
     m = {k: z[name + "_" + k] for k in KEYS}
-    m["ndim"] = 2
+    m["ndim"] = nd
     m["C"] = m["volume"].shape[0]
     m["F"] = m["area"].shape[0]
-    m["box"] = np.ones(2)
+    m["box"] = np.ones(nd)
     ref = {k: z[name + "_ref_" + k] for k in ("grad", "vface", "normal")}
     return m, ref
 
@@ -43,13 +47,13 @@ def bundle(m, kind=1, dt=None):
     return M.prepare(m, m["W"], G, kind=kind, vel=m["vel"], dt=dt)
 
 
-@pytest.mark.parametrize("name", ["per12", "mir10"])
+@pytest.mark.parametrize("name", MESHES)
 def test_gradients_match_reference_bitwise(name):
     m, ref = golden_mesh(name)
     assert np.array_equal(O.gradients(1, m), ref["grad"])
 
 
-@pytest.mark.parametrize("name", ["per12", "mir10"])
+@pytest.mark.parametrize("name", MESHES)
 def test_face_velocity_and_normal_match_reference_bitwise(name):
     m, ref = golden_mesh(name)
     v, n = O.face_velocities(m)
@@ -80,33 +84,39 @@ def test_limiter_hand_computed(case):
     assert abs(got - want) <= 1e-15 * max(1.0, abs(want))
 
 
-def test_zero_gradients_reduce_to_solve_for_flux():
+@pytest.mark.parametrize("name", ["mir10", "m3d"])
+def test_zero_gradients_reduce_to_solve_for_flux(name):
     """With grad = 0 the predictor and limiter leave the boosted states
     unchanged, so dQL = (dt A) solve_for_flux(WL - v, WR - v, n, v)."""
-    m, _ = golden_mesh("mir10")
+    m, _ = golden_mesh(name)
+    nd = m["ndim"]
     for kind in (O.EXACT, O.HLLC):
         b = bundle(m, kind)
-        b["grad"] = np.zeros((2, 4, m["C"]))
+        b["grad"] = np.zeros((nd, nd + 2, m["C"]))
         dQ, dQL, st, _ = O.face_flux_pass(kind, b)
         v, n = O.face_velocities(b)
         L, R = b["left"], b["right"]
         WL = b["W"][:, L].copy()
         WR = np.where(R >= 0, b["W"][:, np.maximum(R, 0)], 0.0)
-        WL[1:3] -= v
-        WR[1:3] -= v
-        F, sts, _ = O.solve_for_flux(kind, 2, WL, WR, n, v=v, refl=(R < 0).astype(np.uint8))
+        WL[1:1 + nd] -= v
+        WR[1:1 + nd] -= v
+        F, sts, _ = O.solve_for_flux(kind, nd, WL, WR, n, v=v, refl=(R < 0).astype(np.uint8))
         dt = np.minimum(b["dt"][L], np.where(R >= 0, b["dt"][np.maximum(R, 0)], np.inf))
         want = (dt * b["area"]) * F
         live = b["area"] != 0
         assert np.array_equal(dQL[:, live], want[:, live])
 
 
-def test_uniform_comoving_flow_gives_no_net_change():
-    m, _ = golden_mesh("per12")
-    W = np.zeros((4, m["C"]))
-    W[0], W[1], W[2], W[3] = 1.3, 0.4, -0.2, 2.0
+@pytest.mark.parametrize("name", ["per12", "p3d"])
+def test_uniform_comoving_flow_gives_no_net_change(name):
+    m, _ = golden_mesh(name)
+    nd = m["ndim"]
+    W = np.zeros((nd + 2, m["C"]))
+    W[0], W[1], W[2], W[-1] = 1.3, 0.4, -0.2, 2.0
+    if nd == 3:
+        W[3] = 0.1
     m = dict(m)
-    m["W"], m["vel"] = W, W[1:3].copy()
+    m["W"], m["vel"] = W, W[1:1 + nd].copy()
     for kind in (O.EXACT, O.HLLC):
         b = bundle(m, kind)
         b["grad"] = O.gradients(kind, b)
@@ -115,10 +125,16 @@ def test_uniform_comoving_flow_gives_no_net_change():
         assert np.abs(dQ).max() <= 1e-13 * np.abs(dQL).max()
 
 
+@pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("kind", [O.EXACT, O.HLLC])
-def test_periodic_pass_conserves(kind):
-    m = M.voronoi_periodic(M.jittered_grid(20, 0.25, seed=2))
-    b = M.prepare(m, M.kh_ics(m), G, kind=kind)
+def test_periodic_pass_conserves(kind, nd):
+    if nd == 2:
+        m = M.voronoi_periodic(M.jittered_grid(20, 0.25, seed=2))
+        W = M.kh_ics(m)
+    else:
+        m = M.voronoi_periodic(M.poisson_points(800, seed=2))
+        W = M.sedov_ics(m, ncells=8)
+    b = M.prepare(m, W, G, kind=kind)
     b["grad"] = O.gradients(kind, b)
     dQ, dQL, st, _ = O.face_flux_pass(kind, b)
     assert st.max() == 0
@@ -197,5 +213,26 @@ def test_linear_field_gradient_is_exact():
         exact = np.isclose(gi, val, rtol=1e-9, atol=1e-12)
         assert exact.mean() > 0.98
         # where the limiter acts it only scales the slope by alpha in [0, 1]
+        a = gi[~exact] / val
+        assert np.all((a >= 0.0) & (a <= 1.0 + 1e-9))
+
+
+def test_linear_field_gradient_is_exact_3d():
+    m = M.voronoi_periodic(M.poisson_points(2000, seed=4))
+    W = np.zeros((5, m["C"]))
+    x, y, z = m["pos"]
+    W[0] = 2.0 + 0.3 * x - 0.2 * y + 0.1 * z
+    W[4] = 3.0 + 0.1 * x + 0.4 * y - 0.3 * z
+    W[1], W[2], W[3] = 0.5 * x, 0.25 * y, -0.5 * z
+    mm = dict(m)
+    mm["W"], mm["vel"] = W, np.zeros((3, m["C"]))
+    g = O.gradients(0, mm)
+    inner = np.all((m["pos"] > 0.2) & (m["pos"] < 0.8), axis=0)
+    want = {(0, 0): 0.3, (1, 0): -0.2, (2, 0): 0.1, (0, 4): 0.1, (1, 4): 0.4, (2, 4): -0.3,
+            (0, 1): 0.5, (1, 2): 0.25, (2, 3): -0.5}
+    for (d, k), val in want.items():
+        gi = g[d, k, inner]
+        exact = np.isclose(gi, val, rtol=1e-9, atol=1e-12)
+        assert exact.mean() > 0.95
         a = gi[~exact] / val
         assert np.all((a >= 0.0) & (a <= 1.0 + 1e-9))
