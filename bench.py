@@ -134,6 +134,67 @@ def bench_fluxpass3d(steps, warmup, dev, n=2097152, kind=0):
     return out
 
 
+def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
+    """Config 4: an ensemble of n_meshes 2D periodic meshes (cell counts
+    log-uniform in [1e3, 2.5e5], Sod / Kelvin-Helmholtz / Sedov states, HLLC)
+    resident in HBM; requests (one flux pass + update of one mesh) arrive as
+    a Poisson process and are served open loop.  Batched mixed serving (the
+    captured graph with persistent kernels, admitted by scratch footprint)
+    against the plain per-mesh FP64 launch; each at load x its own capacity
+    (capacity: the same trace with every request present at t = 0)."""
+    import torch
+    from shadowfax_amd import serving as S
+    from shadowfax_amd.riemann import RiemannSolver
+    w = S.config4(n_meshes)
+    out = {"workload": "%d periodic 2D meshes, cells log-uniform in [1e3, 2.5e5] (%d "
+                       "jittered-grid size classes, each mesh a translated copy with its own "
+                       "states), Sod/KH/Sedov, HLLC; Poisson arrivals, uniform mesh choice"
+                       % (n_meshes, len(np.unique(w["cls"]))),
+           "meshes": n_meshes, "requests": nreq, "load": load}
+    t0 = time.perf_counter()
+    classes = S.config4_classes(w)
+    out["tessellation_s"] = time.perf_counter() - t0
+    for prec, mode, name in (("mixed", S.SERVE_BATCHED, "batched_mixed"),
+                             ("fp64", S.SERVE_PLAIN, "plain_fp64")):
+        t0 = time.perf_counter()
+        solver = RiemannSolver(1, 2, 5.0 / 3.0, 1e-8, -5.0, prec)
+        solver.enable_stats(False)
+        ens, faces = S.build_config4(solver, dev, w, classes, log=log)
+        r = {"build_s": time.perf_counter() - t0}
+        tot_c = sum(ens.cells)
+        r["cells"], r["faces"] = int(tot_c), int(faces.sum())
+        if mode == S.SERVE_BATCHED:
+            # autotune the persistent launch on a batch of about a million cells
+            ids, c = [], 0
+            for k in range(n_meshes):
+                if c + ens.cells[k] > min(ens.arena_cells, 1_000_000):
+                    break
+                ids.append(k)
+                c += ens.cells[k]
+            blk, pcu, ms = ens.autotune(ids, reps=3)
+            r["launch"] = {"block": blk, "blocks_per_cu": pcu, "autotune_batch_cells": c,
+                           "autotune_ms": ms}
+        arr, mesh = S.poisson_trace(nreq, 1.0, n_meshes, w["seed"] + 1)
+        lat, bat, st = ens.serve(np.zeros(nreq), mesh, mode)
+        cap_req = nreq / st["wall_s"]
+        r["capacity"] = {"requests_per_s": cap_req,
+                         "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"],
+                         "mean_batch": st["mean_batch"], "steps": st["nbatch"]}
+        rate = load * cap_req
+        lat, bat, st = ens.serve(arr / rate, mesh, mode)
+        r["open_loop"] = {"rate_per_s": rate, "p50_ms": st["p50_s"] * 1e3,
+                          "p99_ms": st["p99_s"] * 1e3, "mean_ms": st["mean_s"] * 1e3,
+                          "max_ms": st["max_s"] * 1e3, "mean_batch": st["mean_batch"],
+                          "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"]}
+        out[name] = r
+        log("  %s: capacity %.0f req/s, p50 %.3f ms p99 %.3f ms" %
+            (name, cap_req, r["open_loop"]["p50_ms"], r["open_loop"]["p99_ms"]))
+        ens.close()
+        del ens
+        torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernel, grid):
     """HBM bytes per launch of a kernel from the committed hardware-counter
     summary (profiles/hbm_traffic.json), if it was measured at this grid."""
@@ -166,8 +227,12 @@ def copy_probe(mib, dev):
 
 
 def cpu_baseline(b, N):
-    """Reference ExactRiemannSolver compiled from source when oracle/_ref is
-    present, else the oracle restatement; one thread, the full batch."""
+    """The reference ExactRiemannSolver compiled from source (oracle/_ref)
+    when present, else the oracle restatement, over the full batch: on one
+    thread (the reference's serial per-rank loop) and on all host threads of
+    this job (contiguous chunks, one solver per thread).  value: the
+    all-threads rate."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     kind = "port"
@@ -181,14 +246,34 @@ def cpu_baseline(b, N):
         pass
     if kind == "port":
         O.lib()
-    sl = lambda k, a, z: b[k][:, a:z]
+    sl = lambda k, a, z: np.ascontiguousarray(b[k][:, a:z])
     fn(0, 3, sl("WL", 0, 1000), sl("WR", 0, 1000), sl("n", 0, 1000))  # warm
     t0 = time.perf_counter()
     fn(0, 3, b["WL"], b["WR"], b["n"])
-    dt = time.perf_counter() - t0
-    return {"value": N / dt, "unit": "face-fluxes/s", "cores": 1, "kind": kind,
-            "sample": "full batch: %d exact solve_for_flux calls, FP64, 1 thread, %.2f s"
-                      % (N, dt)}
+    t1 = time.perf_counter() - t0
+    T = host_threads()
+    cuts = np.linspace(0, N, T + 1).astype(int)
+    chunks = [(sl("WL", cuts[k], cuts[k + 1]), sl("WR", cuts[k], cuts[k + 1]),
+               sl("n", cuts[k], cuts[k + 1])) for k in range(T)]
+    with ThreadPoolExecutor(T) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda c: fn(0, 3, *c), chunks))
+        tT = time.perf_counter() - t0
+    return {"value": N / tT, "unit": "face-fluxes/s", "cores": T, "kind": kind,
+            "sample": "full batch: %d exact solve_for_flux calls, FP64, %d threads "
+                      "%.2f s (1 thread: %.2f s, %.3g face-fluxes/s)" % (N, T, tT, t1, N / t1),
+            "single_thread": N / t1, "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -200,6 +285,9 @@ def main():
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--no-fluxpass", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D flux pass (config 3)")
+    ap.add_argument("--no-serve", action="store_true", help="skip serving (config 4)")
+    ap.add_argument("--serve-meshes", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--only", default=None,
                     help="comma list of solver variants to time (exact_fp64, exact_mixed, "
                          "hllc_fp64, hllc_mixed); skips the headline line")
@@ -302,6 +390,9 @@ def main():
         if not args.no_3d:
             log("flux pass 3D (config 3)")
             out["fluxpass_3d_sedov"] = bench_fluxpass3d(args.steps, args.warmup, dev)
+    if not args.no_serve:
+        log("serving (config 4)")
+        out["serving"] = bench_serving(dev, args.serve_meshes, args.serve_requests)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(b, N)

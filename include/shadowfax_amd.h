@@ -222,7 +222,7 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left,
  * primitive states W are double for both.
  * Layout: arrays the kernels read through a neighbour index are cell- or
  * face-major (W [C][ndim+2], vel [C][ndim], grad [C][ndim][ndim+2],
- * dQ and Q [C][ndim+2], dQL [F][ndim+2]) so that one gather is one cache
+ * dQ and Q [C][ndim+2], dQL T [F][ndim+2]) so that one gather is one cache
  * line; per-face geometry, read one face per lane, is component-major
  * ([ndim][F]). */
 typedef struct {
@@ -274,23 +274,23 @@ int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* e
 int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream);
 
 /* VorFace::set_v + calculate_flux for every face (src/VorFace.cpp:586-866):
- * dQL: [F][ndim+2] the flux exchanged through each face, dt*A*F after the
+ * dQL: T [F][ndim+2] the flux exchanged through each face, dt*A*F after the
  * mass-flux limiter, seen from the left cell (the right cell receives -dQL);
  * face_status: [F] or NULL. */
 int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad,
-                    double* dQL, uint8_t* face_status, void* stream);
+                    void* dQL, uint8_t* face_status, void* stream);
 
 /* Deterministic scatter: dQ[i*(ndim+2) + k] = sum of the cell's entries in
  * ascending face order (overwrites dQ). */
-int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL,
+int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const void* dQL,
                   double* dQ, void* stream);
 
 /* VorTessManager::hydro (src/VorTessManager.cpp:198-212) preceded by the
  * gradient estimation: sfx_gradients + sfx_face_fluxes + sfx_gather_dQ.
- * grad (T [C][ndim][ndim+2]) and dQL ([F][ndim+2]) are caller-owned work
+ * grad (T [C][ndim][ndim+2]) and dQL (T [F][ndim+2]) are caller-owned work
  * buffers; dQ receives the per-cell change [C][ndim+2]. */
 int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad,
-                       double* dQL, double* dQ, uint8_t* face_status,
+                       void* dQL, double* dQ, uint8_t* face_status,
                        void* stream);
 
 /* GasParticle::update_Q then get_W (src/utilities/GasParticle.cpp:617-624,

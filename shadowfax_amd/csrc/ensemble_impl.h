@@ -70,13 +70,22 @@ template <bool FACES, class Fn>
 __device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
     __shared__ int64_t s_tile;
     const int64_t T = FACES ? p->ftiles : p->ctiles;
+    // with fewer tiles than blocks the first T blocks take them all
+    if((int64_t)blockIdx.x >= T) return;
     const int tile = p->tile;
     const int x0 = blockIdx.x % kXcd;
     for(int c = 0; c < kXcd; c++) {
         const int x = (x0 + c) % kXcd;
         const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
         for(;;) {
-            if(threadIdx.x == 0) s_tile = lo + (int64_t)atomicAdd(&p->ctr[phase][x], 1u);
+            if(threadIdx.x == 0) {
+                // counters only grow: a chunk seen exhausted by a plain load
+                // is exhausted, and costs no read-modify-write
+                const unsigned seen = __atomic_load_n(&p->ctr[phase][x], __ATOMIC_RELAXED);
+                s_tile = lo + (int64_t)seen >= hi
+                                 ? hi
+                                 : lo + (int64_t)atomicAdd(&p->ctr[phase][x], 1u);
+            }
             __syncthreads();
             const int64_t t = s_tile;
             __syncthreads();
@@ -103,7 +112,7 @@ __global__ __launch_bounds__(256) void drain_gradients(flux::MeshView<T> m, T* _
 template <int ND, int KIND, int MIXED, typename T, typename R>
 __global__ __launch_bounds__(256) void drain_faces(flux::MeshView<T> m, const T* __restrict__ grad,
                                                    Consts cd, mixed::CF cf,
-                                                   double* __restrict__ dQL,
+                                                   T* __restrict__ dQL,
                                                    uint8_t* __restrict__ fstatus,
                                                    unsigned long long* __restrict__ ctr,
                                                    Plan* __restrict__ p) {
@@ -115,16 +124,16 @@ __global__ __launch_bounds__(256) void drain_faces(flux::MeshView<T> m, const T*
     block_add_counters(ctr, cnt);
 }
 
-template <int ND, int MIXED>
+template <int ND, int MIXED, typename T>
 __global__ __launch_bounds__(256) void drain_gather(const int64_t* __restrict__ goff,
                                                     const int32_t* __restrict__ gent,
                                                     const uint8_t* __restrict__ ghost,
                                                     const int32_t* __restrict__ twin,
-                                                    const double* __restrict__ dQL,
+                                                    const T* __restrict__ dQL,
                                                     double* __restrict__ dQ,
                                                     Plan* __restrict__ p) {
     drain<false>(p, 2, [&](const Slot& sl, int64_t i) {
-        flux::gather_cell<ND, MIXED>(i, goff, gent, ghost, twin, dQL, sl.qshift, dQ);
+        flux::gather_cell<ND, MIXED, T>(i, goff, gent, ghost, twin, dQL, sl.qshift, dQ);
     });
 }
 
@@ -157,7 +166,7 @@ __global__ __launch_bounds__(256) void range_gradients(flux::MeshView<T> m, T* _
 template <int ND, int KIND, int MIXED, typename T, typename R>
 __global__ __launch_bounds__(256) void range_faces(flux::MeshView<T> m, const T* __restrict__ grad,
                                                    Consts cd, mixed::CF cf,
-                                                   double* __restrict__ dQL,
+                                                   T* __restrict__ dQL,
                                                    uint8_t* __restrict__ fstatus,
                                                    unsigned long long* __restrict__ ctr,
                                                    int64_t face0, int64_t nface, int64_t gshift,
@@ -170,16 +179,17 @@ __global__ __launch_bounds__(256) void range_faces(flux::MeshView<T> m, const T*
     block_add_counters(ctr, cnt);
 }
 
-template <int ND, int MIXED>
+template <int ND, int MIXED, typename T>
 __global__ __launch_bounds__(256) void range_gather(const int64_t* __restrict__ goff,
                                                     const int32_t* __restrict__ gent,
                                                     const uint8_t* __restrict__ ghost,
                                                     const int32_t* __restrict__ twin,
-                                                    const double* __restrict__ dQL,
+                                                    const T* __restrict__ dQL,
                                                     double* __restrict__ dQ, int64_t cell0,
                                                     int64_t ncell, int64_t qshift) {
     const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
-    if(k < ncell) flux::gather_cell<ND, MIXED>(cell0 + k, goff, gent, ghost, twin, dQL, qshift, dQ);
+    if(k < ncell)
+        flux::gather_cell<ND, MIXED, T>(cell0 + k, goff, gent, ghost, twin, dQL, qshift, dQ);
 }
 
 template <int ND>
@@ -205,7 +215,7 @@ template <typename T>
 struct EnsArgs {
     flux::MeshView<T> m;
     T* grad;       // arena [cells][ND][NC]
-    double* dQL;   // arena [faces][NC]
+    T* dQL;        // arena [faces][NC]
     double* dQ;    // [C][NC]
     double* Q;     // [C][NC]
     double* W;     // [C][NC] (the same array as m.W)

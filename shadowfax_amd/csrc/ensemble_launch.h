@@ -22,7 +22,7 @@ hipError_t ens_step_impl(int ndim, int kind, const EnsArgs<T>& a, hipStream_t s)
         else                                                                                   \
             hipLaunchKernelGGL((ens::drain_faces<ND, 1, MIXED, T, double>), g, b, 0, s, a.m,    \
                                a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan);               \
-        hipLaunchKernelGGL((ens::drain_gather<ND, MIXED>), g, b, 0, s, a.m.gather_off,          \
+        hipLaunchKernelGGL((ens::drain_gather<ND, MIXED, T>), g, b, 0, s, a.m.gather_off,          \
                            a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, a.plan);         \
         hipLaunchKernelGGL((ens::drain_update<ND>), g, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,     \
                            a.W, a.cstatus, a.ctr, a.plan);                                      \
@@ -57,7 +57,7 @@ hipError_t ens_range_impl(int ndim, int kind, const EnsArgs<T>& a, int64_t cell0
                                    a.m, a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, face0,        \
                                    nface, gshift, qshift);                                      \
         }                                                                                      \
-        hipLaunchKernelGGL((ens::range_gather<ND, MIXED>), gc, b, 0, s, a.m.gather_off,         \
+        hipLaunchKernelGGL((ens::range_gather<ND, MIXED, T>), gc, b, 0, s, a.m.gather_off,         \
                            a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, cell0, ncell,    \
                            qshift);                                                             \
         hipLaunchKernelGGL((ens::range_update<ND>), gc, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,    \

@@ -57,10 +57,10 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
                               hipStream_t s) {
     if(C <= 0) return hipSuccess;
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 0>), grid_for(C, 256), dim3(256), 0, s, C, F,
+        hipLaunchKernelGGL((flux::gather_kernel<3, 0, double>), grid_for(C, 256), dim3(256), 0, s, C, F,
                            goff, gent, nullptr, nullptr, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 0>), grid_for(C, 256), dim3(256), 0, s, C, F,
+        hipLaunchKernelGGL((flux::gather_kernel<2, 0, double>), grid_for(C, 256), dim3(256), 0, s, C, F,
                            goff, gent, nullptr, nullptr, dQL, dQ);
     return hipGetLastError();
 }

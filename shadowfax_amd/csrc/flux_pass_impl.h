@@ -22,7 +22,7 @@
 //
 // Layout: everything a kernel gathers through a neighbour index is
 // item-major (one cache line per gather): W [C][NC], vel [C][ND],
-// grad [C][ND][NC], dQ [C][NC], Q [C][NC], dQL [F][NC]; per-face geometry,
+// grad [C][ND][NC], dQ [C][NC], Q [C][NC], dQL [F][NC] (T); per-face geometry,
 // read one face per lane, is component-major [ND][F]; K2 reads each cell's
 // faces through per-entry records (entry_geometry_kernel) stored in the
 // cell's walk order, so its loads stream instead of chasing face indices.
@@ -403,7 +403,7 @@ template <int ND, int KIND, int MIXED, typename T, typename R>
 __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                                           const T* __restrict__ grad, int64_t gshift,
                                           const Consts& cd, const mixed::CF& cf,
-                                          double* __restrict__ dQL, int64_t qshift,
+                                          T* __restrict__ dQL, int64_t qshift,
                                           uint8_t* __restrict__ fstatus,
                                           unsigned (&cnt)[CNT_N]) {
     const int NC = ND + 2, P = ND + 1;
@@ -576,7 +576,7 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
             fail = st != 0;
         }
 #pragma unroll
-        for(int k = 0; k < NC; k++) dQL[(f + qshift) * NC + k] = out[k];
+        for(int k = 0; k < NC; k++) dQL[(f + qshift) * NC + k] = (T)out[k];
         if(fstatus) fstatus[f] = st;
     }
     cnt[CNT_NEVAL] += solved;
@@ -589,7 +589,7 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
 template <int ND, int KIND, int MIXED, typename T, typename R>
 __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
                                                         Consts cd, mixed::CF cf,
-                                                        double* __restrict__ dQL,
+                                                        T* __restrict__ dQL,
                                                         uint8_t* __restrict__ fstatus,
                                                         unsigned long long* __restrict__ ctr) {
     const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
@@ -600,13 +600,14 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
 
 // ------------------------------------------------------------ K4 gather --
 
-// Cell i: dQ[i] = its entries' dQL[(f + qshift)] in ascending face order.
-template <int ND, int MIXED>
+// Cell i: dQ[i] = its entries' dQL[(f + qshift)] in ascending face order,
+// summed in FP64 (T: storage of dQL).
+template <int ND, int MIXED, typename T>
 __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict__ goff,
                                             const int32_t* __restrict__ gent,
                                             const uint8_t* __restrict__ ghost,
                                             const int32_t* __restrict__ twin,
-                                            const double* __restrict__ dQL, int64_t qshift,
+                                            const T* __restrict__ dQL, int64_t qshift,
                                             double* __restrict__ dQ) {
     const int NC = ND + 2;
     double acc[NC];
@@ -626,7 +627,7 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
         }
 #pragma unroll
         for(int k = 0; k < NC; k++) {
-            const double x = dQL[((int64_t)f + qshift) * NC + k];
+            const double x = (double)dQL[((int64_t)f + qshift) * NC + k];
             acc[k] += neg ? -x : x;
         }
     }
@@ -634,17 +635,17 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
     for(int k = 0; k < NC; k++) dQ[i * NC + k] = acc[k];
 }
 
-template <int ND, int MIXED>
+template <int ND, int MIXED, typename T>
 __global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
                                                      const int64_t* __restrict__ goff,
                                                      const int32_t* __restrict__ gent,
                                                      const uint8_t* __restrict__ ghost,
                                                      const int32_t* __restrict__ twin,
-                                                     const double* __restrict__ dQL,
+                                                     const T* __restrict__ dQL,
                                                      double* __restrict__ dQ) {
     (void)F;
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if(i < C) gather_cell<ND, MIXED>(i, goff, gent, ghost, twin, dQL, 0, dQ);
+    if(i < C) gather_cell<ND, MIXED, T>(i, goff, gent, ghost, twin, dQL, 0, dQ);
 }
 
 // ------------------------------------------------------------ K5 update --

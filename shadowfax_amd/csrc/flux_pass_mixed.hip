@@ -6,10 +6,12 @@
 // FP32-rounded states: rounding the states alone gave 6e-6 face errors);
 // gradient estimation (in the closure-free difference form), limiting and
 // the MUSCL-Hancock predictor evaluated in FP64 registers; the mixed Riemann
-// solvers (riemann_mixed_impl.h); FP64 face fluxes and FP64 accumulation.  Each periodic boundary face is evaluated
-// once and applied with opposite signs to its two cells, so mass, momentum
-// and energy are conserved to FP64 round-off (the reference evaluates both
-// copies, src/VorFace.cpp:913-948).
+// solvers (riemann_mixed_impl.h); face fluxes dt A F formed in FP64 and
+// stored in FP32 (a 6e-8 relative rounding per face; each stored value
+// enters its two cells with opposite signs); FP64 accumulation.  Each
+// periodic boundary face is evaluated once and applied with opposite signs
+// to its two cells, so mass, momentum and energy are conserved to FP64
+// round-off (the reference evaluates both copies, src/VorFace.cpp:913-948).
 #include "flux_pass_impl.h"
 #include "sfx_launch.h"
 
@@ -45,7 +47,7 @@ hipError_t launch_gradients<float>(int ndim, const flux::MeshView<float>& m, flo
 
 template <>
 hipError_t launch_face_fluxes<float>(int ndim, int kind, const flux::MeshView<float>& m,
-                                     const float* grad, const Consts& c, double* dQL,
+                                     const float* grad, const Consts& c, float* dQL,
                                      uint8_t* fstatus, unsigned long long* ctr, hipStream_t s) {
     if(m.F <= 0) return hipSuccess;
     const mixed::CF cf = mixed::make_cf(c);
@@ -62,13 +64,13 @@ hipError_t launch_face_fluxes<float>(int ndim, int kind, const flux::MeshView<fl
 
 hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* goff,
                                const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
-                               const double* dQL, double* dQ, hipStream_t s) {
+                               const float* dQL, double* dQ, hipStream_t s) {
     if(C <= 0) return hipSuccess;
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 1>), grid_for(C, 256), dim3(256), 0, s, C, F,
+        hipLaunchKernelGGL((flux::gather_kernel<3, 1, float>), grid_for(C, 256), dim3(256), 0, s, C, F,
                            goff, gent, ghost, twin, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 1>), grid_for(C, 256), dim3(256), 0, s, C, F,
+        hipLaunchKernelGGL((flux::gather_kernel<2, 1, float>), grid_for(C, 256), dim3(256), 0, s, C, F,
                            goff, gent, ghost, twin, dQL, dQ);
     return hipGetLastError();
 }

@@ -40,7 +40,7 @@ struct sfx_ensemble {
     // scratch arena
     int64_t arena_cells = 0, arena_faces = 0;
     void* grad = nullptr;
-    double* dQL = nullptr;
+    void* dQL = nullptr;
     // batching
     int32_t* h_batch = nullptr;  // pinned [1 + kMaxBatch]
     int32_t* d_batch = nullptr;
@@ -113,7 +113,7 @@ EnsArgs<T> ens_args(const sfx_ensemble* e) {
     EnsArgs<T> a;
     a.m = ens_view<T>(e);
     a.grad = (T*)e->grad;
-    a.dQL = e->dQL;
+    a.dQL = (T*)e->dQL;
     a.dQ = e->dQ;
     a.Q = e->Q;
     a.W = e->W;
@@ -330,7 +330,7 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     A(&e->d_f0, 8 * max_meshes);
     A(&e->d_nf, 8 * max_meshes);
     A(&e->grad, t * arena_cells * nd * NC);
-    A(&e->dQL, 8 * arena_faces * NC);
+    A(&e->dQL, t * arena_faces * NC);
     A(&e->d_batch, sizeof(int32_t) * (1 + ens::kMaxBatch));
     A(&e->d_plan, sizeof(ens::Plan));
     if(err == hipSuccess) err = hipHostMalloc((void**)&e->h_batch,

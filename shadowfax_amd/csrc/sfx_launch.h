@@ -39,7 +39,7 @@ template <typename T>
 hipError_t launch_gradients(int ndim, const flux::MeshView<T>& m, T* grad, hipStream_t s);
 template <typename T>
 hipError_t launch_face_fluxes(int ndim, int kind, const flux::MeshView<T>& m, const T* grad,
-                              const Consts& c, double* dQL, uint8_t* fstatus,
+                              const Consts& c, T* dQL, uint8_t* fstatus,
                               unsigned long long* ctr, hipStream_t s);
 hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
                          const double* volume, double* W, uint8_t* cstatus,
@@ -49,6 +49,6 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
                               hipStream_t s);
 hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* goff,
                                const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
-                               const double* dQL, double* dQ, hipStream_t s);
+                               const float* dQL, double* dQ, hipStream_t s);
 
 }  // namespace sfx

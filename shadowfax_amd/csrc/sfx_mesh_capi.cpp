@@ -251,7 +251,7 @@ int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream) {
     return SFX_OK;
 }
 
-int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad, double* dQL,
+int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad, void* dQL,
                     uint8_t* face_status, void* stream) {
     int rc = check_mesh(h, m, "sfx_face_fluxes", true, false, false);
     if(rc) return rc;
@@ -260,16 +260,18 @@ int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad, double* 
     unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
     hipError_t e = (h->precision == SFX_PREC_FP64)
                            ? launch_face_fluxes<double>(h->ndim, h->kind, view<double>(m),
-                                                        (const double*)grad, h->c, dQL,
+                                                        (const double*)grad, h->c,
+                                                        (double*)dQL,
                                                         face_status, ctr, (hipStream_t)stream)
                            : launch_face_fluxes<float>(h->ndim, h->kind, view<float>(m),
-                                                       (const float*)grad, h->c, dQL,
+                                                       (const float*)grad, h->c,
+                                                       (float*)dQL,
                                                        face_status, ctr, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "face flux kernel launch");
     return SFX_OK;
 }
 
-int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL, double* dQ,
+int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const void* dQL, double* dQ,
                   void* stream) {
     int rc = check_mesh(h, m, "sfx_gather_dQ", false, false, true);
     if(rc) return rc;
@@ -277,15 +279,16 @@ int sfx_gather_dQ(sfx_solver* h, const sfx_mesh* m, const double* dQL, double* d
     if((rc = bind_device(h))) return rc;
     hipError_t e =
             (h->precision == SFX_PREC_FP64)
-                    ? launch_gather_fp64(h->ndim, m->C, m->F, m->gather_off, m->gather_entry, dQL,
-                                         dQ, (hipStream_t)stream)
+                    ? launch_gather_fp64(h->ndim, m->C, m->F, m->gather_off, m->gather_entry,
+                                         (const double*)dQL, dQ, (hipStream_t)stream)
                     : launch_gather_mixed(h->ndim, m->C, m->F, m->gather_off, m->gather_entry,
-                                          m->ghost, m->twin, dQL, dQ, (hipStream_t)stream);
+                                          m->ghost, m->twin, (const float*)dQL, dQ,
+                                          (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "gather kernel launch");
     return SFX_OK;
 }
 
-int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, double* dQL, double* dQ,
+int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, void* dQL, double* dQ,
                        uint8_t* face_status, void* stream) {
     int rc = check_mesh(h, m, "sfx_face_flux_pass", true, true, true);
     if(rc) return rc;

@@ -318,7 +318,7 @@ class DeviceMesh:
         self.egeo = torch.zeros((self.E, 8 if nd == 2 else 12), dtype=tdt, device=self.dev)
         self.egeo_built = False
         self.grad = torch.zeros((self.C, nd, NC), dtype=tdt, device=self.dev)
-        self.dQL = torch.zeros((self.F, NC), dtype=torch.float64, device=self.dev)
+        self.dQL = torch.zeros((self.F, NC), dtype=tdt, device=self.dev)
         self.dQ = torch.zeros((self.C, NC), dtype=torch.float64, device=self.dev)
         self.fstatus = torch.zeros(self.F, dtype=torch.uint8, device=self.dev)
         self.cstatus = torch.zeros(self.C, dtype=torch.uint8, device=self.dev)
@@ -353,7 +353,7 @@ class DeviceMesh:
         return self.grad.double().cpu().numpy().transpose(1, 2, 0).copy()
 
     def dQL_soa(self):
-        return self.dQL.cpu().numpy().T.copy()
+        return self.dQL.double().cpu().numpy().T.copy()
 
     def dQ_soa(self):
         return self.dQ.cpu().numpy().T.copy()

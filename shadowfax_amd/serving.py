@@ -229,3 +229,43 @@ def poisson_trace(nreq, rate, n_meshes, seed):
     gaps = -np.log1p(-uniform(seed, 320, ctr)) / rate
     mesh = np.minimum((uniform(seed, 321, ctr) * n_meshes).astype(np.int64), n_meshes - 1)
     return np.cumsum(gaps), mesh.astype(np.int32)
+
+
+def config4_classes(w):
+    """One tessellated, prepared mesh per size class in use."""
+    return {int(k): class_bundle(w["sides"][k], w["seed"], w["gamma"], w["kind"])
+            for k in np.unique(w["cls"])}
+
+
+def build_config4(solver, dev, w, classes=None, arena_cells=4_000_000, log=None):
+    """The config-4 ensemble on the device for one solver (precision): every
+    mesh's own states on its class's tessellation.  Returns (Ensemble,
+    per-mesh faces)."""
+    import torch
+    g, kind = w["gamma"], w["kind"]
+    classes = classes or config4_classes(w)
+    tmpl = {k: (b, M.DeviceMesh(b, solver.precision, dev)) for k, b in classes.items()}
+    sizes = [(tmpl[k][0]["C"], tmpl[k][0]["F"], int(tmpl[k][0]["cf_off"][-1])) for k in w["cls"]]
+    C = sum(x[0] for x in sizes)
+    F = sum(x[1] for x in sizes)
+    E = sum(x[2] for x in sizes)
+    cmax = max(x[0] for x in sizes)
+    ac = max(arena_cells, cmax)
+    af = max(int(3.2 * ac), max(x[1] for x in sizes))
+    ens = Ensemble(solver, w["n"], C, F, E, ac, af)
+    tdt = torch.float64 if solver.precision == "fp64" else torch.float32
+    faces = np.zeros(w["n"], np.int64)
+    for i in range(w["n"]):
+        b, dm = tmpl[w["cls"][i]]
+        pos = np.mod(b["pos"] + w["shift"][:, i:i + 1], 1.0)
+        W, vel, Q, dt = mesh_state(b, pos, int(w["ic"][i]), float(w["icu"][i]), g, kind)
+        dm.arr["W"].copy_(torch.from_numpy(np.ascontiguousarray(W.T)))
+        dm.arr["vel"].copy_(torch.from_numpy(np.ascontiguousarray(vel.T)).to(tdt))
+        dm.arr["mass"].copy_(torch.from_numpy(np.ascontiguousarray(Q[0])))
+        dm.arr["dt"].fill_(dt)
+        dm.Q.copy_(torch.from_numpy(np.ascontiguousarray(Q.T)))
+        mid = ens.add(dm)
+        faces[mid] = dm.F
+        if log and (i + 1) % 500 == 0:
+            log("  %d meshes" % (i + 1))
+    return ens, faces
