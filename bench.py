@@ -197,13 +197,14 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
 
 def measured_traffic(kernel, grid):
     """HBM bytes per launch of a kernel from the committed hardware-counter
-    summary (profiles/hbm_traffic.json), if it was measured at this grid."""
+    summary (profiles/hbm_traffic.json), if it was measured (at this grid,
+    unless grid is None)."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
             k = json.load(f)["kernels"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
-    if not k or int(k.get("grid", -1)) != grid:
+    if not k or (grid is not None and int(k.get("grid", -1)) != grid):
         return None
     return float(k["traffic_bytes"])
 
@@ -321,11 +322,15 @@ def main():
     variants = {}
     outs = {}
     only = set(args.only.split(",")) if args.only else None
-    for kname, kind in (("exact", 0), ("hllc", 1)):
-        for prec in ("fp64", "mixed"):
+    for kname, kind, prec in (("exact", 0, "fp64"), ("exact", 0, "mixed"),
+                              ("exact", 0, "mixed_onekernel"), ("hllc", 1, "fp64"),
+                              ("hllc", 1, "mixed")):
             if only is not None and "%s_%s" % (kname, prec) not in only:
                 continue
-            s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"], prec)
+            s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"],
+                              "mixed" if prec.startswith("mixed") else prec)
+            if prec == "mixed_onekernel":
+                s.set_split(0)  # without the wave-level compaction, for comparison
             s.enable_stats(False)
             src = d64 if prec == "fp64" else d32
             F = torch.empty_like(src["WL"])
@@ -366,12 +371,15 @@ def main():
     value = N / (ms * 1e-3)
     bytes_per = (2 * 5 + 3) * 4 + 5 * 4  # WL, WR, n in + F out, FP32
     achieved = bytes_per * N / (ms * 1e-3) / 1e9
-    traffic = measured_traffic("sfx::mixed::solve_for_flux_kernel<3, 0, float>", N)
+    kernels = ["sfx::mixed::exact_phase1_kernel<3, float>",
+               "sfx::mixed::exact_phase2_kernel<3, float>"]
+    parts = [measured_traffic(k, None) for k in kernels]
+    traffic = sum(parts) if all(p is not None for p in parts) else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": "profiles/hbm_traffic_pmc.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                               "scaled by a copy kernel of known size)" if traffic else None,
-            "kernel": "sfx::mixed::solve_for_flux_kernel<3,0,float>",
+            "kernel": " + ".join(kernels) + " (one step: the compacted exact solve)",
             "bytes_per_problem": bytes_per,
             "note": "exact solver is VALU-bound (Newton on FP32 transcendentals + one FP64 "
                     "step); HBM fraction shown as the contract asks"}

@@ -95,6 +95,16 @@ int sfx_solver_reset_stats(sfx_solver* h);
 /* counters are on by default; off removes their atomics from the kernels */
 int sfx_solver_enable_stats(sfx_solver* h, int enabled);
 
+/* Wave-level compaction of the mixed exact solver's batched path: every
+ * problem gets `iterations` FP32 Newton iterations in the first kernel;
+ * the ones still iterating (strong shocks, near vacuum) are queued with
+ * their state and finished, densely packed, by a persistent second kernel,
+ * so that a few slow lanes do not hold up their wavefront.  0 runs one
+ * kernel.  Results are identical either way.  Default 4.  (No reference
+ * counterpart: the reference solves one problem at a time.) */
+int sfx_solver_set_split(sfx_solver* h, int iterations);
+int sfx_solver_get_split(const sfx_solver* h);
+
 /* ---- scalar drop-ins (host pointers, one problem on the GPU) --------- */
 
 /* RiemannSolver::solve(WL, WR, n, mach, reflective)
@@ -135,6 +145,26 @@ int sfx_exact_solve_batch(sfx_solver* h, int64_t N, const double* WL,
                           const uint8_t* reflective, double* Whalf,
                           double* mach, uint8_t* status, void* stream);
 
+
+/* RiemannSolver::get_time_derivative (src/riemann/RiemannSolver.hpp:89-90;
+ * ExactRiemannSolver.cpp:892-934, HLLCRiemannSolver.cpp:415-450, including
+ * the 2D dWdt[2] that uses gradients[1].vy twice) over N states, FP64.
+ * W, dWdt: [ndim+2][N]; grad: [ndim][ndim+2][N] (dW_k/dx_d). */
+int sfx_time_derivative_batch(sfx_solver* h, int64_t N, const double* W,
+                              const double* grad, double* dWdt, void* stream);
+
+/* RiemannSolver::get_W(volume, Q, use_energy = true)
+ * (src/riemann/RiemannSolver.hpp:120-121; ExactRiemannSolver.cpp:716-766)
+ * over N cells, FP64: W, Q [ndim+2][N], volume [N].  Where the pressure
+ * floor acts, Q's energy is reset, as the reference does through Q&. */
+int sfx_get_W_batch(sfx_solver* h, int64_t N, const double* volume, double* Q,
+                    double* W, void* stream);
+
+/* RiemannSolver::get_Q(volume, W) (src/riemann/RiemannSolver.hpp:108;
+ * ExactRiemannSolver.cpp:776-804, HLLCRiemannSolver.cpp:471-483) over N
+ * cells, FP64. */
+int sfx_get_Q_batch(sfx_solver* h, int64_t N, const double* volume,
+                    const double* W, double* Q, void* stream);
 
 /* ---- meshes (host side; tessellation stays on the host) -------------- */
 

@@ -82,6 +82,11 @@ def lib():
     L.sfx_solve_for_flux_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     L.sfx_solve_for_flux_batch_f32.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     L.sfx_exact_solve_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.sfx_time_derivative_batch.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.sfx_solver_set_split.argtypes = [vp, ct.c_int]
+    L.sfx_solver_get_split.argtypes = [vp]
+    L.sfx_get_W_batch.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.sfx_get_Q_batch.argtypes = [vp, i64, vp, vp, vp, vp]
     for name, args in _mesh_sigs().items():
         if hasattr(L, name):
             getattr(L, name).argtypes = args

@@ -33,6 +33,7 @@ SOURCES = [
     ("riemann_mixed.hip", ["-ffp-contract=on",
                            "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("flux_pass.hip", ["-ffp-contract=off"]),
+    ("cell_ops.hip", ["-ffp-contract=off"]),
     ("flux_pass_mixed.hip", ["-ffp-contract=on",
                              "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sfx_capi.cpp", []),

@@ -651,6 +651,49 @@ __global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
 // ------------------------------------------------------------ K5 update --
 
 // GasParticle::update_Q then get_W (use_energy = true).
+// get_W with use_energy (ExactRiemannSolver.cpp:716-766, the same
+// expressions as HLLCRiemannSolver.cpp:495-535): primitives w of a cell of
+// volume V from its conserved q; where the pressure floor acts, q's energy is
+// reset to match, as the reference does through its Q reference.
+template <int ND>
+__device__ __forceinline__ void get_W_state(double g, double V, double* q, double* w) {
+    const int NC = ND + 2, P = ND + 1;
+#pragma unroll
+    for(int k = 0; k < NC; k++) w[k] = 0.;
+    if(q[0] != 0.) {
+        w[0] = q[0] / V;
+#pragma unroll
+        for(int d = 0; d < ND; d++) w[1 + d] = q[1 + d] / q[0];
+        double p2 = q[1] * q[1] + q[2] * q[2];
+        if(ND == 3) p2 = q[1] * q[1] + q[2] * q[2] + q[ND] * q[ND];
+        w[P] = (g - 1.) * (q[P] - 0.5 * p2 / q[0]) / V;
+        if(w[P] < 1.e-30) {
+            w[P] = 1.e-30;
+            q[P] = 0.5 * p2 / q[0] + w[P] * V / (g - 1.);
+        }
+        if(w[0] < 1.e-30) {
+#pragma unroll
+            for(int k = 0; k < NC; k++) w[k] = 0.;
+        }
+    }
+}
+
+// get_Q (ExactRiemannSolver.cpp:776-804 / HLLCRiemannSolver.cpp:471-483:
+// the two associate the energy differently).
+template <int ND, int KIND>
+__device__ __forceinline__ void get_Q_state(double g, double V, const double* w, double* q) {
+    const int P = ND + 1;
+    q[0] = w[0] * V;
+#pragma unroll
+    for(int d = 0; d < ND; d++) q[1 + d] = q[0] * w[1 + d];
+    double v2 = w[1] * w[1] + w[2] * w[2];
+    if(ND == 3) v2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+    if(KIND == 1)
+        q[P] = V * w[P] / (g - 1.) + 0.5 * q[0] * v2;
+    else
+        q[P] = (0.5 * w[0] * v2 + w[P] / (g - 1.)) * V;
+}
+
 // Cell i; returns 1 when m or e went negative.
 template <int ND>
 __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
@@ -668,25 +711,7 @@ __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
             dQ[i * NC + k] = 0.0;
         }
         uint8_t st = (q[0] < 0. || q[P] < 0.) ? ST_UPDATEQ : 0;
-        const double g = cd.gamma, V = volume[i];
-#pragma unroll
-        for(int k = 0; k < NC; k++) w[k] = 0.;
-        if(q[0] != 0.) {
-            w[0] = q[0] / V;
-#pragma unroll
-            for(int d = 0; d < ND; d++) w[1 + d] = q[1 + d] / q[0];
-            double p2 = q[1] * q[1] + q[2] * q[2];
-            if(ND == 3) p2 = q[1] * q[1] + q[2] * q[2] + q[ND] * q[ND];
-            w[P] = (g - 1.) * (q[P] - 0.5 * p2 / q[0]) / V;
-            if(w[P] < 1.e-30) {
-                w[P] = 1.e-30;
-                q[P] = 0.5 * p2 / q[0] + w[P] * V / (g - 1.);
-            }
-            if(w[0] < 1.e-30) {
-#pragma unroll
-                for(int k = 0; k < NC; k++) w[k] = 0.;
-            }
-        }
+        get_W_state<ND>(cd.gamma, volume[i], q, w);
 #pragma unroll
         for(int k = 0; k < NC; k++) {
             Q[i * NC + k] = q[k];

@@ -3,6 +3,8 @@
 #include "riemann_mixed_impl.h"
 #include "sfx_launch.h"
 
+#include <algorithm>
+
 namespace sfx {
 namespace mixed {
 
@@ -51,7 +53,187 @@ __global__ __launch_bounds__(256) void solve_for_flux_kernel(
     block_add_counters(ctr, cv);
 }
 
+// ---- exact solver in two phases (wave-level compaction) -----------------
+//
+// Phase 1 runs every problem's setup and at most `stop` FP32 Newton
+// iterations; problems still iterating after that (strong shocks, near
+// vacuum) are appended to a queue -- one atomic per wave, slots from a
+// ballot -- with their iteration state, and the wave finishes the converged
+// ones together instead of idling while a few lanes iterate on.  Phase 2, a
+// persistent grid, takes the queue in dense chunks, resumes each problem
+// where it stopped and finishes it.  The iteration sequence per problem is
+// unchanged, so results are bit-identical to the one-kernel path.
+
+struct Straggler {
+    int32_t i;
+    float p, lo, hi;
+    int32_t it;
+};
+
+__global__ void reset_queue_kernel(unsigned* q) {
+    if(threadIdx.x < 2) q[threadIdx.x] = 0u;
+}
+
+template <int ND, typename T>
+__device__ __forceinline__ void load_problem(int64_t N, int64_t i, const T* __restrict__ WL,
+                                             const T* __restrict__ WR, const T* __restrict__ n,
+                                             const T* __restrict__ v, double* wl64, double* wr64,
+                                             double* n64, double* v64, float* wl, float* wr,
+                                             float* nn) {
+    const int NC = ND + 2;
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        wl64[k] = (double)WL[k * N + i];
+        wr64[k] = (double)WR[k * N + i];
+        wl[k] = (float)wl64[k];
+        wr[k] = (float)wr64[k];
+    }
+#pragma unroll
+    for(int d = 0; d < ND; d++) {
+        n64[d] = (double)n[d * N + i];
+        v64[d] = v ? (double)v[d * N + i] : 0.0;
+        nn[d] = (float)n64[d];
+    }
+}
+
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void exact_phase1_kernel(
+        CF c, Consts cd, int64_t N, const T* __restrict__ WL, const T* __restrict__ WR,
+        const T* __restrict__ n, const T* __restrict__ v, const uint8_t* __restrict__ refl,
+        T* __restrict__ F, uint8_t* __restrict__ status, unsigned long long* __restrict__ ctr,
+        Straggler* __restrict__ queue, unsigned* __restrict__ qcount, int stop) {
+    const int NC = ND + 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < N;
+    unsigned iters = 0, vac = 0, fail = 0, done = 0;
+    double wl64[NC], wr64[NC], n64[ND], v64[ND];
+    float wl[NC], wr[NC], nn[ND];
+    ExactCase s;
+    s.phase = EX_DONE;
+    if(valid) {
+        load_problem<ND, T>(N, i, WL, WR, n, v, wl64, wr64, n64, v64, wl, wr, nn);
+        exact_setup<ND>(c, cd, wl, wr, nn, refl ? refl[i] != 0 : false, wl64, wr64, n64, s);
+        if(s.phase == EX_ITER) exact_iterate(c, s, stop);
+    }
+    const bool strag = valid && s.phase == EX_ITER;
+    // wave-wide append of the stragglers
+    const unsigned long long mask = __ballot(strag);
+    const int lane = __lane_id();
+    unsigned base = 0;
+    if(mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if(lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    if(strag) {
+        const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+        Straggler g;
+        g.i = (int32_t)i;
+        g.p = s.p;
+        g.lo = s.lo;
+        g.hi = s.hi;
+        g.it = s.it;
+        queue[slot] = g;
+    } else if(valid) {
+        double f[NC];
+        const uint8_t st = exact_finish<ND>(c, cd, s, wl64, wr64, n64, v64, f, iters, vac);
+#pragma unroll
+        for(int k = 0; k < NC; k++) F[k * N + i] = (T)f[k];
+        if(status) status[i] = st;
+        fail = st != 0;
+        done = 1;
+    }
+    const unsigned cv[CNT_N] = {done, 0u, vac, iters, fail};
+    block_add_counters(ctr, cv);
+}
+
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void exact_phase2_kernel(
+        CF c, Consts cd, int64_t N, const T* __restrict__ WL, const T* __restrict__ WR,
+        const T* __restrict__ n, const T* __restrict__ v, const uint8_t* __restrict__ refl,
+        T* __restrict__ F, uint8_t* __restrict__ status, unsigned long long* __restrict__ ctr,
+        const Straggler* __restrict__ queue, const unsigned* __restrict__ qcount,
+        unsigned* __restrict__ qtake) {
+    const int NC = ND + 2;
+    __shared__ unsigned s_base;
+    const unsigned total = *qcount;
+    unsigned iters = 0, vac = 0, fail = 0, done = 0;
+    for(;;) {
+        if(threadIdx.x == 0) s_base = atomicAdd(qtake, blockDim.x);
+        __syncthreads();
+        const unsigned base = s_base;
+        __syncthreads();
+        if(base >= total) break;
+        const unsigned k = base + threadIdx.x;
+        if(k < total) {
+            const Straggler g = queue[k];
+            const int64_t i = g.i;
+            double wl64[NC], wr64[NC], n64[ND], v64[ND], f[NC];
+            float wl[NC], wr[NC], nn[ND];
+            load_problem<ND, T>(N, i, WL, WR, n, v, wl64, wr64, n64, v64, wl, wr, nn);
+            ExactCase s;
+            exact_setup<ND>(c, cd, wl, wr, nn, refl ? refl[i] != 0 : false, wl64, wr64, n64, s);
+            s.p = g.p;
+            s.lo = g.lo;
+            s.hi = g.hi;
+            s.it = g.it;
+            s.phase = EX_ITER;
+            exact_iterate(c, s, c.itmax);
+            const uint8_t st = exact_finish<ND>(c, cd, s, wl64, wr64, n64, v64, f, iters, vac);
+#pragma unroll
+            for(int q = 0; q < NC; q++) F[q * N + i] = (T)f[q];
+            if(status) status[i] = st;
+            fail += st != 0;
+            done++;
+        }
+    }
+    const unsigned cv[CNT_N] = {done, 0u, vac, iters, fail};
+    block_add_counters(ctr, cv);
+}
+
 }  // namespace mixed
+
+size_t exact_split_scratch_bytes(int64_t N) {
+    return 256 + sizeof(mixed::Straggler) * (size_t)std::max<int64_t>(N, 1);
+}
+
+template <typename T>
+hipError_t launch_exact_mixed_split(int ndim, const Consts& c, int64_t N, const T* WL,
+                                    const T* WR, const T* n, const T* v, const uint8_t* refl,
+                                    T* F, uint8_t* status, unsigned long long* ctr,
+                                    void* scratch, int stop, int persistent_blocks,
+                                    hipStream_t stream) {
+    if(N <= 0) return hipSuccess;
+    const mixed::CF cf = mixed::make_cf(c);
+    unsigned* q = (unsigned*)scratch;  // [0] appended, [1] taken
+    mixed::Straggler* queue = (mixed::Straggler*)((char*)scratch + 256);
+    hipLaunchKernelGGL(mixed::reset_queue_kernel, dim3(1), dim3(64), 0, stream, q);
+    const dim3 grid((unsigned)((N + 255) / 256)), block(256), pgrid((unsigned)persistent_blocks);
+#define SFX_SPLIT(ND)                                                                          \
+    do {                                                                                       \
+        hipLaunchKernelGGL((mixed::exact_phase1_kernel<ND, T>), grid, block, 0, stream, cf, c, N, \
+                           WL, WR, n, v, refl, F, status, ctr, queue, q, stop);                 \
+        hipLaunchKernelGGL((mixed::exact_phase2_kernel<ND, T>), pgrid, block, 0, stream, cf, c,  \
+                           N, WL, WR, n, v, refl, F, status, ctr, queue, q, q + 1);             \
+    } while(0)
+    if(ndim == 3)
+        SFX_SPLIT(3);
+    else
+        SFX_SPLIT(2);
+#undef SFX_SPLIT
+    return hipGetLastError();
+}
+
+template hipError_t launch_exact_mixed_split<float>(int, const Consts&, int64_t, const float*,
+                                                    const float*, const float*, const float*,
+                                                    const uint8_t*, float*, uint8_t*,
+                                                    unsigned long long*, void*, int, int,
+                                                    hipStream_t);
+template hipError_t launch_exact_mixed_split<double>(int, const Consts&, int64_t, const double*,
+                                                     const double*, const double*, const double*,
+                                                     const uint8_t*, double*, uint8_t*,
+                                                     unsigned long long*, void*, int, int,
+                                                     hipStream_t);
 
 template <typename T>
 hipError_t launch_solve_for_flux_mixed(int kind, int ndim, const Consts& c, int64_t N,

@@ -176,11 +176,30 @@ __device__ __forceinline__ void state_flux_d(double gm1inv, double rho, const do
 
 // Exact Riemann solver, mixed precision.  WL/WR: FP32 copies for the
 // iteration; WL64/WR64/n64/v64: the same values in FP64.  F: FP64 flux.
+// The exact solver in three parts so that a batch can pause the FP32
+// iteration of its slow problems and resume them elsewhere (wave-level
+// compaction): exact_setup (normal velocities, sound speeds, the vacuum
+// test, the initial guess), exact_iterate (bracketed Newton from a saved
+// state, up to a given iteration count) and exact_finish (the FP64 Newton
+// step, sampling and flux).  exact_mixed composes them.
+
+enum : uint8_t { EX_ITER = 0, EX_DONE = 1, EX_STUCK = 2, EX_VACUUM = 3 };
+
+struct ExactCase {
+    double uLd, uRd, aLd, aRd;  // FP64 normal velocities and sound speeds
+    SideF sL, sR;               // FP32 side data of the pressure function
+    float du;                   // uR - uL
+    bool pure_newton;           // f(0) < cutoff: the reference's Newton-only scheme
+    float p, lo, hi;            // iterate and bracket
+    int it;                     // iterations done
+    uint8_t phase;
+};
+
 template <int ND>
-__device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, float* WR,
-                               const float* n, bool reflective, const double* WL64,
-                               double* WR64, const double* n64, const double* v64, double* F,
-                               unsigned& iters, unsigned& vac) {
+__device__ __forceinline__ void exact_setup(const CF& c, const Consts& cd, const float* WL,
+                                            float* WR, const float* n, bool reflective,
+                                            const double* WL64, double* WR64,
+                                            const double* n64, ExactCase& s) {
     const int NC = ND + 2, P = ND + 1;
     float uL = 0.f, uR = 0.f;
 #pragma unroll
@@ -196,7 +215,6 @@ __device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, f
         }
         uR = -uL;
     }
-    double sol[NC];
     const bool vacuum_in = !(WL64[0] != 0.0) || !(WR64[0] != 0.0);
     const float aL = (WL[0] != 0.f) ? sqrtf_fast(c.gamma * WL[P] * rcpf(WL[0])) : 0.f;
     const float aR = (WR[0] != 0.f) ? sqrtf_fast(c.gamma * WR[P] * rcpf(WR[0])) : 0.f;
@@ -211,75 +229,113 @@ __device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, f
     if(reflective) uRd = -uLd;
     const double aLd = (WL64[0] != 0.0) ? sqrt(cd.gamma * WL64[P] / WL64[0]) : 0.0;
     const double aRd = (WR64[0] != 0.0) ? sqrt(cd.gamma * WR64[P] / WR64[0]) : 0.0;
+    s.uLd = uLd;
+    s.uRd = uRd;
+    s.aLd = aLd;
+    s.aRd = aRd;
+    s.it = 0;
     if(vacuum_in || 2. * aLd / (cd.gamma - 1.) + 2. * aRd / (cd.gamma - 1.) <= uRd - uLd) {
-        // closed forms, no iteration: the FP64 reference expressions
+        s.phase = EX_VACUUM;  // closed forms, no iteration
+        return;
+    }
+    s.phase = EX_ITER;
+    s.sL = make_side(c, WL[0], WL[P], aL);
+    s.sR = make_side(c, WR[0], WR[P], aR);
+    const float du = uR - uL;
+    s.du = du;
+    // f(0) decides between the reference's two iteration schemes
+    // (ExactRiemannSolver.cpp:143-176)
+    const double f0d = -(cd.tdgm1 * aLd) + -(cd.tdgm1 * aRd) + (uRd - uLd);
+    s.pure_newton = f0d < cd.cutoff;
+    // initial guess, ExactRiemannSolver.cpp:644-673 (two-rarefaction /
+    // primitive-variable / two-shock)
+    const float pmin = fminf(WL[P], WR[P]), pmax = fmaxf(WL[P], WR[P]);
+    const float psum = WL[P] + WR[P];
+    float ppv = 0.5f * psum - 0.125f * du * (WL[0] + WR[0]) * (aL + aR);
+    const float pfloor = c.halftol * psum;
+    ppv = fmaxf(pfloor, ppv);
+    float p;
+    if(pmax <= 2.f * pmin && pmin <= ppv && ppv <= pmax) {
+        p = ppv;
+    } else if(ppv < pmin) {
+        float num = aL + aR - c.gm1d2 * du;
+        float den = aL * powf_fast(WL[P], -c.gm1d2g) + aR * powf_fast(WR[P], -c.gm1d2g);
+        float z = num * rcpf(den);
+        p = c.fast53 ? ipow5(z) : powf_fast(z, c.tgdgm1);
+    } else {
+        float gL = sqrtf_fast(s.sL.A * rcpf(ppv + s.sL.B));
+        float gR = sqrtf_fast(s.sR.A * rcpf(ppv + s.sR.B));
+        p = (gL * WL[P] + gR * WR[P] - du) * rcpf(gL + gR);
+    }
+    s.p = fmaxf(pfloor, p);
+    s.lo = 0.f;
+    s.hi = __builtin_inff();
+}
+
+// Bracketed Newton in FP32 until convergence, the iteration cap, or `stop`
+// iterations done (then s.phase stays EX_ITER and the state can be resumed).
+// In the reference's pure-Newton scheme a first step below zero is clamped
+// to p = 0, where f' is infinite, and the iteration stops there with p* = 0
+// (ExactRiemannSolver.cpp:146-154); that outcome is reproduced.
+__device__ __forceinline__ void exact_iterate(const CF& c, ExactCase& s, int stop) {
+    float p = s.p, lo = s.lo, hi = s.hi;
+    int it = s.it;
+    uint8_t phase = EX_ITER;
+    for(; it < c.itmax; it++) {
+        if(it >= stop) break;
+        float fl, dl, fr, dr;
+        side_f(c, s.sL, p, fl, dl);
+        side_f(c, s.sR, p, fr, dr);
+        const float fv = fl + fr + s.du;
+        if(fv < 0.f) lo = p;
+        else hi = p;
+        float pn = p - fv * rcpf(dl + dr);
+        if(it == 0 && s.pure_newton && fv > 0.f && pn < 0.f) {
+            phase = EX_STUCK;
+            break;
+        }
+        if(!(pn >= lo && pn <= hi))
+            pn = (hi < __builtin_inff()) ? ((lo > 0.f) ? sqrtf_fast(lo) * sqrtf_fast(hi)
+                                                       : 0.01f * hi)
+                                         : 2.f * p;
+        const bool done = fabsf(pn - p) <= 2e-6f * (pn + p);
+        p = pn;
+        if(done) {
+            phase = EX_DONE;
+            break;
+        }
+    }
+    if(it >= c.itmax) phase = EX_DONE;
+    s.p = p;
+    s.lo = lo;
+    s.hi = hi;
+    s.it = it;
+    s.phase = phase;
+}
+
+// FP64 final Newton step, sampling at x/t = 0 and flux; returns the status.
+template <int ND>
+__device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, const ExactCase& s,
+                                                const double* WL64, const double* WR64,
+                                                const double* n64, const double* v64,
+                                                double* F, unsigned& iters, unsigned& vac) {
+    const int NC = ND + 2, P = ND + 1;
+    const double uLd = s.uLd, uRd = s.uRd, aLd = s.aLd, aRd = s.aRd;
+    double sol[NC];
+    if(s.phase == EX_VACUUM) {
+        // closed forms: the FP64 reference expressions
         vac = 1;
         fp64::ex_solve_vacuum<ND>(cd, WL64, WR64, uLd, uRd, aLd, aRd, n64, sol);
     } else {
-        const SideF sL = make_side(c, WL[0], WL[P], aL);
-        const SideF sR = make_side(c, WR[0], WR[P], aR);
-        const float du = uR - uL;
-        // f(0) decides between the reference's two iteration schemes
-        // (ExactRiemannSolver.cpp:143-176)
-        const double f0d = -(cd.tdgm1 * aLd) + -(cd.tdgm1 * aRd) + (uRd - uLd);
-        const bool pure_newton = f0d < cd.cutoff;
-        // initial guess, ExactRiemannSolver.cpp:644-673 (two-rarefaction /
-        // primitive-variable / two-shock)
-        const float pmin = fminf(WL[P], WR[P]), pmax = fmaxf(WL[P], WR[P]);
-        const float psum = WL[P] + WR[P];
-        float ppv = 0.5f * psum - 0.125f * du * (WL[0] + WR[0]) * (aL + aR);
-        const float pfloor = c.halftol * psum;
-        ppv = fmaxf(pfloor, ppv);
-        float p;
-        if(pmax <= 2.f * pmin && pmin <= ppv && ppv <= pmax) {
-            p = ppv;
-        } else if(ppv < pmin) {
-            float num = aL + aR - c.gm1d2 * du;
-            float den = aL * powf_fast(WL[P], -c.gm1d2g) + aR * powf_fast(WR[P], -c.gm1d2g);
-            float z = num * rcpf(den);
-            p = c.fast53 ? ipow5(z) : powf_fast(z, c.tgdgm1);
-        } else {
-            float gL = sqrtf_fast(sL.A * rcpf(ppv + sL.B));
-            float gR = sqrtf_fast(sR.A * rcpf(ppv + sR.B));
-            p = (gL * WL[P] + gR * WR[P] - du) * rcpf(gL + gR);
-        }
-        p = fmaxf(pfloor, p);
-        // Bracketed Newton in FP32.  In the reference's pure-Newton scheme a
-        // first step below zero is clamped to p = 0, where f' is infinite,
-        // and the iteration stops there with p* = 0
-        // (ExactRiemannSolver.cpp:146-154); that outcome is reproduced.
-        float lo = 0.f, hi = __builtin_inff();
-        bool stuck = false;
-        int it = 0;
-        for(; it < c.itmax; it++) {
-            float fl, dl, fr, dr;
-            side_f(c, sL, p, fl, dl);
-            side_f(c, sR, p, fr, dr);
-            const float fv = fl + fr + du;
-            if(fv < 0.f) lo = p;
-            else hi = p;
-            float pn = p - fv * rcpf(dl + dr);
-            if(it == 0 && pure_newton && fv > 0.f && pn < 0.f) {
-                stuck = true;
-                break;
-            }
-            if(!(pn >= lo && pn <= hi))
-                pn = (hi < __builtin_inff()) ? ((lo > 0.f) ? sqrtf_fast(lo) * sqrtf_fast(hi)
-                                                           : 0.01f * hi)
-                                             : 2.f * p;
-            const bool done = fabsf(pn - p) <= 2e-6f * (pn + p);
-            p = pn;
-            if(done) break;
-        }
-        iters += it + 1;
+        iters += s.it + 1;
         double ps, us;
-        if(stuck) {
+        if(s.phase == EX_STUCK) {
             // p* = 0: fK(0) = -2 aK/(g-1)
             ps = 0.0;
             us = 0.5 * (uLd + uRd) + 0.5 * (-(cd.tdgm1 * aRd) - -(cd.tdgm1 * aLd));
         } else {
             // FP64 final Newton step for p* (and u*)
-            const double pd = (double)p;
+            const double pd = (double)s.p;
             double fLd, dLd, fRd, dRd;
             side_f_d(c, WL64[0], WL64[P], aLd, pd, fLd, dLd);
             side_f_d(c, WR64[0], WR64[P], aRd, pd, fRd, dRd);
@@ -332,6 +388,17 @@ __device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, f
 #pragma unroll
     for(int d = 0; d < ND; d++) bad |= (sol[1 + d] != sol[1 + d]);
     return bad ? ST_SOLVER : 0;
+}
+
+template <int ND>
+__device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, float* WR,
+                               const float* n, bool reflective, const double* WL64,
+                               double* WR64, const double* n64, const double* v64, double* F,
+                               unsigned& iters, unsigned& vac) {
+    ExactCase s;
+    exact_setup<ND>(c, cd, WL, WR, n, reflective, WL64, WR64, n64, s);
+    if(s.phase == EX_ITER) exact_iterate(c, s, c.itmax);
+    return exact_finish<ND>(c, cd, s, WL64, WR64, n64, v64, F, iters, vac);
 }
 
 inline CF make_cf(const Consts& c) {

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,23 @@ int set_error(int code, const char* fmt, ...) {
 
 int hip_error(hipError_t e, const char* where) {
     return set_error(SFX_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+int ensure_scratch(sfx_solver* h, size_t bytes) {
+    if(h->scratch_bytes >= bytes) return SFX_OK;
+    // the old buffer may still be read by queued work
+    hipError_t e = hipDeviceSynchronize();
+    if(e != hipSuccess) return hip_error(e, "scratch: synchronize");
+    release_scratch(h);
+    const size_t want = bytes + bytes / 2;
+    e = hipMalloc(&h->d_scratch, want);
+    if(e != hipSuccess) {
+        h->d_scratch = nullptr;
+        return e == hipErrorOutOfMemory ? set_error(SFX_ERR_NOMEM, "scratch: out of memory")
+                                        : hip_error(e, "scratch: hipMalloc");
+    }
+    h->scratch_bytes = want;
+    return SFX_OK;
 }
 
 void release_scratch(sfx_solver* h) {
@@ -101,6 +119,8 @@ int sfx_solver_create(int kind, int ndim, double gamma, double tolerance, double
         delete h;
         return hip_error(e, "hipGetDevice");
     }
+    if(e == hipSuccess)
+        e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     e = hipMalloc(&h->d_ctr, sizeof(unsigned long long) * CNT_N * kCtrSlots);
     if(e != hipSuccess) {
         delete h;
@@ -182,6 +202,16 @@ int sfx_solver_enable_stats(sfx_solver* h, int enabled) {
     return SFX_OK;
 }
 
+int sfx_solver_set_split(sfx_solver* h, int iterations) {
+    if(!h) return set_error(SFX_ERR_ARG, "sfx_solver_set_split: NULL handle");
+    if(iterations < 0 || iterations > 64)
+        return set_error(SFX_ERR_ARG, "sfx_solver_set_split: %d (0..64)", iterations);
+    h->split = iterations;
+    return SFX_OK;
+}
+
+int sfx_solver_get_split(const sfx_solver* h) { return h ? h->split : -1; }
+
 int sfx_solve_for_flux_batch(sfx_solver* h, int64_t N, const double* WL, const double* WR,
                              const double* n, const double* v, const uint8_t* reflective,
                              double* F, uint8_t* status, void* stream) {
@@ -197,7 +227,12 @@ int sfx_solve_for_flux_batch(sfx_solver* h, int64_t N, const double* WL, const d
     if(h->precision == SFX_PREC_FP64)
         e = launch_solve_for_flux_fp64(h->kind, h->ndim, h->c, N, WL, WR, n, v, reflective, F,
                                        status, ctr, (hipStream_t)stream);
-    else
+    else if(h->kind == SFX_EXACT && h->split > 0) {
+        if((rc = ensure_scratch(h, exact_split_scratch_bytes(N)))) return rc;
+        e = launch_exact_mixed_split<double>(h->ndim, h->c, N, WL, WR, n, v, reflective, F,
+                                             status, ctr, h->d_scratch, h->split,
+                                             4 * std::max(h->ncu, 1), (hipStream_t)stream);
+    } else
         e = launch_solve_for_flux_mixed<double>(h->kind, h->ndim, h->c, N, WL, WR, n, v,
                                                 reflective, F, status, ctr, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "solve_for_flux kernel launch");
@@ -215,9 +250,16 @@ int sfx_solve_for_flux_batch_f32(sfx_solver* h, int64_t N, const float* WL, cons
     int rc = bind_device(h);
     if(rc) return rc;
     unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
-    hipError_t e = launch_solve_for_flux_mixed<float>(h->kind, h->ndim, h->c, N, WL, WR, n, v,
-                                                      reflective, F, status, ctr,
-                                                      (hipStream_t)stream);
+    hipError_t e;
+    if(h->kind == SFX_EXACT && h->split > 0) {
+        if((rc = ensure_scratch(h, exact_split_scratch_bytes(N)))) return rc;
+        e = launch_exact_mixed_split<float>(h->ndim, h->c, N, WL, WR, n, v, reflective, F,
+                                            status, ctr, h->d_scratch, h->split,
+                                            4 * std::max(h->ncu, 1), (hipStream_t)stream);
+    } else {
+        e = launch_solve_for_flux_mixed<float>(h->kind, h->ndim, h->c, N, WL, WR, n, v,
+                                               reflective, F, status, ctr, (hipStream_t)stream);
+    }
     if(e != hipSuccess) return hip_error(e, "solve_for_flux_f32 kernel launch");
     return SFX_OK;
 }
@@ -240,6 +282,48 @@ int sfx_exact_solve_batch(sfx_solver* h, int64_t N, const double* WL, const doub
                                            nullptr, mach, status, ctr, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "exact_solve kernel launch");
     return SFX_OK;
+}
+
+// ---- per-cell operations over batches (FP64) -----------------------------
+
+int sfx_time_derivative_batch(sfx_solver* h, int64_t N, const double* W, const double* grad,
+                              double* dWdt, void* stream) {
+    if(!h) return set_error(SFX_ERR_ARG, "sfx_time_derivative_batch: NULL handle");
+    if(N < 0) return set_error(SFX_ERR_ARG, "sfx_time_derivative_batch: N < 0");
+    if(N == 0) return SFX_OK;
+    if(!W || !grad || !dWdt)
+        return set_error(SFX_ERR_ARG, "sfx_time_derivative_batch: NULL array argument");
+    int rc = bind_device(h);
+    if(rc) return rc;
+    hipError_t e = launch_cell_op(0, h->kind, h->ndim, h->c.gamma, N, W, grad, dWdt, nullptr,
+                                  (hipStream_t)stream);
+    return e == hipSuccess ? SFX_OK : hip_error(e, "time derivative kernel launch");
+}
+
+int sfx_get_W_batch(sfx_solver* h, int64_t N, const double* volume, double* Q, double* W,
+                    void* stream) {
+    if(!h) return set_error(SFX_ERR_ARG, "sfx_get_W_batch: NULL handle");
+    if(N < 0) return set_error(SFX_ERR_ARG, "sfx_get_W_batch: N < 0");
+    if(N == 0) return SFX_OK;
+    if(!volume || !Q || !W) return set_error(SFX_ERR_ARG, "sfx_get_W_batch: NULL array");
+    int rc = bind_device(h);
+    if(rc) return rc;
+    hipError_t e = launch_cell_op(1, h->kind, h->ndim, h->c.gamma, N, volume, nullptr, W, Q,
+                                  (hipStream_t)stream);
+    return e == hipSuccess ? SFX_OK : hip_error(e, "get_W kernel launch");
+}
+
+int sfx_get_Q_batch(sfx_solver* h, int64_t N, const double* volume, const double* W, double* Q,
+                    void* stream) {
+    if(!h) return set_error(SFX_ERR_ARG, "sfx_get_Q_batch: NULL handle");
+    if(N < 0) return set_error(SFX_ERR_ARG, "sfx_get_Q_batch: N < 0");
+    if(N == 0) return SFX_OK;
+    if(!volume || !Q || !W) return set_error(SFX_ERR_ARG, "sfx_get_Q_batch: NULL array");
+    int rc = bind_device(h);
+    if(rc) return rc;
+    hipError_t e = launch_cell_op(2, h->kind, h->ndim, h->c.gamma, N, volume, W, Q, nullptr,
+                                  (hipStream_t)stream);
+    return e == hipSuccess ? SFX_OK : hip_error(e, "get_Q kernel launch");
 }
 
 // ---- scalar drop-ins: stage one problem through a pinned buffer -----------

@@ -19,9 +19,12 @@ struct sfx_solver {
     unsigned long long* d_ctr = nullptr;  // CNT_N device counters
     double* d_stage = nullptr;            // scalar entry points: device staging
     double* h_stage = nullptr;            // pinned host staging
-    // face-flux pass work buffers, grown on demand
+    // work buffer (the straggler queue of the two-phase exact solve), grown
+    // on demand
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    int split = 4;  // FP32 iterations before a problem is queued (0: one kernel)
+    int ncu = 0;    // compute units of the device
 };
 
 namespace sfx {
@@ -29,4 +32,5 @@ int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int bind_device(const sfx_solver* h);
 void release_scratch(sfx_solver* h);
+int ensure_scratch(sfx_solver* h, size_t bytes);
 }  // namespace sfx

@@ -27,6 +27,22 @@ hipError_t launch_solve_for_flux_mixed(int kind, int ndim, const Consts& c, int6
                                        const uint8_t* refl, T* F, uint8_t* status,
                                        unsigned long long* ctr, hipStream_t stream);
 
+// cell_ops.hip: op 0 time derivative (a = W, b = grad, c = dW/dt), 1 get_W
+// (a = volume, d = Q in/out, c = W), 2 get_Q (a = volume, b = W, c = Q)
+hipError_t launch_cell_op(int op, int kind, int ndim, double gamma, int64_t N,
+                          const double* a, const double* b, double* c, double* d,
+                          hipStream_t s);
+
+// riemann_mixed.hip: the exact solver in two phases (stop = FP32 iterations
+// before a problem is queued for the persistent second phase)
+size_t exact_split_scratch_bytes(int64_t N);
+template <typename T>
+hipError_t launch_exact_mixed_split(int ndim, const Consts& c, int64_t N, const T* WL,
+                                    const T* WR, const T* n, const T* v, const uint8_t* refl,
+                                    T* F, uint8_t* status, unsigned long long* ctr,
+                                    void* scratch, int stop, int persistent_blocks,
+                                    hipStream_t stream);
+
 // flux pass (flux_pass.hip: T = double; flux_pass_mixed.hip: T = float)
 namespace flux {
 template <typename T>

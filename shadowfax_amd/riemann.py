@@ -189,6 +189,59 @@ class RiemannSolver:
         return Whalf
 
 
+    def set_split(self, iterations):
+        """FP32 Newton iterations before a slow problem of a batched mixed
+        exact solve is queued for the second, compacted kernel (0: one
+        kernel).  Results do not depend on it."""
+        C.check(C.lib().sfx_solver_set_split(self._h, int(iterations)), "set_split")
+
+    def get_split(self):
+        return C.lib().sfx_solver_get_split(self._h)
+
+    # ---- per-cell operations over batches (FP64, device SoA) ----------
+
+    def get_time_derivative_batch(self, W, grad, out=None, stream=None):
+        """RiemannSolver::get_time_derivative for every column of W
+        ([ndim+2][N]) with gradients grad ([ndim][ndim+2][N])."""
+        import torch
+        NC, ND = self.ndim + 2, self.ndim
+        N = W.shape[1]
+        f64 = torch.float64
+        if out is None:
+            out = torch.empty((NC, N), dtype=f64, device=W.device)
+        C.check(C.lib().sfx_time_derivative_batch(
+            self._h, N, _ptr(W, f64, (NC, N), "W"), _ptr(grad, f64, (ND, NC, N), "grad"),
+            _ptr(out, f64, (NC, N), "dWdt"), _stream(stream)), "get_time_derivative_batch")
+        return out
+
+    def get_W_batch(self, volume, Q, W=None, stream=None):
+        """RiemannSolver::get_W(volume, Q, true) per column; Q is updated in
+        place where the pressure floor resets the energy."""
+        import torch
+        NC = self.ndim + 2
+        N = Q.shape[1]
+        f64 = torch.float64
+        if W is None:
+            W = torch.empty((NC, N), dtype=f64, device=Q.device)
+        C.check(C.lib().sfx_get_W_batch(
+            self._h, N, _ptr(volume, f64, (N,), "volume"), _ptr(Q, f64, (NC, N), "Q"),
+            _ptr(W, f64, (NC, N), "W"), _stream(stream)), "get_W_batch")
+        return W
+
+    def get_Q_batch(self, volume, W, Q=None, stream=None):
+        """RiemannSolver::get_Q(volume, W) per column."""
+        import torch
+        NC = self.ndim + 2
+        N = W.shape[1]
+        f64 = torch.float64
+        if Q is None:
+            Q = torch.empty((NC, N), dtype=f64, device=W.device)
+        C.check(C.lib().sfx_get_Q_batch(
+            self._h, N, _ptr(volume, f64, (N,), "volume"), _ptr(W, f64, (NC, N), "W"),
+            _ptr(Q, f64, (NC, N), "Q"), _stream(stream)), "get_Q_batch")
+        return Q
+
+
 class ExactRiemannSolver(RiemannSolver):
     """src/riemann/ExactRiemannSolver.hpp:107: defaults gamma=1.66667,
     tolerance=1e-8, cutoff=-5."""

@@ -190,3 +190,51 @@ def test_uniform_state_gives_euler_flux_gpu(device, kind):
         err = summarize(normwise_error(F, R, WL, WL, G))
         print(kind, prec, err)
         assert err["max"] <= tol
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_cell_operations_match_reference_golden(device, ndim, kind):
+    """get_time_derivative, get_Q and get_W (with the pressure-floor and
+    vacuum edge cases in the fixtures) on the GPU against the compiled
+    reference, bit for bit (no transcendental on these paths)."""
+    import torch
+    g = np.load(os.path.join(HERE, "golden", "riemann_%dd.npz" % ndim))
+    name = ("exact", "hllc")[kind]
+    s = RiemannSolver(kind, ndim, G, 1e-8, -5.0, "fp64")
+    f64 = torch.float64
+    d = s.get_time_derivative_batch(_t(g["dwdt_W"], f64, device), _t(g["dwdt_grad"], f64, device))
+    Q = s.get_Q_batch(_t(g["q_vol"], f64, device), _t(g["q_W"], f64, device))
+    Qin = _t(g["w_Qin_" + name], f64, device)
+    W = s.get_W_batch(_t(g["q_vol"], f64, device), Qin)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy(), g["dwdt_" + name])
+    assert np.array_equal(Q.cpu().numpy(), g["q_Q_" + name])
+    assert np.array_equal(W.cpu().numpy(), g["w_W_" + name])
+    assert np.array_equal(Qin.cpu().numpy(), g["w_Qout_" + name])
+
+
+@pytest.mark.parametrize("f32", [True, False])
+def test_compacted_exact_equals_one_kernel(device, f32):
+    """The two-phase (wave-compacted) mixed exact solve returns bit for bit
+    what the one-kernel path returns, with the same counters, for split
+    points that queue many, few or no problems."""
+    import torch
+    b = WK.riemann_batch(200000, seed=5, family="mixed", ndim=3)
+    dt = torch.float32 if f32 else torch.float64
+    WL, WR, n = (_t(b[k], dt, device) for k in ("WL", "WR", "n"))
+    refl = _t((np.arange(200000) % 7 == 0).astype(np.uint8), torch.uint8, device)
+    out = {}
+    for split in (0, 1, 4, 30):
+        s = RiemannSolver(0, 3, G, 1e-8, -5.0, "mixed")
+        s.set_split(split)
+        st = torch.zeros(200000, dtype=torch.uint8, device=device)
+        F = s.solve_for_flux_batch(WL, WR, n, None, refl, status=st)
+        torch.cuda.synchronize()
+        out[split] = (F.cpu().numpy(), st.cpu().numpy(), s.stats())
+    F0, st0, c0 = out[0]
+    for split in (1, 4, 30):
+        F1, st1, c1 = out[split]
+        assert np.array_equal(F1, F0), split
+        assert np.array_equal(st1, st0), split
+        assert c1 == c0, split
