@@ -323,14 +323,14 @@ def main():
     outs = {}
     only = set(args.only.split(",")) if args.only else None
     for kname, kind, prec in (("exact", 0, "fp64"), ("exact", 0, "mixed"),
-                              ("exact", 0, "mixed_onekernel"), ("hllc", 1, "fp64"),
+                              ("exact", 0, "mixed_compacted"), ("hllc", 1, "fp64"),
                               ("hllc", 1, "mixed")):
             if only is not None and "%s_%s" % (kname, prec) not in only:
                 continue
             s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"],
                               "mixed" if prec.startswith("mixed") else prec)
-            if prec == "mixed_onekernel":
-                s.set_split(0)  # without the wave-level compaction, for comparison
+            if prec == "mixed_compacted":
+                s.set_split(8)  # with the wave-level compaction, for comparison
             s.enable_stats(False)
             src = d64 if prec == "fp64" else d32
             F = torch.empty_like(src["WL"])
@@ -371,15 +371,14 @@ def main():
     value = N / (ms * 1e-3)
     bytes_per = (2 * 5 + 3) * 4 + 5 * 4  # WL, WR, n in + F out, FP32
     achieved = bytes_per * N / (ms * 1e-3) / 1e9
-    kernels = ["sfx::mixed::exact_phase1_kernel<3, float>",
-               "sfx::mixed::exact_phase2_kernel<3, float>"]
+    kernels = ["sfx::mixed::solve_for_flux_kernel<3, 0, float>"]
     parts = [measured_traffic(k, None) for k in kernels]
     traffic = sum(parts) if all(p is not None for p in parts) else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": "profiles/hbm_traffic_pmc.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                               "scaled by a copy kernel of known size)" if traffic else None,
-            "kernel": " + ".join(kernels) + " (one step: the compacted exact solve)",
+            "kernel": kernels[0] + " (one step: the batched mixed exact solve)",
             "bytes_per_problem": bytes_per,
             "note": "exact solver is VALU-bound (Newton on FP32 transcendentals + one FP64 "
                     "step); HBM fraction shown as the contract asks"}

@@ -12,7 +12,8 @@
 // batch give every mesh its arena offsets and its range of work tiles, and
 // it zeroes the tile counters) followed by four persistent drains --
 // gradients, face fluxes, gather, update -- each a fixed grid whose blocks
-// claim tiles of `tile` consecutive cells or faces until none are left.
+// claim tiles of `tile` consecutive cells or faces, eight per atomic, until
+// none are left.
 // The tile range of a phase is cut into one contiguous chunk per XCD; a
 // block drains the chunk of the XCD it runs on (dispatch index % 8) first,
 // then the others, so neighbouring cells mostly meet in one L2.  Every block
@@ -69,9 +70,11 @@ __device__ __forceinline__ int find_slot(const Plan& p, int64_t t) {
 template <bool FACES, class Fn>
 __device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
     __shared__ int64_t s_tile;
+    // tiles claimed per atomic: the counters see tiles / kClaim requests
+    constexpr int kClaim = 8;
     const int64_t T = FACES ? p->ftiles : p->ctiles;
-    // with fewer tiles than blocks the first T blocks take them all
-    if((int64_t)blockIdx.x >= T) return;
+    // with fewer claims than blocks the first ones take them all
+    if((int64_t)blockIdx.x * kClaim >= T) return;
     const int tile = p->tile;
     const int x0 = blockIdx.x % kXcd;
     for(int c = 0; c < kXcd; c++) {
@@ -84,19 +87,22 @@ __device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
                 const unsigned seen = __atomic_load_n(&p->ctr[phase][x], __ATOMIC_RELAXED);
                 s_tile = lo + (int64_t)seen >= hi
                                  ? hi
-                                 : lo + (int64_t)atomicAdd(&p->ctr[phase][x], 1u);
+                                 : lo + (int64_t)atomicAdd(&p->ctr[phase][x], (unsigned)kClaim);
             }
             __syncthreads();
-            const int64_t t = s_tile;
+            const int64_t t0 = s_tile;
             __syncthreads();
-            if(t >= hi) break;
-            const int s = find_slot<FACES>(*p, t);
-            const Slot& sl = p->slot[s];
-            const int64_t base = FACES ? sl.face0 + (t - sl.ftile0) * tile
-                                       : sl.cell0 + (t - sl.ctile0) * tile;
-            const int64_t end = FACES ? sl.face0 + sl.nface : sl.cell0 + sl.ncell;
-            const int64_t item = base + threadIdx.x;
-            if(threadIdx.x < tile && item < end) fn(sl, item);
+            if(t0 >= hi) break;
+            const int64_t t1 = (t0 + kClaim < hi) ? t0 + kClaim : hi;
+            for(int64_t t = t0; t < t1; t++) {
+                const int s = find_slot<FACES>(*p, t);
+                const Slot& sl = p->slot[s];
+                const int64_t base = FACES ? sl.face0 + (t - sl.ftile0) * tile
+                                           : sl.cell0 + (t - sl.ctile0) * tile;
+                const int64_t end = FACES ? sl.face0 + sl.nface : sl.cell0 + sl.ncell;
+                const int64_t item = base + threadIdx.x;
+                if(threadIdx.x < tile && item < end) fn(sl, item);
+            }
         }
     }
 }

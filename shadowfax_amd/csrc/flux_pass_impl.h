@@ -37,6 +37,14 @@
 #include "riemann_mixed_impl.h"
 #include "sfx_device.h"
 
+// Entries loaded ahead per lane in the per-cell loops (gradients, gather).
+#ifndef SFX_GRAD_KB
+#define SFX_GRAD_KB 2
+#endif
+#ifndef SFX_GATHER_KB
+#define SFX_GATHER_KB 8
+#endif
+
 namespace sfx {
 namespace flux {
 
@@ -284,7 +292,7 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
     // faces in reverse (src/VorCell.cpp:471), KB at a time: the records and
     // neighbour states of a chunk are loaded before any is used, so a lane
     // has KB independent loads in flight instead of one dependent chain
-    constexpr int KB = 4;
+    constexpr int KB = SFX_GRAD_KB;
     for(int64_t jb = j1; jb > j0; jb -= KB) {
         R A[KB], c[KB][ND], rij[KB][ND], Wj[KB][NC];
         int32_t o[KB];
@@ -614,21 +622,41 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
 #pragma unroll
     for(int k = 0; k < NC; k++) acc[k] = 0.0;
     const int64_t j0 = goff[i], j1 = goff[i + 1];
-    for(int64_t j = j0; j < j1; j++) {
-        const int32_t e = gent[j];
-        int32_t f = ent_face(e);
-        bool neg = ent_right(e);
-        if(MIXED && twin && ghost[f]) {
-            const int32_t t = twin[f];
-            if(t >= 0 && t < f) {
-                f = t;       // the other copy carries this face's flux
-                neg = !neg;  // seen from its right cell
+    // SFX_GATHER_KB entries at a time: their indices, then their rows, are
+    // loaded before any is added, so a lane has that many independent loads
+    // in flight; the additions keep ascending face order
+    constexpr int KB = SFX_GATHER_KB;
+    for(int64_t jb = j0; jb < j1; jb += KB) {
+        int32_t f[KB];
+        bool neg[KB];
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            const int32_t e = gent[(jb + u < j1) ? jb + u : j0];
+            f[u] = ent_face(e);
+            neg[u] = ent_right(e);
+        }
+        if(MIXED && twin) {
+#pragma unroll
+            for(int u = 0; u < KB; u++) {
+                if(jb + u < j1 && ghost[f[u]]) {
+                    const int32_t t = twin[f[u]];
+                    if(t >= 0 && t < f[u]) {
+                        f[u] = t;          // the other copy carries this face's flux
+                        neg[u] = !neg[u];  // seen from its right cell
+                    }
+                }
             }
         }
+        double x[KB][NC];
 #pragma unroll
-        for(int k = 0; k < NC; k++) {
-            const double x = (double)dQL[((int64_t)f + qshift) * NC + k];
-            acc[k] += neg ? -x : x;
+        for(int u = 0; u < KB; u++)
+#pragma unroll
+            for(int k = 0; k < NC; k++) x[u][k] = (double)dQL[((int64_t)f[u] + qshift) * NC + k];
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            if(jb + u >= j1) break;
+#pragma unroll
+            for(int k = 0; k < NC; k++) acc[k] += neg[u] ? -x[u][k] : x[u][k];
         }
     }
 #pragma unroll

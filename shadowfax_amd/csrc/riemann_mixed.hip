@@ -57,12 +57,14 @@ __global__ __launch_bounds__(256) void solve_for_flux_kernel(
 //
 // Phase 1 runs every problem's setup and at most `stop` FP32 Newton
 // iterations; problems still iterating after that (strong shocks, near
-// vacuum) are appended to a queue -- one atomic per wave, slots from a
-// ballot -- with their iteration state, and the wave finishes the converged
-// ones together instead of idling while a few lanes iterate on.  Phase 2, a
-// persistent grid, takes the queue in dense chunks, resumes each problem
-// where it stopped and finishes it.  The iteration sequence per problem is
-// unchanged, so results are bit-identical to the one-kernel path.
+// vacuum) are written, with their iteration state, to their block's segment
+// of a queue (slots from wave ballots and one LDS counter per block -- no
+// global atomics), and the wave finishes the converged ones together instead
+// of idling while a few lanes iterate on.  Phase 2 gives each block the
+// stragglers of kSegs consecutive phase-1 segments, packed densely through a
+// prefix sum of their counts, resumes each problem where it stopped and
+// finishes it.  The iteration sequence per problem is unchanged, so results
+// are bit-identical to the one-kernel path.
 
 struct Straggler {
     int32_t i;
@@ -70,9 +72,7 @@ struct Straggler {
     int32_t it;
 };
 
-__global__ void reset_queue_kernel(unsigned* q) {
-    if(threadIdx.x < 2) q[threadIdx.x] = 0u;
-}
+constexpr int kSegs = 16;  // phase-1 segments per phase-2 block
 
 template <int ND, typename T>
 __device__ __forceinline__ void load_problem(int64_t N, int64_t i, const T* __restrict__ WL,
@@ -101,8 +101,11 @@ __global__ __launch_bounds__(256) void exact_phase1_kernel(
         CF c, Consts cd, int64_t N, const T* __restrict__ WL, const T* __restrict__ WR,
         const T* __restrict__ n, const T* __restrict__ v, const uint8_t* __restrict__ refl,
         T* __restrict__ F, uint8_t* __restrict__ status, unsigned long long* __restrict__ ctr,
-        Straggler* __restrict__ queue, unsigned* __restrict__ qcount, int stop) {
+        Straggler* __restrict__ queue, unsigned* __restrict__ counts, int stop) {
     const int NC = ND + 2;
+    __shared__ unsigned s_cnt;
+    if(threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < N;
     unsigned iters = 0, vac = 0, fail = 0, done = 0;
@@ -116,15 +119,12 @@ __global__ __launch_bounds__(256) void exact_phase1_kernel(
         if(s.phase == EX_ITER) exact_iterate(c, s, stop);
     }
     const bool strag = valid && s.phase == EX_ITER;
-    // wave-wide append of the stragglers
+    // slots in the block's segment: a ballot per wave, one LDS atomic per wave
     const unsigned long long mask = __ballot(strag);
     const int lane = __lane_id();
     unsigned base = 0;
-    if(mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        if(lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(mask));
-        base = __shfl(base, leader);
-    }
+    if(lane == 0 && mask) base = atomicAdd(&s_cnt, (unsigned)__popcll(mask));
+    base = __shfl(base, 0);
     if(strag) {
         const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
         Straggler g;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void exact_phase1_kernel(
         g.lo = s.lo;
         g.hi = s.hi;
         g.it = s.it;
-        queue[slot] = g;
+        queue[(int64_t)blockIdx.x * blockDim.x + slot] = g;
     } else if(valid) {
         double f[NC];
         const uint8_t st = exact_finish<ND>(c, cd, s, wl64, wr64, n64, v64, f, iters, vac);
@@ -143,6 +143,8 @@ __global__ __launch_bounds__(256) void exact_phase1_kernel(
         fail = st != 0;
         done = 1;
     }
+    __syncthreads();
+    if(threadIdx.x == 0) counts[blockIdx.x] = s_cnt;
     const unsigned cv[CNT_N] = {done, 0u, vac, iters, fail};
     block_add_counters(ctr, cv);
 }
@@ -152,40 +154,44 @@ __global__ __launch_bounds__(256) void exact_phase2_kernel(
         CF c, Consts cd, int64_t N, const T* __restrict__ WL, const T* __restrict__ WR,
         const T* __restrict__ n, const T* __restrict__ v, const uint8_t* __restrict__ refl,
         T* __restrict__ F, uint8_t* __restrict__ status, unsigned long long* __restrict__ ctr,
-        const Straggler* __restrict__ queue, const unsigned* __restrict__ qcount,
-        unsigned* __restrict__ qtake) {
+        const Straggler* __restrict__ queue, const unsigned* __restrict__ counts,
+        int64_t nsegs) {
     const int NC = ND + 2;
-    __shared__ unsigned s_base;
-    const unsigned total = *qcount;
-    unsigned iters = 0, vac = 0, fail = 0, done = 0;
-    for(;;) {
-        if(threadIdx.x == 0) s_base = atomicAdd(qtake, blockDim.x);
-        __syncthreads();
-        const unsigned base = s_base;
-        __syncthreads();
-        if(base >= total) break;
-        const unsigned k = base + threadIdx.x;
-        if(k < total) {
-            const Straggler g = queue[k];
-            const int64_t i = g.i;
-            double wl64[NC], wr64[NC], n64[ND], v64[ND], f[NC];
-            float wl[NC], wr[NC], nn[ND];
-            load_problem<ND, T>(N, i, WL, WR, n, v, wl64, wr64, n64, v64, wl, wr, nn);
-            ExactCase s;
-            exact_setup<ND>(c, cd, wl, wr, nn, refl ? refl[i] != 0 : false, wl64, wr64, n64, s);
-            s.p = g.p;
-            s.lo = g.lo;
-            s.hi = g.hi;
-            s.it = g.it;
-            s.phase = EX_ITER;
-            exact_iterate(c, s, c.itmax);
-            const uint8_t st = exact_finish<ND>(c, cd, s, wl64, wr64, n64, v64, f, iters, vac);
-#pragma unroll
-            for(int q = 0; q < NC; q++) F[q * N + i] = (T)f[q];
-            if(status) status[i] = st;
-            fail += st != 0;
-            done++;
+    __shared__ unsigned s_pre[kSegs + 1];
+    const int64_t seg0 = (int64_t)blockIdx.x * kSegs;
+    if(threadIdx.x == 0) {
+        unsigned acc = 0;
+        for(int q = 0; q < kSegs; q++) {
+            s_pre[q] = acc;
+            acc += (seg0 + q < nsegs) ? counts[seg0 + q] : 0u;
         }
+        s_pre[kSegs] = acc;
+    }
+    __syncthreads();
+    const unsigned total = s_pre[kSegs];
+    unsigned iters = 0, vac = 0, fail = 0, done = 0;
+    for(unsigned k = threadIdx.x; k < total; k += blockDim.x) {
+        int q = 0;
+        while(q + 1 < kSegs && s_pre[q + 1] <= k) q++;
+        const Straggler g = queue[(seg0 + q) * blockDim.x + (k - s_pre[q])];
+        const int64_t i = g.i;
+        double wl64[NC], wr64[NC], n64[ND], v64[ND], f[NC];
+        float wl[NC], wr[NC], nn[ND];
+        load_problem<ND, T>(N, i, WL, WR, n, v, wl64, wr64, n64, v64, wl, wr, nn);
+        ExactCase s;
+        exact_setup<ND>(c, cd, wl, wr, nn, refl ? refl[i] != 0 : false, wl64, wr64, n64, s);
+        s.p = g.p;
+        s.lo = g.lo;
+        s.hi = g.hi;
+        s.it = g.it;
+        s.phase = EX_ITER;
+        exact_iterate(c, s, c.itmax);
+        const uint8_t st = exact_finish<ND>(c, cd, s, wl64, wr64, n64, v64, f, iters, vac);
+#pragma unroll
+        for(int r = 0; r < NC; r++) F[r * N + i] = (T)f[r];
+        if(status) status[i] = st;
+        fail += st != 0;
+        done++;
     }
     const unsigned cv[CNT_N] = {done, 0u, vac, iters, fail};
     block_add_counters(ctr, cv);
@@ -194,7 +200,8 @@ __global__ __launch_bounds__(256) void exact_phase2_kernel(
 }  // namespace mixed
 
 size_t exact_split_scratch_bytes(int64_t N) {
-    return 256 + sizeof(mixed::Straggler) * (size_t)std::max<int64_t>(N, 1);
+    const int64_t nb = (std::max<int64_t>(N, 1) + 255) / 256;
+    return (size_t)(((4 * nb + 255) / 256) * 256) + sizeof(mixed::Straggler) * (size_t)(nb * 256);
 }
 
 template <typename T>
@@ -203,18 +210,21 @@ hipError_t launch_exact_mixed_split(int ndim, const Consts& c, int64_t N, const 
                                     T* F, uint8_t* status, unsigned long long* ctr,
                                     void* scratch, int stop, int persistent_blocks,
                                     hipStream_t stream) {
+    (void)persistent_blocks;
     if(N <= 0) return hipSuccess;
     const mixed::CF cf = mixed::make_cf(c);
-    unsigned* q = (unsigned*)scratch;  // [0] appended, [1] taken
-    mixed::Straggler* queue = (mixed::Straggler*)((char*)scratch + 256);
-    hipLaunchKernelGGL(mixed::reset_queue_kernel, dim3(1), dim3(64), 0, stream, q);
-    const dim3 grid((unsigned)((N + 255) / 256)), block(256), pgrid((unsigned)persistent_blocks);
+    const int64_t nb = (N + 255) / 256;
+    unsigned* counts = (unsigned*)scratch;
+    mixed::Straggler* queue =
+            (mixed::Straggler*)((char*)scratch + ((4 * nb + 255) / 256) * 256);
+    const dim3 grid((unsigned)nb), block(256),
+            grid2((unsigned)((nb + mixed::kSegs - 1) / mixed::kSegs));
 #define SFX_SPLIT(ND)                                                                          \
     do {                                                                                       \
         hipLaunchKernelGGL((mixed::exact_phase1_kernel<ND, T>), grid, block, 0, stream, cf, c, N, \
-                           WL, WR, n, v, refl, F, status, ctr, queue, q, stop);                 \
-        hipLaunchKernelGGL((mixed::exact_phase2_kernel<ND, T>), pgrid, block, 0, stream, cf, c,  \
-                           N, WL, WR, n, v, refl, F, status, ctr, queue, q, q + 1);             \
+                           WL, WR, n, v, refl, F, status, ctr, queue, counts, stop);            \
+        hipLaunchKernelGGL((mixed::exact_phase2_kernel<ND, T>), grid2, block, 0, stream, cf, c,  \
+                           N, WL, WR, n, v, refl, F, status, ctr, queue, counts, nb);           \
     } while(0)
     if(ndim == 3)
         SFX_SPLIT(3);

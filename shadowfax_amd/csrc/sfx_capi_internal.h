@@ -23,7 +23,7 @@ struct sfx_solver {
     // on demand
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
-    int split = 4;  // FP32 iterations before a problem is queued (0: one kernel)
+    int split = 0;  // FP32 iterations before a problem is queued (0: one kernel)
     int ncu = 0;    // compute units of the device
 };
 
