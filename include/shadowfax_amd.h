@@ -262,7 +262,8 @@ typedef struct {
     int ndim;
     int64_t C, F;
     /* cells */
-    const double* W;           /* [C][ndim+2] primitives (FP64 in both passes) */
+    const double* W;           /* [C][ndim+2] primitives (FP64 in both passes);
+                                  16-byte aligned, as egeo */
     const void* vel;           /* T [C][ndim] generator velocities */
     const double* mass;        /* [C] Q.m */
     const double* total_area;  /* [C] sum of the cell's face areas */
@@ -284,21 +285,32 @@ typedef struct {
     /* topology */
     const int64_t* cf_off;     /* [C+1] */
     const int32_t* cf_entry;   /* sfx_mesh_cell_entries */
-    const void* egeo;          /* T [cf_off[C]][ndim == 2 ? 8 : 12] per-entry
-                                  face records (sfx_mesh_entry_geometry) */
+    const void* egeo;          /* T [cf_off[C]][sfx_mesh_entry_record(...)]
+                                  per-entry face records
+                                  (sfx_mesh_entry_geometry) */
     const int64_t* gather_off; /* [C+1] */
     const int32_t* gather_entry; /* sfx_mesh_gather_index */
 } sfx_mesh;
 
 /* Per-entry face records for the gradient pass, built once per
- * tessellation from the face arrays and cf_entry: for entry j of cell i
- * (face f seen from side s) [area, mc (ndim), rRL seen from i (ndim),
- * dL or dR (ndim), the neighbour's index bits (-1: mirror), (3D) pad].
- * The same values the gradient loop of VorCell::estimate_gradient reads per
- * face (src/VorCell.cpp:471-500), stored in the cell's walk order so that
- * the pass streams them.  E = cf_off[C]; egeo: T [E][ndim == 2 ? 8 : 12]. */
+ * tessellation from the face arrays and cf_entry, in each cell's walk order
+ * so that the pass streams them; for entry j of cell i (face f seen from
+ * side s):
+ *   FP64:  [area, mc (ndim), rRL seen from i (ndim), dL or dR (ndim),
+ *           the neighbour's index bits (-1: mirror), (3D) pad]
+ *          -- the values the gradient loop of VorCell::estimate_gradient
+ *          reads per face (src/VorCell.cpp:471-500);
+ *   mixed: [area (mc - r/2)/|r| (ndim), the neighbour's index bits
+ *          (~f: mirror), (2D) pad | dL or dR (ndim), area, (2D) pad],
+ *          r = rRL seen from i -- the weights of the difference form of
+ *          the same sum, in two 16-byte halves.
+ * E = cf_off[C]; egeo: T [E][sfx_mesh_entry_record(ndim, precision)]. */
 int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* egeo,
                             void* stream);
+
+/* Elements of type T per entry record: 8 / 12 (FP64, 2D / 3D), 8 (mixed);
+ * -1 for other arguments. */
+int sfx_mesh_entry_record(int ndim, int precision);
 
 /* VorCell::estimate_gradient for every active cell (src/VorCell.cpp:464-530,
  * loop src/Simulation.cpp:414-420).  grad: T [C][ndim][ndim+2]

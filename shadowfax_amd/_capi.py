@@ -98,6 +98,7 @@ def _mesh_sigs():
     md = ct.POINTER(MeshDesc)
     return {
         "sfx_mesh_entry_geometry": [vp, md, i64, vp, vp],
+        "sfx_mesh_entry_record": [ct.c_int, ct.c_int],
         "sfx_gradients": [vp, md, vp, vp],
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],

@@ -74,16 +74,66 @@ struct MeshView {
     const T* rcent;  // [F]
     const int64_t* cf_off;
     const int32_t* cf_entry;
-    const T* egeo;   // [cf_off[C]][entry_rec(ND)]
+    const T* egeo;   // [cf_off[C]][entry_rec<ND, T>()]
     const int64_t* gather_off;
     const int32_t* gather_entry;
 };
 
-// Per-entry record of a cell's face, in the cell's walk order:
+// Per-entry record of a cell's face, in the cell's walk order.  FP64
+// storage (the parity pass):
 //   [A, c (ND), r_ij (ND), d (ND), neighbour index bits, (3D: pad)]
 // with c = mc, r_ij = rRL seen from the cell (-rRL for the left cell),
-// d = dL or dR, neighbour = the other cell or -1 (mirror).
-__host__ __device__ constexpr int entry_rec(int ND) { return ND == 2 ? 8 : 12; }
+// d = dL or dR, neighbour = the other cell or -1 (mirror).  FP32 storage
+// (the mixed pass, which sums the difference form of the gradient), two
+// 16-byte halves:
+//   [g (ND), neighbour index bits, (2D: pad) | d (ND), A, (2D: pad)]
+// with g = A (c - r_ij/2)/|r_ij| formed in FP64 and rounded once, and the
+// neighbour = the other cell or ~f for a mirror (so the face normal can be
+// read back); the limiter's second loop reads only the second half.
+__host__ __device__ constexpr int entry_rec(int ND, bool f32) {
+    return f32 ? 8 : (ND == 2 ? 8 : 12);
+}
+// field offsets of the FP32 record
+template <int ND>
+struct Rec32 {
+    static constexpr int G = 0, NB = ND, D = 4, A = 4 + ND;
+};
+
+// Row o of an item-major FP64 array of NC = 4 or 5 values per item
+// ([n][NC], 16-byte aligned base) with 16-byte loads: for NC = 5 the aligned
+// window of six values that holds the row is read and the row picked out of
+// it by the row's parity; the last row, whose window would end past the
+// array, is read value by value.
+template <int NC>
+__device__ __forceinline__ void load_row(const double* __restrict__ a, int64_t o, int64_t n,
+                                         double* out) {
+    static_assert(NC == 4 || NC == 5, "rows of 4 or 5 values");
+    if constexpr(NC == 4) {
+        const double2* p = reinterpret_cast<const double2*>(a + o * 4);
+        const double2 x = p[0], y = p[1];
+        out[0] = x.x;
+        out[1] = x.y;
+        out[2] = y.x;
+        out[3] = y.y;
+    } else {
+        const int64_t s = o * 5, base = s & ~int64_t(1);
+        if(base + 6 <= n * 5) {
+            const double2* p = reinterpret_cast<const double2*>(a + base);
+            const double2 x = p[0], y = p[1], z = p[2];
+            const double w[6] = {x.x, x.y, y.x, y.y, z.x, z.y};
+            const bool odd = (s & 1) != 0;
+#pragma unroll
+            for(int k = 0; k < 5; k++) out[k] = odd ? w[k + 1] : w[k];
+        } else {
+#pragma unroll
+            for(int k = 0; k < 5; k++) out[k] = a[s + k];
+        }
+    }
+}
+template <int ND, typename T>
+__host__ __device__ constexpr int entry_rec() {
+    return entry_rec(ND, sizeof(T) == 4);
+}
 
 __device__ __forceinline__ double index_bits(int32_t i, double) {
     return __longlong_as_double((long long)i);
@@ -237,6 +287,78 @@ __device__ __forceinline__ void predict_side(R gamma, const R* Wraw, const R* Wo
     for(int k = 0; k < NC; k++) out[k] = Wt[k];
 }
 
+// predict_side for the mixed pass: the state is the FP64 cell state plus an
+// FP32 offset.  Slopes, the time derivative and the pairwise limiter act on
+// offsets from phiL = Wraw (phiR - phiL formed in FP64, then rounded), so
+// their FP32 rounding is relative to the offsets, not to the state; the two
+// sign tests of the limiter's phi+/phi- (discontinuous in phiL) stay in
+// FP64.  With D = phiR - phiL, d1 = psi1 |D|:
+//   phibar - phiL = |d|/r D
+//   phiL < phiR: phi- - phiL = -d1, or -phiL d1/(phiL + d1) for 0 <= phiL <= d1
+//   phiL > phiR: phi+ - phiL = +d1, or |phiL| d1/(|phiL| + d1) for -d1 <= phiL <= 0
+// (the closed forms of phimin/(1 + d1/|phimin|) - phiL).
+template <int ND>
+__device__ __forceinline__ void predict_side_offset(float gamma, const double* Wraw,
+                                                    const double* Wother,
+                                                    const float (*g)[ND + 2], const float* d,
+                                                    float r, bool limited, double dtp,
+                                                    double* out) {
+    constexpr int NC = ND + 2, P = ND + 1;
+    float Wf[NC], dWdt[NC], dW[NC], off[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) Wf[k] = (float)Wraw[k];
+    time_derivative<ND, float>(gamma, Wf, g, dWdt);
+    const float hdt = 0.5f * (float)dtp;
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        dW[k] = hdt * dWdt[k];
+        off[k] = (ND == 3) ? g[0][k] * d[0] + g[1][k] * d[1] + g[ND - 1][k] * d[ND - 1]
+                           : g[0][k] * d[0] + g[1][k] * d[1];
+    }
+    if(limited) {
+        // |d|/r and phibar - phiL in FP64: the limiter's bound is an offset
+        // of the size of the jump itself, rounded once
+        const double d0 = d[0], d1v = d[1], d2v = (ND == 3) ? (double)d[ND - 1] : 0.0;
+        const double t = sqrt(d0 * d0 + d1v * d1v + d2v * d2v) / (double)r;
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            const double phiL = Wraw[k], phiR = Wother[k];
+            if(phiL == phiR) {
+                off[k] = 0.f;
+                continue;
+            }
+            const double Dd = phiR - phiL;
+            const double d1d = 0.5 * fabs(Dd);
+            const float d1 = (float)d1d, d2 = (float)(0.25 * fabs(Dd));
+            const float pl = (float)phiL;
+            if(phiL < phiR) {
+                const float om = ((phiL - d1d) * phiL > 0.0) ? -d1
+                                 : (pl == 0.f)               ? 0.f
+                                                             : -(d1 * pl) / (pl + d1);
+                off[k] = fmaxf(om, fminf((float)(t * Dd) + d2, off[k]));
+            } else {
+                const float op = ((phiL + d1d) * phiL > 0.0) ? d1
+                                 : (pl == 0.f)               ? 0.f
+                                                             : (d1 * -pl) / (d1 - pl);
+                off[k] = fminf(op, fmaxf((float)(t * Dd) - d2, off[k]));
+            }
+        }
+    }
+    double Wt[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) Wt[k] = Wraw[k] + (double)off[k];
+    if(Wt[0] < 0.0 || Wt[P] < 0.0) {
+#pragma unroll
+        for(int k = 0; k < NC; k++) off[k] = 0.f;
+    }
+    double Wp[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) Wp[k] = Wraw[k] + (double)(off[k] + dW[k]);
+    const bool keep = !(Wp[0] < 0.0 || Wp[P] < 0.0);
+#pragma unroll
+    for(int k = 0; k < NC; k++) out[k] = keep ? Wp[k] : Wraw[k] + (double)off[k];
+}
+
 // ------------------------------------------------------------- K2 grads --
 
 // One lane per cell entry: the record of entry j (face f = e >> 1 seen from
@@ -244,7 +366,7 @@ __device__ __forceinline__ void predict_side(R gamma, const R* Wraw, const R* Wo
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int64_t E,
                                                              T* __restrict__ egeo) {
-    const int ER = entry_rec(ND);
+    constexpr int ER = entry_rec<ND, T>();
     const int64_t j = xcd_block() * blockDim.x + threadIdx.x;
     if(j >= E) return;
     const int64_t F = m.F;
@@ -252,16 +374,39 @@ __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int6
     const int32_t f = ent_face(e);
     const bool isR = ent_right(e);
     T rec[ER];
-    rec[0] = m.area[f];
+    if(sizeof(T) == 4) {
+        // g = A (c - r_ij/2)/|r_ij| in FP64
+        double r[ND], r2 = 0.0;
 #pragma unroll
-    for(int l = 0; l < ND; l++) {
-        const T rr = m.rRL[l * F + f];
-        rec[1 + l] = m.mc[l * F + f];
-        rec[1 + ND + l] = isR ? rr : -rr;
-        rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+        for(int l = 0; l < ND; l++) {
+            const double rr = (double)m.rRL[l * F + f];
+            r[l] = isR ? rr : -rr;
+            r2 += r[l] * r[l];
+        }
+        const double s = (double)m.area[f] / sqrt(r2);
+        using RL = Rec32<ND>;
+#pragma unroll
+        for(int k = 0; k < ER; k++) rec[k] = T(0);
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            rec[RL::G + l] = (T)(s * ((double)m.mc[l * F + f] - 0.5 * r[l]));
+            rec[RL::D + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+        }
+        const int32_t nb = isR ? m.left[f] : m.right[f];
+        rec[RL::NB] = index_bits(nb >= 0 ? nb : ~f, T(0));
+        rec[RL::A] = m.area[f];
+    } else {
+        rec[0] = m.area[f];
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            const T rr = m.rRL[l * F + f];
+            rec[1 + l] = m.mc[l * F + f];
+            rec[1 + ND + l] = isR ? rr : -rr;
+            rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+        }
+        rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], T(0));
+        if(ER > 2 + 3 * ND) rec[ER - 1] = T(0);
     }
-    rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], T(0));
-    if(ER > 2 + 3 * ND) rec[ER - 1] = T(0);
 #pragma unroll
     for(int k = 0; k < ER; k++) egeo[j * ER + k] = rec[k];
 }
@@ -274,12 +419,131 @@ __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int6
 // itself.  DIFF = true (the mixed pass) sums the algebraically equal
 //   sum_f A (Wj - Wi) (c - r/2)/|r|
 // that needs no cancellation.
+// The gradients of the mixed pass from FP32 records (entry_rec<ND, float>):
+//   delta[l][k] = sum_f (Wj_k - Wi_k) g_l / V
+// with the differences and sums in FP64, and the cell-wise limiter
+// (src/VorCell.cpp:505-521) without a division per face: since Wmax >= W >=
+// Wmin, a face's candidate max((Wmax - W)/dp, (Wmin - W)/dp) is a/b with
+// a = Wmax - W, b = dp for dp > 0 and a = W - Wmin, b = -dp for dp < 0
+// (dp = 0 gives 1, which never lowers alpha <= 1); the smallest a/b is kept
+// as a pair, compared by cross products, and divided out once per component.
+template <int ND>
+__device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t i,
+                                              float* __restrict__ grad, int64_t gshift) {
+    constexpr int NC = ND + 2, ER = entry_rec<ND, float>();
+    using RL = Rec32<ND>;
+    if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
+    double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
+    load_row<NC>(m.W, i, m.C, W);
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        Wmax[k] = Wmin[k] = W[k];
+#pragma unroll
+        for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
+    }
+    const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
+    constexpr int KB = SFX_GRAD_KB;
+    for(int64_t jb = j1; jb > j0; jb -= KB) {
+        float4 lo[KB], hi[KB];
+        double Wj[KB][NC];
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            const int64_t j = jb - 1 - u;
+            const float4* r = reinterpret_cast<const float4*>(m.egeo + ((j >= j0) ? j : j0) * ER);
+            lo[u] = r[0];
+            hi[u] = r[1];
+            if(j < j0) hi[u].w = hi[u].z = 0.f;  // padding: A = 0 (A sits at .z or .w)
+        }
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            const float nbf = (ND == 3) ? lo[u].w : lo[u].z;
+            const int32_t o = bits_index(nbf);
+            if(o >= 0) {
+                load_row<NC>(m.W, o, m.C, Wj[u]);
+            } else {
+                // mirror neighbour (src/VorCell.cpp:482-489) of face ~o;
+                // only a left cell has one, so rRL is seen from it
+                const int32_t f = ~o;
+                double rr[ND];
+#pragma unroll
+                for(int l = 0; l < ND; l++) rr[l] = (double)m.rRL[l * m.F + f];
+#pragma unroll
+                for(int k = 0; k < NC; k++) Wj[u][k] = W[k];
+                mirror_velocity<ND, double>(Wj[u] + 1, rr);
+            }
+        }
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            const float A = (ND == 3) ? hi[u].w : hi[u].z;
+            if(!(A != 0.f)) continue;  // also the padding past j0
+            const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
+                                  (ND == 3) ? (double)lo[u].z : 0.0};
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                const double dk = Wj[u][k] - W[k];
+#pragma unroll
+                for(int l = 0; l < ND; l++) delta[l][k] += dk * gl[l];
+                Wmax[k] = smaxr<double>(Wmax[k], Wj[u][k]);
+                Wmin[k] = sminr<double>(Wmin[k], Wj[u][k]);
+            }
+        }
+    }
+    const double Vinv = 1.0 / m.volume[i];
+#pragma unroll
+    for(int l = 0; l < ND; l++)
+#pragma unroll
+        for(int k = 0; k < NC; k++) delta[l][k] *= Vinv;
+    double na[NC], nb[NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) na[k] = nb[k] = 1.0;
+    constexpr int KL = 4;
+    for(int64_t jb = j1; jb > j0; jb -= KL) {
+        float4 hi[KL];
+#pragma unroll
+        for(int u = 0; u < KL; u++) {
+            const int64_t j = jb - 1 - u;
+            hi[u] = reinterpret_cast<const float4*>(m.egeo + ((j >= j0) ? j : j0) * ER)[1];
+            if(j < j0) hi[u].w = hi[u].z = 0.f;
+        }
+#pragma unroll
+        for(int u = 0; u < KL; u++) {
+            const float A = (ND == 3) ? hi[u].w : hi[u].z;
+            if(!(A != 0.f)) continue;
+            const double dd[3] = {(double)hi[u].x, (double)hi[u].y,
+                                  (ND == 3) ? (double)hi[u].z : 0.0};
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                double dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
+                if(ND == 3) dp += delta[ND - 1][k] * dd[ND - 1];
+                const double a = (dp > 0.0) ? Wmax[k] - W[k] : W[k] - Wmin[k];
+                const double b = fabs(dp);
+                if(dp != 0.0 && a * nb[k] < na[k] * b) {
+                    na[k] = a;
+                    nb[k] = b;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        const double alpha = na[k] / nb[k];
+#pragma unroll
+        for(int l = 0; l < ND; l++)
+            grad[((i + gshift) * ND + l) * NC + k] = (float)(delta[l][k] * alpha);
+    }
+}
+
 // Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
 // for a whole mesh, the arena offset of the cell's mesh in an ensemble).
 template <int ND, typename T, typename R, bool DIFF>
 __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __restrict__ grad,
                                           int64_t gshift) {
-    const int NC = ND + 2, ER = entry_rec(ND);
+    if constexpr(sizeof(T) == 4) {
+        static_assert(DIFF, "FP32 records carry the difference-form weights");
+        grad_cell_f32<ND>(m, i, grad, gshift);
+        return;
+    }
+    const int NC = ND + 2, ER = entry_rec(ND, false);
     if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
     R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
 #pragma unroll
@@ -475,19 +739,37 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
             const R rc = has_r ? (R)m.rcent[f] : R(0);
             R WRrec[NC], WLrec[NC];
             const R gam = (R)cd.gamma;
-            if(has_r) {
-                R g[ND][NC], d[ND];
-                load_grad<ND, T, R>(grad, C, Rc + gshift, g);
+            if constexpr(MIXED) {
+                const float gf = (float)cd.gamma, rcf = (float)m.rcent[f];
+                if(has_r) {
+                    float g[ND][NC], d[ND];
+                    load_grad<ND, T, float>(grad, C, Rc + gshift, g);
 #pragma unroll
-                for(int l = 0; l < ND; l++) d[l] = (R)m.dR[l * F + f];
-                predict_side<ND, R>(gam, WR, WL, g, d, rc, true, (R)dt, WRrec);
-            }
-            {
-                R g[ND][NC], d[ND];
-                load_grad<ND, T, R>(grad, C, L + gshift, g);
+                    for(int l = 0; l < ND; l++) d[l] = (float)m.dR[l * F + f];
+                    predict_side_offset<ND>(gf, WR, WL, g, d, rcf, true, dt, WRrec);
+                }
+                {
+                    float g[ND][NC], d[ND];
+                    load_grad<ND, T, float>(grad, C, L + gshift, g);
 #pragma unroll
-                for(int l = 0; l < ND; l++) d[l] = (R)m.dL[l * F + f];
-                predict_side<ND, R>(gam, WL, WR, g, d, rc, has_r, (R)dt, WLrec);
+                    for(int l = 0; l < ND; l++) d[l] = (float)m.dL[l * F + f];
+                    predict_side_offset<ND>(gf, WL, WR, g, d, rcf, has_r, dt, WLrec);
+                }
+            } else {
+                if(has_r) {
+                    R g[ND][NC], d[ND];
+                    load_grad<ND, T, R>(grad, C, Rc + gshift, g);
+#pragma unroll
+                    for(int l = 0; l < ND; l++) d[l] = (R)m.dR[l * F + f];
+                    predict_side<ND, R>(gam, WR, WL, g, d, rc, true, (R)dt, WRrec);
+                }
+                {
+                    R g[ND][NC], d[ND];
+                    load_grad<ND, T, R>(grad, C, L + gshift, g);
+#pragma unroll
+                    for(int l = 0; l < ND; l++) d[l] = (R)m.dL[l * F + f];
+                    predict_side<ND, R>(gam, WL, WR, g, d, rc, has_r, (R)dt, WLrec);
+                }
             }
             if(WL[0] != R(0)) {
 #pragma unroll

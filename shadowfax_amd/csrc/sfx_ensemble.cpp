@@ -294,7 +294,7 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     e->arena_faces = arena_faces;
     const int64_t C = max_cells, F = max_faces, E = max_entries, nd = e->nd, NC = e->NC;
     const size_t t = e->tsz;
-    const int ER = flux::entry_rec(e->nd);
+    const int ER = flux::entry_rec(e->nd, t == 4);
     hipError_t err = hipSuccess;
     auto A = [&](auto** p, size_t bytes) {
         if(err == hipSuccess) err = dalloc(e, p, bytes);

@@ -70,6 +70,10 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
     if(m->C < 0 || m->F < 0) return set_error(SFX_ERR_ARG, "%s: negative size", who);
     if(m->C > INT32_MAX || m->F > (INT32_MAX >> 1))
         return set_error(SFX_ERR_ARG, "%s: mesh too large for 32-bit indices", who);
+    if(m->W && ((uintptr_t)m->W & 15))
+        return set_error(SFX_ERR_ARG, "%s: W must be 16-byte aligned", who);
+    if(m->egeo && ((uintptr_t)m->egeo & 15))
+        return set_error(SFX_ERR_ARG, "%s: egeo must be 16-byte aligned", who);
     if(gradients && (!m->W || !m->volume || !m->cf_off || !m->egeo))
         return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass (W, volume, "
                          "cf_off, egeo)", who);
@@ -214,6 +218,12 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left, const int32
         if(right[f] >= 0 && !(ghost && ghost[f])) gather_entry[fill[right[f]]++] = (int32_t)(2 * f + 1);
     }
     return SFX_OK;
+}
+
+int sfx_mesh_entry_record(int ndim, int precision) {
+    if((ndim != 2 && ndim != 3) || (precision != SFX_PREC_FP64 && precision != SFX_PREC_MIXED))
+        return -1;
+    return sfx::flux::entry_rec(ndim, precision == SFX_PREC_MIXED);
 }
 
 int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* egeo,

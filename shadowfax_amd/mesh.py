@@ -315,7 +315,9 @@ class DeviceMesh:
         nd, NC = self.ndim, self.ndim + 2
         self.E = int(b["cf_off"][-1])
         self.G = int(b["gather_off"][-1])
-        self.egeo = torch.zeros((self.E, 8 if nd == 2 else 12), dtype=tdt, device=self.dev)
+        ER = Cp.lib().sfx_mesh_entry_record(
+            nd, Cp.SFX_PREC_FP64 if storage == "fp64" else Cp.SFX_PREC_MIXED)
+        self.egeo = torch.zeros((self.E, ER), dtype=tdt, device=self.dev)
         self.egeo_built = False
         self.grad = torch.zeros((self.C, nd, NC), dtype=tdt, device=self.dev)
         self.dQL = torch.zeros((self.F, NC), dtype=tdt, device=self.dev)

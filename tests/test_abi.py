@@ -83,3 +83,14 @@ def test_struct_layouts_match_ctypes(tmp_path):
         assert got[(cname, "size")] == ct.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_entry_record_sizes():
+    """Record lengths of the gradient pass's per-entry face records: the
+    FP64 records carry c and r_ij, the mixed ones the combined weights."""
+    L = C.lib()
+    f64, mixed = C.SFX_PREC_FP64, C.SFX_PREC_MIXED
+    assert [L.sfx_mesh_entry_record(nd, f64) for nd in (2, 3)] == [8, 12]
+    assert [L.sfx_mesh_entry_record(nd, mixed) for nd in (2, 3)] == [8, 8]
+    assert L.sfx_mesh_entry_record(1, f64) == -1
+    assert L.sfx_mesh_entry_record(3, 7) == -1
