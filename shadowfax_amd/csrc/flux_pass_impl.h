@@ -41,6 +41,9 @@
 #ifndef SFX_GRAD_KB
 #define SFX_GRAD_KB 2
 #endif
+#ifndef SFX_GRAD_KB_2D  // FP32-record gradients of 2D meshes (about 6 entries a cell)
+#define SFX_GRAD_KB_2D 8
+#endif
 #ifndef SFX_GATHER_KB
 #define SFX_GATHER_KB 8
 #endif
@@ -120,10 +123,12 @@ __device__ __forceinline__ void load_row(const double* __restrict__ a, int64_t o
         if(base + 6 <= n * 5) {
             const double2* p = reinterpret_cast<const double2*>(a + base);
             const double2 x = p[0], y = p[1], z = p[2];
-            const double w[6] = {x.x, x.y, y.x, y.y, z.x, z.y};
             const bool odd = (s & 1) != 0;
-#pragma unroll
-            for(int k = 0; k < 5; k++) out[k] = odd ? w[k + 1] : w[k];
+            out[0] = odd ? x.y : x.x;
+            out[1] = odd ? y.x : x.y;
+            out[2] = odd ? y.y : y.x;
+            out[3] = odd ? z.x : y.y;
+            out[4] = odd ? z.y : z.x;
         } else {
 #pragma unroll
             for(int k = 0; k < 5; k++) out[k] = a[s + k];
@@ -442,7 +447,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
     }
     const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
-    constexpr int KB = SFX_GRAD_KB;
+    constexpr int KB = (ND == 2) ? SFX_GRAD_KB_2D : SFX_GRAD_KB;
     for(int64_t jb = j1; jb > j0; jb -= KB) {
         float4 lo[KB], hi[KB];
         double Wj[KB][NC];
@@ -496,7 +501,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     double na[NC], nb[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) na[k] = nb[k] = 1.0;
-    constexpr int KL = 4;
+    constexpr int KL = (ND == 2) ? SFX_GRAD_KB_2D : 4;
     for(int64_t jb = j1; jb > j0; jb -= KL) {
         float4 hi[KL];
 #pragma unroll
@@ -827,8 +832,7 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                     st |= mixed::exact_mixed<ND>(cf, cd, wlf, wrf, nf, !has_r, wl64, wr64, n64,
                                                  v64, flux, iters, vac);
                 } else {
-                    st |= fp64::hllc_solve_for_flux<ND>(cd, wl64, wr64, n64, v64, !has_r, flux,
-                                                        vac);
+                    st |= mixed::hllc_mixed<ND>(cd, wl64, wr64, n64, v64, !has_r, flux, vac);
                 }
             }
             // dQL = (dt A) F, dQR = -dQL (src/VorFace.cpp:834-835)

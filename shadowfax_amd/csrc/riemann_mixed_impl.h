@@ -390,6 +390,118 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
     return bad ? ST_SOLVER : 0;
 }
 
+// 1/x in FP64 from the hardware estimate v_rcp_f64 and two Newton-Raphson
+// steps r <- r + r (1 - x r) (each step doubles the correct bits): a few
+// FP64 FMAs instead of the full division sequence, within an ulp or two of
+// 1/x for normal x.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// sqrt(x) in FP64 for x >= 0 from the hardware estimate y = v_rsq_f64(x)
+// and one Newton-Raphson step on y = 1/sqrt(x), y <- y (3/2 - x/2 y^2)
+// (relative error about 3/2 of the square of the estimate's), then x y.
+__device__ __forceinline__ double sqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return (x == 0.0) ? 0.0 : x * y;
+}
+
+// HLLCRiemannSolver::solve_for_flux (HLLCRiemannSolver.cpp:258-403) for the
+// mixed path.  On this GPU an FP64 add, multiply or FMA issues at the FP32
+// rate, while the IEEE FP64 division and square root are long instruction
+// sequences, and they are what HLLC spends its time on.  Here every
+// quotient and root goes through rcp_nr / sqrt_nr (hardware estimate plus
+// Newton-Raphson), close to FP64 accuracy at a fraction of the cost.  FP32
+// wave speeds were tried and rejected: with rho and p spread over six
+// decades the sound speeds alone cost 9e-7 and the q factors 1.1e-6
+// normwise.  The star-state correction SK (U*K - UK) is written through
+// r = (S* - uK)/(SK - S*):
+//   U*K - UK = rhoK [r, r vK + (1 + r)(S* - uK) n,
+//                    r eK + (1 + r)(S* - uK)(S* + pK/(rhoK (SK - uK)))]
+// (fac = rhoK (SK - uK)/(SK - S*) = rhoK (1 + r)), so it vanishes with
+// S* - uK rather than as a difference of two rounded products (in cold gas,
+// a = 4e-3, p = 1e-5, a rounded fac would leave SK rhoK eps, ~4e-5 of the
+// pressure flux with FP32 eps).  Vacuum faces take the FP64 closed forms; a
+// reflective face has S* = 0 exactly (the FP64 reference's numerator
+// vanishes identically there).
+template <int ND>
+__device__ uint8_t hllc_mixed(const Consts& c, const double* WL, double* WR, const double* n,
+                              const double* v, bool reflective, double* flux, unsigned& vac) {
+    constexpr int NC = ND + 2, P = ND + 1;
+    const double g = c.gamma;
+#pragma unroll
+    for(int k = 0; k < NC; k++) flux[k] = 0.;
+    if(!(WL[0] != 0.) && (reflective || !(WR[0] != 0.))) return 0;
+    double uL = WL[1] * n[0] + WL[2] * n[1];
+    double uR = WR[1] * n[0] + WR[2] * n[1];
+    if(ND == 3) {
+        uL += WL[ND] * n[ND - 1];
+        uR += WR[ND] * n[ND - 1];
+    }
+    if(reflective) {
+#pragma unroll
+        for(int k = 0; k < NC; k++) WR[k] = WL[k];
+        uR = -uL;
+    }
+    if(!(WL[0] != 0.) || !(WR[0] != 0.)) {
+        vac = 1;
+        const double aL = sqrt(g * WL[P] / WL[0]), aR = sqrt(g * WR[P] / WR[0]);
+        fp64::hllc_vacuum_flux<ND>(c, WL, WR, n, v, uL, uR, aL, aR, flux);
+        return 0;
+    }
+    const double aL = sqrt_nr(g * WL[P] * rcp_nr(WL[0]));
+    const double aR = sqrt_nr(g * WR[P] * rcp_nr(WR[0]));
+    if(c.tdgm1 * aL + c.tdgm1 * aR <= uR - uL) {
+        vac = 1;
+        const double aL64 = sqrt(g * WL[P] / WL[0]), aR64 = sqrt(g * WR[P] / WR[0]);
+        fp64::hllc_vacuum_flux<ND>(c, WL, WR, n, v, uL, uR, aL64, aR64, flux);
+        return 0;
+    }
+    const double rhobar = 0.5 * (WL[0] + WR[0]);
+    const double abar = 0.5 * (aL + aR);
+    const double pstar = smax(0., 0.5 * (WL[P] + WR[P]) - 0.5 * (uR - uL) * rhobar * abar);
+    const double qc = c.gp1d2g;  // (g + 1)/(2 g)
+    double qL = 1., qR = 1.;
+    if(pstar > WL[P]) qL = sqrt_nr(1. + qc * ((pstar - WL[P]) * rcp_nr(WL[P])));
+    if(pstar > WR[P]) qR = sqrt_nr(1. + qc * ((pstar - WR[P]) * rcp_nr(WR[P])));
+    const double SL = uL - aL * qL;
+    const double SR = uR + aR * qR;
+    // the denominator is -(rhoL aL qL + rhoR aR qR) < 0: no cancellation
+    const double num = WR[P] - WL[P] + WL[0] * uL * (SL - uL) - WR[0] * uR * (SR - uR);
+    const double den = WL[0] * (SL - uL) - WR[0] * (SR - uR);
+    const double Sstar = reflective ? 0.0 : num * rcp_nr(den);
+    const bool left = Sstar >= 0.;
+    const double* WK = left ? WL : WR;
+    const double uK = left ? uL : uR;
+    const double SK = left ? SL : SR;
+    const double rho = WK[0], pK = WK[P];
+    const double rinv = rcp_nr(rho);
+    flux[0] = rho * uK;
+#pragma unroll
+    for(int d = 0; d < ND; d++) flux[1 + d] = rho * WK[1 + d] * uK + pK * n[d];
+    double v2 = WK[1] * WK[1] + WK[2] * WK[2];
+    if(ND == 3) v2 += WK[ND] * WK[ND];
+    const double eK = 0.5 * c.tdgm1 * pK * rinv + 0.5 * v2;
+    flux[P] = rho * eK * uK + pK * uK;
+    if(left ? (SK < 0.) : (SK > 0.)) {
+        // |SK - S*| >= |SK|: S* lies between SL and SR
+        const double ds = Sstar - uK;
+        const double r = ds * rcp_nr(SK - Sstar);
+        const double sr = SK * rho;
+        const double t = sr * (1. + r) * ds;
+        flux[0] += sr * r;
+#pragma unroll
+        for(int d = 0; d < ND; d++) flux[1 + d] += sr * r * WK[1 + d] + t * n[d];
+        flux[P] += sr * r * eK + t * (Sstar + pK * rinv * rcp_nr(SK - uK));
+    }
+    fp64::deboost<ND>(c, v, flux);
+    return 0;
+}
+
 template <int ND>
 __device__ uint8_t exact_mixed(const CF& c, const Consts& cd, const float* WL, float* WR,
                                const float* n, bool reflective, const double* WL64,
