@@ -58,6 +58,27 @@ __device__ __forceinline__ float ipow5(float b) {
     return b2 * b2 * b;
 }
 
+// 1/x in FP64 from the hardware estimate v_rcp_f64 and two Newton-Raphson
+// steps r <- r + r (1 - x r) (each step doubles the correct bits): a few
+// FP64 FMAs instead of the full division sequence, within an ulp or two of
+// 1/x for normal x; 1/0 = inf and 1/inf = 0 as for a division.
+__device__ __forceinline__ double rcp_nr(double x) {
+    const double r0 = __builtin_amdgcn_rcp(x);
+    double r = fma(fma(-x, r0, 1.0), r0, r0);
+    r = fma(fma(-x, r, 1.0), r, r);
+    // 1/0 and 1/inf: the estimate is exact and the steps would give NaN
+    return (r0 == 0.0 || __builtin_isinf(r0)) ? r0 : r;
+}
+
+// sqrt(x) in FP64 for x >= 0 from the hardware estimate y = v_rsq_f64(x)
+// and one Newton-Raphson step on y = 1/sqrt(x), y <- y (3/2 - x/2 y^2)
+// (relative error about 3/2 of the square of the estimate's), then x y.
+__device__ __forceinline__ double sqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return (x == 0.0) ? 0.0 : x * y;
+}
+
 // x^(1/5) in FP64 for x > 0: exponent split so the FP32 seed stays in range,
 // then one Newton step on y^5 = z.
 __device__ __forceinline__ double fifth_root_d(double x) {
@@ -68,7 +89,7 @@ __device__ __forceinline__ double fifth_root_d(double x) {
     double z = __builtin_ldexp(m, r);            // [0.5, 16)
     double y = (double)exp2f_fast(0.2f * log2f_fast((float)z));
     double y2 = y * y;
-    y = 0.8 * y + 0.2 * z / (y2 * y2);
+    y = 0.8 * y + 0.2 * z * rcp_nr(y2 * y2);
     return __builtin_ldexp(y, q);
 }
 
@@ -121,18 +142,18 @@ __device__ __forceinline__ void side_f(const CF& c, const SideF& s, float p, flo
 __device__ __forceinline__ void side_f_d(const CF& c, double rho, double pK, double a, double p,
                                          double& f, double& df) {
     if(p > pK) {
-        double A = c.tdgp1_d / rho;
+        double A = c.tdgp1_d * rcp_nr(rho);
         double B = c.gm1dgp1_d * pK;
-        double r = 1.0 / (p + B);
-        double sq = sqrt(A * r);
+        double r = rcp_nr(p + B);
+        double sq = sqrt_nr(A * r);
         double dp = p - pK;
         f = dp * sq;
         df = (1.0 - 0.5 * dp * r) * sq;
     } else {
-        double x = p / pK;
+        double x = p * rcp_nr(pK);
         double y = pow_gm1d2g_d(c, x);
         f = c.tdgm1_d * a * (y - 1.0);
-        df = y / (rho * a * x);
+        df = y * rcp_nr(rho * a * x);
     }
 }
 
@@ -227,14 +248,14 @@ __device__ __forceinline__ void exact_setup(const CF& c, const Consts& cd, const
         uRd += WR64[1 + d] * n64[d];
     }
     if(reflective) uRd = -uLd;
-    const double aLd = (WL64[0] != 0.0) ? sqrt(cd.gamma * WL64[P] / WL64[0]) : 0.0;
-    const double aRd = (WR64[0] != 0.0) ? sqrt(cd.gamma * WR64[P] / WR64[0]) : 0.0;
+    const double aLd = (WL64[0] != 0.0) ? sqrt_nr(cd.gamma * WL64[P] * rcp_nr(WL64[0])) : 0.0;
+    const double aRd = (WR64[0] != 0.0) ? sqrt_nr(cd.gamma * WR64[P] * rcp_nr(WR64[0])) : 0.0;
     s.uLd = uLd;
     s.uRd = uRd;
     s.aLd = aLd;
     s.aRd = aRd;
     s.it = 0;
-    if(vacuum_in || 2. * aLd / (cd.gamma - 1.) + 2. * aRd / (cd.gamma - 1.) <= uRd - uLd) {
+    if(vacuum_in || cd.tdgm1 * aLd + cd.tdgm1 * aRd <= uRd - uLd) {
         s.phase = EX_VACUUM;  // closed forms, no iteration
         return;
     }
@@ -339,7 +360,7 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
             double fLd, dLd, fRd, dRd;
             side_f_d(c, WL64[0], WL64[P], aLd, pd, fLd, dLd);
             side_f_d(c, WR64[0], WR64[P], aRd, pd, fRd, dRd);
-            const double step = (fLd + fRd + (uRd - uLd)) / (dLd + dRd);
+            const double step = (fLd + fRd + (uRd - uLd)) * rcp_nr(dLd + dRd);
             ps = pd - step;
             if(!(ps > 0.0)) ps = pd;  // keep the FP32 root if the step would leave (0, inf)
             const double delta = ps - pd;
@@ -355,11 +376,11 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
 #pragma unroll
         for(int k = 0; k < NC; k++) sol[k] = WK[k];
         double vhalf = 0.0;
-        const double pdp = ps / pK;
+        const double pdp = ps * rcp_nr(pK);
         if(ps > pK) {
-            const double S = uK + sgn * aK * sqrt(cd.gp1d2g * pdp + cd.gm1d2g);
+            const double S = uK + sgn * aK * sqrt_nr(cd.gp1d2g * pdp + cd.gm1d2g);
             if(sgn * S > 0.0) {
-                sol[0] = WK[0] * (pdp + cd.gm1dgp1) / (cd.gm1dgp1 * pdp + 1.);
+                sol[0] = WK[0] * (pdp + cd.gm1dgp1) * rcp_nr(cd.gm1dgp1 * pdp + 1.);
                 sol[P] = ps;
                 vhalf = us - uK;
             }
@@ -369,7 +390,7 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
                 const double y = (pdp > 0.0) ? pow_gm1d2g_d(c, pdp) : 0.0;
                 const double ST = us + sgn * aK * y;
                 if(right ? (ST <= 0.0) : (ST > 0.0)) {
-                    const double base = cd.tdgp1 - sgn * cd.gm1dgp1 / aK * uK;
+                    const double base = cd.tdgp1 - sgn * cd.gm1dgp1 * rcp_nr(aK) * uK;
                     sol[0] = WK[0] * (c.fast53 ? ipow3d(base) : pow(base, cd.tdgm1));
                     sol[P] = pK * (c.fast53 ? ipow5d(base) : pow(base, cd.tgdgm1));
                     vhalf = cd.tdgp1 * (-sgn * aK + cd.gm1d2 * uK) - uK;
@@ -383,31 +404,11 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
 #pragma unroll
         for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n64[d];
     }
-    state_flux_d<ND>(1.0 / (cd.gamma - 1.0), sol[0], sol + 1, sol[P], n64, v64, F);
+    state_flux_d<ND>(0.5 * cd.tdgm1, sol[0], sol + 1, sol[P], n64, v64, F);
     bool bad = (sol[0] < 0.0) | (sol[0] != sol[0]) | (sol[P] < 0.0) | (sol[P] != sol[P]);
 #pragma unroll
     for(int d = 0; d < ND; d++) bad |= (sol[1 + d] != sol[1 + d]);
     return bad ? ST_SOLVER : 0;
-}
-
-// 1/x in FP64 from the hardware estimate v_rcp_f64 and two Newton-Raphson
-// steps r <- r + r (1 - x r) (each step doubles the correct bits): a few
-// FP64 FMAs instead of the full division sequence, within an ulp or two of
-// 1/x for normal x.
-__device__ __forceinline__ double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(fma(-x, r, 1.0), r, r);
-    r = fma(fma(-x, r, 1.0), r, r);
-    return r;
-}
-
-// sqrt(x) in FP64 for x >= 0 from the hardware estimate y = v_rsq_f64(x)
-// and one Newton-Raphson step on y = 1/sqrt(x), y <- y (3/2 - x/2 y^2)
-// (relative error about 3/2 of the square of the estimate's), then x y.
-__device__ __forceinline__ double sqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-0.5 * x * y, y, 1.5);
-    return (x == 0.0) ? 0.0 : x * y;
 }
 
 // HLLCRiemannSolver::solve_for_flux (HLLCRiemannSolver.cpp:258-403) for the
