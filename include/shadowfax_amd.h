@@ -156,6 +156,47 @@ int sfx_exact_solve_batch(sfx_solver* h, int64_t N, const double* WL,
 int sfx_time_derivative_batch(sfx_solver* h, int64_t N, const double* W,
                               const double* grad, double* dWdt, void* stream);
 
+/* The integer timeline of TimeLine (src/TimeLine.hpp:52-113, accessors
+ * :145-166): the values calculate_timestep reads.  No cosmology. */
+typedef struct sfx_timeline {
+    uint64_t integer_maxtime; /* _integer_maxtime */
+    uint64_t current_time;    /* _current_time */
+    uint64_t timestep;        /* _timestep, the system step */
+    uint64_t max_timestep;    /* _max_timestep */
+    uint64_t min_timestep;    /* _min_timestep */
+    double tottime;           /* _tottime */
+    double mintime;           /* _mintime */
+    double cfl;               /* _cfl */
+    int32_t global_timestep;  /* ParticleVector::global_timestep() */
+    int32_t pad;
+} sfx_timeline;
+
+#define SFX_ST_TIMESTEP 16u /* the reference exits with "Timestep 0!" */
+
+/* TimeLine::calculate_timestep for the gas cells, no gravity, no tree walk
+ * (src/TimeLine.cpp:233-320, global-timestep tail :418-431), over C cells on
+ * the device, FP64.  A cell is active when endtime == current_time +
+ * timestep; it gets h/(|v - v_gen| + c) (h from the volume as
+ * VorCell::set_h, src/VorCell.cpp:930-938; c = the handle's get_soundspeed,
+ * ExactRiemannSolver.cpp:681-687 / HLLCRiemannSolver.cpp:458-460) times the
+ * CFL number, put on the power-of-two hierarchy of the integer timeline and
+ * made to synchronise; its start and end times advance
+ * (Particle::set_timestep, src/utilities/Particle.cpp:304-307).  In global
+ * mode every cell takes the smallest step.
+ *   W [C][ndim+2], vel (generator velocity) [C][ndim], volume [C]: FP64,
+ *   cell-major, as the mesh arrays;  starttime, endtime [C]: in/out;
+ *   dt [C] out: get_realtime(endtime - starttime), the per-cell value the
+ *   face pass reads (VorFace.cpp:695-699);  active [C] out: the cells that
+ *   were active, i.e. those whose faces the next pass computes
+ *   (VorFace.cpp:675-683);  result: device uint64[2] out, {system step =
+ *   the smallest new step (max_timestep if no cell is active), number of
+ *   active cells};  status [C] out (may be NULL): SFX_ST_TIMESTEP.
+ * Captures into a HIP graph (no memset, no host synchronisation). */
+int sfx_hydro_timestep(sfx_solver* h, int64_t C, const double* W, const double* vel,
+                       const double* volume, const sfx_timeline* tl, uint64_t* starttime,
+                       uint64_t* endtime, double* dt, uint8_t* active, uint64_t* result,
+                       uint8_t* status, void* stream);
+
 /* RiemannSolver::get_W(volume, Q, use_energy = true)
  * (src/riemann/RiemannSolver.hpp:120-121; ExactRiemannSolver.cpp:716-766)
  * over N cells, FP64: W, Q [ndim+2][N], volume [N].  Where the pressure

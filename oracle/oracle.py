@@ -57,6 +57,22 @@ class MeshStruct(ct.Structure):
                 ("cf_face", _i32p)]
 
 
+class Timeline(ct.Structure):
+    """sfxo_timeline: the integer timeline of TimeLine."""
+    _fields_ = [("integer_maxtime", ct.c_uint64), ("current_time", ct.c_uint64),
+                ("timestep", ct.c_uint64), ("max_timestep", ct.c_uint64),
+                ("min_timestep", ct.c_uint64), ("tottime", ct.c_double),
+                ("mintime", ct.c_double), ("cfl", ct.c_double),
+                ("global_timestep", ct.c_int32), ("pad", ct.c_int32)]
+
+
+def timeline(**kw):
+    t = Timeline()
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
 _lib = None
 
 
@@ -98,6 +114,11 @@ def lib():
         L.sfxo_update.restype = ct.c_uint
         L.sfxo_update.argtypes = [ct.POINTER(SolverParams), ct.c_int64, _dp,
                                   _dp, _dp, _dp, _u8p]
+        _u64p = ct.POINTER(ct.c_uint64)
+        L.sfxo_hydro_timestep.restype = ct.c_uint
+        L.sfxo_hydro_timestep.argtypes = [
+            ct.POINTER(SolverParams), ct.c_int64, _dp, _dp, _dp,
+            ct.POINTER(Timeline), _u64p, _u64p, _dp, _u8p, _u64p, _u64p, _u8p]
         _lib = L
     return _lib
 
@@ -277,6 +298,28 @@ def update(kind, ndim, Q, dQ, volume, gamma=5.0 / 3.0):
     lib().sfxo_update(ct.byref(s), C, _p(Q), _p(dQ), _p(volume), _p(W),
                       _p(st, _u8p))
     return Q, W, st
+
+
+def hydro_timestep(kind, ndim, W, vel, volume, tl, starttime, endtime, gamma=5.0 / 3.0):
+    """TimeLine::calculate_timestep (gas, no gravity) over C cells; W [C][ndim+2],
+    vel [C][ndim] item-major.  Returns (start, end, dt, active, system_step,
+    nactive, status)."""
+    s = params(kind, ndim, gamma)
+    W = _c(W)
+    vel = _c(vel)
+    volume = _c(volume)
+    C = volume.shape[0]
+    start = _c(starttime, np.uint64).copy()
+    end = _c(endtime, np.uint64).copy()
+    dt = np.zeros(C)
+    act = np.zeros(C, np.uint8)
+    st = np.zeros(C, np.uint8)
+    ss, na = ct.c_uint64(0), ct.c_uint64(0)
+    u64 = ct.POINTER(ct.c_uint64)
+    lib().sfxo_hydro_timestep(ct.byref(s), C, _p(W), _p(vel), _p(volume), ct.byref(tl),
+                              _p(start, u64), _p(end, u64), _p(dt), _p(act, _u8p),
+                              ct.byref(ss), ct.byref(na), _p(st, _u8p))
+    return start, end, dt, act, ss.value, na.value, st
 
 
 # ------------------------------------------------------------------ _ref --

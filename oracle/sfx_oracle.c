@@ -26,11 +26,21 @@
  *       cases, brute force on tiny meshes, and invariants: dQR == -dQL
  *       bitwise, conservation to round-off, uniform flow gives zero net dQ,
  *       zero gradients reduce the face flux to solve_for_flux exactly.
+ *   hydro time step (TimeLine::calculate_timestep) -- UNPINNED by a reference
+ *       run (TimeLine.cpp needs Boost.property_tree).  Pinned by a brute-force
+ *       search over all power-of-two fractions of the timeline on small
+ *       timelines, hand-worked cases and the invariants of the hierarchy
+ *       (the step is the largest admissible one, it divides the remaining
+ *       time, it stays within [min, max]).
  */
 #include "sfx_oracle.h"
 
 #include <math.h>
 #include <string.h>
+
+#ifndef M_PI /* strict C11 <math.h> leaves it out; glibc's value */
+#define M_PI 3.14159265358979323846
+#endif
 
 /* std::max / std::min semantics (src/StateVector.hpp uses std::max/min):
  * std::max(a,b) = (a < b) ? b : a ; std::min(a,b) = (b < a) ? b : a */
@@ -1254,4 +1264,95 @@ unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
         all |= st;
     }
     return all;
+}
+
+/* ------------------------------------------------------------ time step -- */
+
+/* TimeLine::get_realtime / get_realtime_interval (src/TimeLine.hpp:145-166),
+ * no cosmology */
+static double tl_realtime(const sfxo_timeline* tl, uint64_t t) {
+    double x = (double)t / (double)tl->integer_maxtime;
+    return tl->tottime * x + tl->mintime;
+}
+static double tl_interval(const sfxo_timeline* tl, uint64_t t) {
+    double x = (double)t / (double)tl->integer_maxtime;
+    return tl->tottime * x;
+}
+
+/* TimeLine::calculate_timestep, gas branch (src/TimeLine.cpp:233-320) and the
+ * global-timestep tail (:418-431) */
+unsigned sfxo_hydro_timestep(const sfxo_solver* s, int64_t C, const double* W,
+                             const double* vel, const double* volume,
+                             const sfxo_timeline* tl, uint64_t* starttime,
+                             uint64_t* endtime, double* dt, uint8_t* active,
+                             uint64_t* system_step, uint64_t* nactive,
+                             uint8_t* cell_status) {
+    const int nd = s->ndim, NC = nd + 2, P = nd + 1;
+    const uint64_t Smax = tl->max_timestep;
+    const uint64_t now = tl->current_time + tl->timestep;
+    uint64_t Smin = Smax, na = 0;
+    unsigned st_all = 0;
+    for(int64_t i = 0; i < C; i++) {
+        uint8_t st = 0;
+        const int act = endtime[i] == now;
+        active[i] = (uint8_t)act;
+        if(act) {
+            na++;
+            const double* w = W + i * NC;
+            /* v = W.v - particle velocity; Vec::norm (src/Vec.hpp:140,378) */
+            double vx = w[1] - vel[i * nd], vy = w[2] - vel[i * nd + 1];
+            double vi;
+            if(nd == 3) {
+                double vz = w[3] - vel[i * nd + 2];
+                vi = sqrt(vx * vx + vy * vy + vz * vz);
+            } else {
+                vi = sqrt(vx * vx + vy * vy);
+            }
+            /* get_soundspeed: ExactRiemannSolver.cpp:681-687 guards rho = 0,
+             * HLLCRiemannSolver.cpp:458-460 does not */
+            double ci;
+            if(s->kind == SFXO_EXACT && !(w[0] != 0.))
+                ci = 0.;
+            else
+                ci = sqrt(s->gamma * w[P] / w[0]);
+            /* VorCell::set_h, src/VorCell.cpp:930-938 */
+            const double V = volume[i];
+            const double h = (nd == 3) ? cbrt(V * 3. / 4. / M_PI) : sqrt(V / M_PI);
+            const double rdt = h / (vi + ci);
+            const double S = tl->cfl * rdt;
+            uint64_t S2;
+            if(tl_interval(tl, Smax) < S) {
+                S2 = Smax;
+            } else {
+                S2 = tl->integer_maxtime;
+                while(tl_interval(tl, S2) > S) S2 >>= 1;
+                if(!S2) {
+                    st = SFXO_ST_TIMESTEP;
+                } else {
+                    while((tl->integer_maxtime - endtime[i]) % S2) S2 >>= 1;
+                    S2 = (S2 < tl->min_timestep) ? tl->min_timestep : S2;
+                    S2 = (S2 > tl->max_timestep) ? tl->max_timestep : S2;
+                }
+            }
+            if(!st) {
+                Smin = (S2 < Smin) ? S2 : Smin;
+                if(!tl->global_timestep) {
+                    starttime[i] = endtime[i];
+                    endtime[i] = starttime[i] + S2;
+                }
+            }
+        }
+        if(cell_status) cell_status[i] = st;
+        st_all |= st;
+    }
+    if(tl->global_timestep) {
+        for(int64_t i = 0; i < C; i++) {
+            starttime[i] = endtime[i];
+            endtime[i] = starttime[i] + Smin;
+        }
+    }
+    for(int64_t i = 0; i < C; i++) dt[i] = tl_realtime(tl, endtime[i] - starttime[i]);
+    *system_step = Smin;
+    *nactive = na;
+    return st_all;
 }

@@ -148,6 +148,40 @@ unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
 unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
                      const double* volume, double* W, uint8_t* cell_status);
 
+/* The integer timeline of TimeLine (src/TimeLine.hpp:52-113): the values
+ * calculate_timestep reads. */
+typedef struct {
+    uint64_t integer_maxtime; /* _integer_maxtime */
+    uint64_t current_time;    /* _current_time */
+    uint64_t timestep;        /* _timestep (system step) */
+    uint64_t max_timestep;    /* _max_timestep */
+    uint64_t min_timestep;    /* _min_timestep */
+    double tottime;           /* _tottime */
+    double mintime;           /* _mintime */
+    double cfl;               /* _cfl */
+    int32_t global_timestep;  /* ParticleVector::global_timestep() */
+    int32_t pad;
+} sfxo_timeline;
+
+/* TimeLine::calculate_timestep for the gas cells, no gravity, no tree walk
+ * (src/TimeLine.cpp:233-320,418-431): cells with endtime == current_time +
+ * timestep are active; each gets the CFL time step h/(|v - v_gen| + c)
+ * (h from the volume as VorCell::set_h, src/VorCell.cpp:930-938; c from the
+ * solver's get_soundspeed), put on the power-of-two hierarchy of the integer
+ * timeline and made to synchronise; start/end times advance
+ * (Particle::set_timestep).  W: [C][ndim+2], vel: [C][ndim] (item-major).
+ * Outputs: dt[C] = get_realtime(end - start) (the value VorFace reads),
+ * active[C], *system_step = the smallest new step (Smax if none), *nactive;
+ * cell_status bit SFXO_ST_TIMESTEP where the reference exits on "Timestep 0".
+ * Returns the OR of the status bits. */
+#define SFXO_ST_TIMESTEP 16u
+unsigned sfxo_hydro_timestep(const sfxo_solver* s, int64_t C, const double* W,
+                             const double* vel, const double* volume,
+                             const sfxo_timeline* tl, uint64_t* starttime,
+                             uint64_t* endtime, double* dt, uint8_t* active,
+                             uint64_t* system_step, uint64_t* nactive,
+                             uint8_t* cell_status);
+
 #ifdef __cplusplus
 }
 #endif

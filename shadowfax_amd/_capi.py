@@ -87,11 +87,22 @@ def lib():
     L.sfx_solver_get_split.argtypes = [vp]
     L.sfx_get_W_batch.argtypes = [vp, i64, vp, vp, vp, vp]
     L.sfx_get_Q_batch.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.sfx_hydro_timestep.argtypes = [vp, i64, vp, vp, vp, ct.POINTER(Timeline), vp, vp, vp, vp,
+                                     vp, vp, vp]
     for name, args in _mesh_sigs().items():
         if hasattr(L, name):
             getattr(L, name).argtypes = args
     _lib = L
     return L
+
+
+class Timeline(ct.Structure):
+    """sfx_timeline: the integer timeline of TimeLine (src/TimeLine.hpp:52-113)."""
+    _fields_ = [("integer_maxtime", ct.c_uint64), ("current_time", ct.c_uint64),
+                ("timestep", ct.c_uint64), ("max_timestep", ct.c_uint64),
+                ("min_timestep", ct.c_uint64), ("tottime", ct.c_double),
+                ("mintime", ct.c_double), ("cfl", ct.c_double),
+                ("global_timestep", ct.c_int32), ("pad", ct.c_int32)]
 
 
 def _mesh_sigs():

@@ -313,6 +313,38 @@ int sfx_get_W_batch(sfx_solver* h, int64_t N, const double* volume, double* Q, d
     return e == hipSuccess ? SFX_OK : hip_error(e, "get_W kernel launch");
 }
 
+int sfx_hydro_timestep(sfx_solver* h, int64_t C, const double* W, const double* vel,
+                       const double* volume, const sfx_timeline* tl, uint64_t* starttime,
+                       uint64_t* endtime, double* dt, uint8_t* active, uint64_t* result,
+                       uint8_t* status, void* stream) {
+    if(!h || !tl) return set_error(SFX_ERR_ARG, "sfx_hydro_timestep: NULL handle or timeline");
+    if(C < 0) return set_error(SFX_ERR_ARG, "sfx_hydro_timestep: C < 0");
+    if(!result) return set_error(SFX_ERR_ARG, "sfx_hydro_timestep: NULL result");
+    if(C > 0 && (!W || !vel || !volume || !starttime || !endtime || !dt || !active))
+        return set_error(SFX_ERR_ARG, "sfx_hydro_timestep: NULL array");
+    if(tl->integer_maxtime == 0 || tl->max_timestep == 0 ||
+       tl->min_timestep > tl->max_timestep)
+        return set_error(SFX_ERR_ARG,
+                         "sfx_hydro_timestep: integer_maxtime and max_timestep must be > 0 "
+                         "and min_timestep <= max_timestep");
+    int rc = bind_device(h);
+    if(rc) return rc;
+    TimelineArgs a;
+    a.integer_maxtime = tl->integer_maxtime;
+    a.current_time = tl->current_time;
+    a.timestep = tl->timestep;
+    a.max_timestep = tl->max_timestep;
+    a.min_timestep = tl->min_timestep;
+    a.tottime = tl->tottime;
+    a.mintime = tl->mintime;
+    a.cfl = tl->cfl;
+    a.global_timestep = tl->global_timestep != 0;
+    hipError_t e = launch_hydro_timestep(h->kind, h->ndim, h->c.gamma, C, a, W, vel, volume,
+                                         starttime, endtime, dt, active, result, status,
+                                         (hipStream_t)stream);
+    return e == hipSuccess ? SFX_OK : hip_error(e, "time step kernel launch");
+}
+
 int sfx_get_Q_batch(sfx_solver* h, int64_t N, const double* volume, const double* W, double* Q,
                     void* stream) {
     if(!h) return set_error(SFX_ERR_ARG, "sfx_get_Q_batch: NULL handle");

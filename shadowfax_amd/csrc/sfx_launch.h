@@ -33,6 +33,18 @@ hipError_t launch_cell_op(int op, int kind, int ndim, double gamma, int64_t N,
                           const double* a, const double* b, double* c, double* d,
                           hipStream_t s);
 
+// cell_ops.hip: TimeLine::calculate_timestep over C cells (gas, no gravity)
+struct TimelineArgs {
+    uint64_t integer_maxtime, current_time, timestep, max_timestep, min_timestep;
+    double tottime, mintime, cfl;
+    int global_timestep;
+};
+hipError_t launch_hydro_timestep(int kind, int ndim, double gamma, int64_t C,
+                                 const TimelineArgs& tl, const double* W, const double* vel,
+                                 const double* volume, uint64_t* starttime, uint64_t* endtime,
+                                 double* dt, uint8_t* active, uint64_t* result,
+                                 uint8_t* status, hipStream_t s);
+
 // riemann_mixed.hip: the exact solver in two phases (stop = FP32 iterations
 // before a problem is queued for the persistent second phase)
 size_t exact_split_scratch_bytes(int64_t N);

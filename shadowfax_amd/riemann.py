@@ -228,6 +228,42 @@ class RiemannSolver:
             _ptr(W, f64, (NC, N), "W"), _stream(stream)), "get_W_batch")
         return W
 
+    def hydro_timestep(self, W, vel, volume, timeline, starttime, endtime, dt=None,
+                       active=None, status=None, result=None, stream=None):
+        """TimeLine::calculate_timestep for gas cells (src/TimeLine.cpp:233-320):
+        W [C][ndim+2], vel [C][ndim], volume [C] (FP64, cell-major);
+        starttime/endtime [C] int64 tensors holding the unsigned times,
+        advanced in place; timeline: dict (or _capi.Timeline) of integer_maxtime,
+        current_time, timestep, max_timestep, min_timestep, tottime, mintime,
+        cfl, global_timestep.  Returns (dt, active, status, result) with
+        result = device int64[2]: {system step, number of active cells}."""
+        import torch
+        nd, NC = self.ndim, self.ndim + 2
+        Cn = volume.shape[0]
+        f64, dev = torch.float64, volume.device
+        if dt is None:
+            dt = torch.empty(Cn, dtype=f64, device=dev)
+        if active is None:
+            active = torch.empty(Cn, dtype=torch.uint8, device=dev)
+        if status is None:
+            status = torch.empty(Cn, dtype=torch.uint8, device=dev)
+        if result is None:
+            result = torch.empty(2, dtype=torch.int64, device=dev)
+        tl = timeline
+        if isinstance(timeline, dict):
+            tl = C.Timeline()
+            for k, v in timeline.items():
+                setattr(tl, k, v)
+        i64 = torch.int64
+        C.check(C.lib().sfx_hydro_timestep(
+            self._h, Cn, _ptr(W, f64, (Cn, NC), "W"), _ptr(vel, f64, (Cn, nd), "vel"),
+            _ptr(volume, f64, (Cn,), "volume"), ct.byref(tl),
+            _ptr(starttime, i64, (Cn,), "starttime"), _ptr(endtime, i64, (Cn,), "endtime"),
+            _ptr(dt, f64, (Cn,), "dt"), _ptr(active, torch.uint8, (Cn,), "active"),
+            _ptr(result, i64, (2,), "result"), _ptr(status, torch.uint8, (Cn,), "status"),
+            _stream(stream)), "hydro_timestep")
+        return dt, active, status, result
+
     def get_Q_batch(self, volume, W, Q=None, stream=None):
         """RiemannSolver::get_Q(volume, W) per column."""
         import torch

@@ -59,15 +59,18 @@ def test_header_compiles_as_c():
 
 
 def test_struct_layouts_match_ctypes(tmp_path):
-    """The ctypes mirrors of sfx_mesh and sfx_host_mesh have the C layout
-    (size and every field offset)."""
+    """The ctypes mirrors of sfx_mesh, sfx_host_mesh and sfx_timeline have
+    the C layout (size and every field offset), and the oracle's timeline
+    matches the ABI's."""
     import ctypes as ct
 
     from shadowfax_amd import mesh as M
 
     root = os.path.dirname(os.path.dirname(C.LIBPATH))
     lines = []
-    for cname, py in (("sfx_mesh", C.MeshDesc), ("sfx_host_mesh", M.HostMesh)):
+    structs = (("sfx_mesh", C.MeshDesc), ("sfx_host_mesh", M.HostMesh),
+               ("sfx_timeline", C.Timeline))
+    for cname, py in structs:
         lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
         for f, _ in py._fields_:
             lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
@@ -79,10 +82,12 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
     got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
-    for cname, py in (("sfx_mesh", C.MeshDesc), ("sfx_host_mesh", M.HostMesh)):
+    for cname, py in structs:
         assert got[(cname, "size")] == ct.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+    import oracle as O
+    assert O.Timeline._fields_ == C.Timeline._fields_
 
 
 def test_entry_record_sizes():
