@@ -832,7 +832,8 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                     st |= mixed::exact_mixed<ND>(cf, cd, wlf, wrf, nf, !has_r, wl64, wr64, n64,
                                                  v64, flux, iters, vac);
                 } else {
-                    st |= mixed::hllc_mixed<ND>(cd, wl64, wr64, n64, v64, !has_r, flux, vac);
+                    st |= mixed::hllc_mixed<ND>(cf, cd, wl64, wr64, n64, v64, !has_r, flux,
+                                                vac);
                 }
             }
             // dQL = (dt A) F, dQR = -dQL (src/VorFace.cpp:834-835)

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void solve_for_flux_kernel(
             for(int d = 0; d < ND; d++) nn[d] = (float)n64[d];
             st = exact_mixed<ND>(c, cd, wl, wr, nn, r, wl64, wr64, n64, v64, f, iters, vac);
         } else {
-            st = hllc_mixed<ND>(cd, wl64, wr64, n64, v64, r, f, vac);
+            st = hllc_mixed<ND>(c, cd, wl64, wr64, n64, v64, r, f, vac);
         }
 #pragma unroll
         for(int k = 0; k < NC; k++) F[k * N + i] = (T)f[k];

@@ -334,6 +334,68 @@ __device__ __forceinline__ void exact_iterate(const CF& c, ExactCase& s, int sto
     s.phase = phase;
 }
 
+// The half state at x/t = 0 of a Riemann problem with vacuum on one side or
+// vacuum generated in between: ExactRiemannSolver::solve_vacuum
+// (ExactRiemannSolver.cpp:304-446) and the state part of
+// HLLCRiemannSolver::vacuum_flux (HLLCRiemannSolver.cpp:44-209) decide the
+// same cases in the same order (the two files spell the comparisons with
+// their operands swapped).  Mixed path: the case is decided first and the
+// rarefaction-fan state formed once, with rcp_nr for the quotients and, for
+// gamma = 5/3, the fan powers 2/(g-1) = 3 and 2g/(g-1) = 5 as products
+// (pow otherwise).  In a batch with vacuum problems nearly every wavefront
+// holds one, so this path's cost is paid by almost every wave.
+template <int ND>
+__device__ __forceinline__ void vacuum_state_mixed(const CF& c, const Consts& cd,
+                                                   const double* WL, const double* WR, double vL,
+                                                   double vR, double aL, double aR,
+                                                   const double* n, double* sol) {
+    constexpr int NC = ND + 2, P = ND + 1;
+    // 0: zero state, 1: WL, 2: WR; fan: the rarefaction fan of that side
+    int which = 0;
+    bool fan = false;
+    const bool rl = WL[0] != 0., rr = WR[0] != 0.;
+    if(!rr && !rl) {
+        which = 0;
+    } else if(!rr) {
+        which = 1;
+        if(vL < aL) {
+            fan = vL + cd.tdgm1 * aL > 0.;
+            which = fan ? 1 : 0;
+        }
+    } else if(!rl) {
+        which = 2;
+        if(-aR < vR) {
+            fan = !(vR - cd.tdgm1 * aR >= 0.);
+            which = fan ? 2 : 0;
+        }
+    } else {
+        const double SR = vR - cd.tdgm1 * aR, SL = vL + cd.tdgm1 * aL;
+        if(SR > 0. && SL < 0.) {
+            which = 0;
+        } else if(SL >= 0.) {
+            which = 1;
+            fan = aL > vL;
+        } else {
+            which = 2;
+            fan = -aR < vR;
+        }
+    }
+    const double* WK = (which == 2) ? WR : WL;
+    double vhalf = 0.;
+#pragma unroll
+    for(int k = 0; k < NC; k++) sol[k] = (which == 0) ? 0. : WK[k];
+    if(fan) {
+        const double sgn = (which == 1) ? 1. : -1.;
+        const double aK = (which == 1) ? aL : aR, vK = (which == 1) ? vL : vR;
+        const double base = cd.tdgp1 + sgn * cd.gm1dgp1 * rcp_nr(aK) * vK;
+        vhalf = cd.tdgp1 * (sgn * aK + cd.gm1d2 * vK) - vK;
+        sol[0] = WK[0] * (c.fast53 ? ipow3d(base) : pow(base, cd.tdgm1));
+        sol[P] = WK[P] * (c.fast53 ? ipow5d(base) : pow(base, cd.tgdgm1));
+    }
+#pragma unroll
+    for(int d = 0; d < ND; d++) sol[1 + d] += vhalf * n[d];
+}
+
 // FP64 final Newton step, sampling at x/t = 0 and flux; returns the status.
 template <int ND>
 __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, const ExactCase& s,
@@ -346,7 +408,7 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
     if(s.phase == EX_VACUUM) {
         // closed forms: the FP64 reference expressions
         vac = 1;
-        fp64::ex_solve_vacuum<ND>(cd, WL64, WR64, uLd, uRd, aLd, aRd, n64, sol);
+        vacuum_state_mixed<ND>(c, cd, WL64, WR64, uLd, uRd, aLd, aRd, n64, sol);
     } else {
         iters += s.it + 1;
         double ps, us;
@@ -429,9 +491,37 @@ __device__ __forceinline__ uint8_t exact_finish(const CF& c, const Consts& cd, c
 // pressure flux with FP32 eps).  Vacuum faces take the FP64 closed forms; a
 // reflective face has S* = 0 exactly (the FP64 reference's numerator
 // vanishes identically there).
+// HLLCRiemannSolver::vacuum_flux's flux of the half state
+// (HLLCRiemannSolver.cpp:180-208), mixed path
 template <int ND>
-__device__ uint8_t hllc_mixed(const Consts& c, const double* WL, double* WR, const double* n,
-                              const double* v, bool reflective, double* flux, unsigned& vac) {
+__device__ __forceinline__ void vacuum_flux_mixed(const CF& cf, const Consts& c, const double* WL,
+                                                  const double* WR, const double* n,
+                                                  const double* v, double uL, double uR,
+                                                  double aL, double aR, double* flux) {
+    constexpr int NC = ND + 2, P = ND + 1;
+    double sol[NC];
+    vacuum_state_mixed<ND>(cf, c, WL, WR, uL, uR, aL, aR, n, sol);
+#pragma unroll
+    for(int k = 0; k < NC; k++) flux[k] = 0.;
+    if(!(sol[0] != 0.)) return;
+    double vsol = sol[1] * n[0] + sol[2] * n[1];
+    double v2 = sol[1] * sol[1] + sol[2] * sol[2];
+    if(ND == 3) {
+        vsol += sol[ND] * n[ND - 1];
+        v2 += sol[ND] * sol[ND];
+    }
+    flux[0] = sol[0] * vsol;
+#pragma unroll
+    for(int d = 0; d < ND; d++) flux[1 + d] = sol[0] * sol[1 + d] * vsol + sol[P] * n[d];
+    const double e = 0.5 * c.tdgm1 * sol[P] * rcp_nr(sol[0]) + 0.5 * v2;
+    flux[P] = sol[0] * e * vsol + sol[P] * vsol;
+    fp64::deboost<ND>(c, v, flux);
+}
+
+template <int ND>
+__device__ uint8_t hllc_mixed(const CF& cf, const Consts& c, const double* WL, double* WR,
+                              const double* n, const double* v, bool reflective, double* flux,
+                              unsigned& vac) {
     constexpr int NC = ND + 2, P = ND + 1;
     const double g = c.gamma;
 #pragma unroll
@@ -448,18 +538,13 @@ __device__ uint8_t hllc_mixed(const Consts& c, const double* WL, double* WR, con
         for(int k = 0; k < NC; k++) WR[k] = WL[k];
         uR = -uL;
     }
-    if(!(WL[0] != 0.) || !(WR[0] != 0.)) {
-        vac = 1;
-        const double aL = sqrt(g * WL[P] / WL[0]), aR = sqrt(g * WR[P] / WR[0]);
-        fp64::hllc_vacuum_flux<ND>(c, WL, WR, n, v, uL, uR, aL, aR, flux);
-        return 0;
-    }
+    // sound speeds: HLLCRiemannSolver::get_soundspeed has no rho = 0 guard;
+    // the vacuum cases below read a side's speed only where its rho != 0
     const double aL = sqrt_nr(g * WL[P] * rcp_nr(WL[0]));
     const double aR = sqrt_nr(g * WR[P] * rcp_nr(WR[0]));
-    if(c.tdgm1 * aL + c.tdgm1 * aR <= uR - uL) {
+    if(!(WL[0] != 0.) || !(WR[0] != 0.) || c.tdgm1 * aL + c.tdgm1 * aR <= uR - uL) {
         vac = 1;
-        const double aL64 = sqrt(g * WL[P] / WL[0]), aR64 = sqrt(g * WR[P] / WR[0]);
-        fp64::hllc_vacuum_flux<ND>(c, WL, WR, n, v, uL, uR, aL64, aR64, flux);
+        vacuum_flux_mixed<ND>(cf, c, WL, WR, n, v, uL, uR, aL, aR, flux);
         return 0;
     }
     const double rhobar = 0.5 * (WL[0] + WR[0]);
