@@ -324,7 +324,7 @@ __device__ __forceinline__ void predict_side_offset(float gamma, const double* W
         // |d|/r and phibar - phiL in FP64: the limiter's bound is an offset
         // of the size of the jump itself, rounded once
         const double d0 = d[0], d1v = d[1], d2v = (ND == 3) ? (double)d[ND - 1] : 0.0;
-        const double t = sqrt(d0 * d0 + d1v * d1v + d2v * d2v) / (double)r;
+        const double t = mixed::sqrt_nr(d0 * d0 + d1v * d1v + d2v * d2v) * mixed::rcp_nr((double)r);
 #pragma unroll
         for(int k = 0; k < NC; k++) {
             const double phiL = Wraw[k], phiR = Wother[k];
@@ -493,7 +493,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             }
         }
     }
-    const double Vinv = 1.0 / m.volume[i];
+    const double Vinv = mixed::rcp_nr(m.volume[i]);
 #pragma unroll
     for(int l = 0; l < ND; l++)
 #pragma unroll
@@ -531,7 +531,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     }
 #pragma unroll
     for(int k = 0; k < NC; k++) {
-        const double alpha = na[k] / nb[k];
+        const double alpha = na[k] * mixed::rcp_nr(nb[k]);
 #pragma unroll
         for(int l = 0; l < ND; l++)
             grad[((i + gshift) * ND + l) * NC + k] = (float)(delta[l][k] * alpha);
@@ -720,7 +720,10 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                 R fac = (vl[0] - vr[0]) * mcv[0];
                 fac += (vl[1] - vr[1]) * mcv[1];
                 if(ND == 3) fac += (vl[ND - 1] - vr[ND - 1]) * mcv[ND - 1];
-                fac /= rRL2;
+                if constexpr(MIXED)
+                    fac *= mixed::rcp_nr(rRL2);
+                else
+                    fac /= rRL2;
 #pragma unroll
                 for(int l = 0; l < ND; l++) vface[l] = R(0.5) * (vl[l] + vr[l]) + fac * rr[l];
             } else {
@@ -807,12 +810,12 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                     st |= fp64::hllc_solve_for_flux<ND>(cd, wl, wr, n64, v64, !has_r, flux, vac);
             } else {
                 double n64[ND], v64[ND], wl64[NC], wr64[NC];
-                const double nrm64 = sqrt((double)rr[0] * (double)rr[0] +
-                                          (double)rr[1] * (double)rr[1] +
-                                          (ND == 3 ? (double)rr[ND - 1] * (double)rr[ND - 1] : 0.0));
+                const double ninv = mixed::rcp_nr(mixed::sqrt_nr(
+                        (double)rr[0] * (double)rr[0] + (double)rr[1] * (double)rr[1] +
+                        (ND == 3 ? (double)rr[ND - 1] * (double)rr[ND - 1] : 0.0)));
 #pragma unroll
                 for(int l = 0; l < ND; l++) {
-                    n64[l] = (double)rr[l] / nrm64;
+                    n64[l] = (double)rr[l] * ninv;
                     v64[l] = (double)vface[l];
                 }
 #pragma unroll
@@ -842,25 +845,32 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
 #pragma unroll
             for(int k = 0; k < NC; k++) dQ[k] = dA * flux[k];
             // mass flux limiter, :840-866; dQR stays -dQL bitwise
+            // the mixed pass takes its quotients through rcp_nr
+            auto quot = [](double a, double b) {
+                if constexpr(MIXED)
+                    return a * mixed::rcp_nr(b);
+                else
+                    return a / b;
+            };
             double timefac = 1.;
             if(has_r) {
                 double dtmax = m.dt[L];
                 dtmax = smax(dtmax, m.dt[Rc]);
-                timefac = dt / dtmax;
+                timefac = quot(dt, dtmax);
             }
-            double areafac = (double)A / m.total_area[L];
+            double areafac = quot((double)A, m.total_area[L]);
             areafac *= timefac;
             if(dQ[0] > areafac * m.mass[L]) {
-                double corfac = fabs(areafac * m.mass[L] / dQ[0]);
+                double corfac = fabs(quot(areafac * m.mass[L], dQ[0]));
 #pragma unroll
                 for(int k = 0; k < NC; k++) dQ[k] *= corfac;
             }
             if(has_r) {
-                double af = (double)A / m.total_area[Rc];
+                double af = quot((double)A, m.total_area[Rc]);
                 af *= timefac;
                 // dQR[0] = -dQL[0]
                 if(-dQ[0] > af * m.mass[Rc]) {
-                    double corfac = fabs(af * m.mass[Rc] / -dQ[0]);
+                    double corfac = fabs(quot(af * m.mass[Rc], -dQ[0]));
 #pragma unroll
                     for(int k = 0; k < NC; k++) dQ[k] = -(-dQ[k] * corfac);
                 }
