@@ -448,13 +448,17 @@ int sfx_ensemble_sync(sfx_ensemble* e);
 /* Copies W, Q or dQ (SFX_ENS_*) of one mesh to host_out ([C][ndim+2]). */
 int sfx_ensemble_read(sfx_ensemble* e, int32_t id, int what, double* host_out);
 
-/* Launch shape of the persistent kernels: threads per block (= cells or
- * faces per work tile; 64..256, a multiple of 64) and blocks per CU. */
+/* Launch shape of the batched step's kernels: threads per block (= cells
+ * or faces per work tile; 64..256, a multiple of 64) and blocks per CU of
+ * the persistent grid (1..32), or 0 for one block per possible tile (the
+ * grid then covers the largest tile count the arena allows, and blocks
+ * without a tile exit at once). */
 int sfx_ensemble_set_launch(sfx_ensemble* e, int block, int blocks_per_cu);
 int sfx_ensemble_get_launch(const sfx_ensemble* e, int* block, int* blocks_per_cu);
 /* Times reps batched steps of ids for each block size in {64, 128, 256}
- * and blocks per CU in {1, 2, 4, 8}, keeps the fastest (best_ms: its time
- * per step) and restores the meshes' states afterwards. */
+ * and blocks per CU in {0 (one block per tile), 1, 2, 4, 8}, keeps the
+ * fastest (best_ms: its time per step) and restores the meshes' states
+ * afterwards. */
 int sfx_ensemble_autotune(sfx_ensemble* e, const int32_t* ids, int32_t n, int reps,
                           double* best_ms);
 

@@ -13,7 +13,8 @@
 // it zeroes the tile counters) followed by four persistent drains --
 // gradients, face fluxes, gather, update -- each a fixed grid whose blocks
 // claim tiles of `tile` consecutive cells or faces, eight per atomic, until
-// none are left.
+// none are left (or, as the autotuner may choose, a grid of one block per
+// possible tile, see drain).
 // The tile range of a phase is cut into one contiguous chunk per XCD; a
 // block drains the chunk of the XCD it runs on (dispatch index % 8) first,
 // then the others, so neighbouring cells mostly meet in one L2.  Every block
@@ -68,11 +69,34 @@ __device__ __forceinline__ int find_slot(const Plan& p, int64_t t) {
 // Drains the tiles of one phase: fn(slot, item) for every item of every
 // admitted mesh.  Called by every thread of the block.
 template <bool FACES, class Fn>
-__device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
+__device__ __forceinline__ void run_tile(const Plan* __restrict__ p, int64_t t, int tile, Fn& fn) {
+    const int s = find_slot<FACES>(*p, t);
+    const Slot& sl = p->slot[s];
+    const int64_t base = FACES ? sl.face0 + (t - sl.ftile0) * tile
+                               : sl.cell0 + (t - sl.ctile0) * tile;
+    const int64_t end = FACES ? sl.face0 + sl.nface : sl.cell0 + sl.ncell;
+    const int64_t item = base + threadIdx.x;
+    if(threadIdx.x < tile && item < end) fn(sl, item);
+}
+
+// grid != 0: one tile per block instead -- the grid covers the largest tile
+// count the arena allows, block b takes tile lo_x + b / 8 of the chunk of
+// XCD x = b % 8 (the same chunks as the persistent drains), and blocks past
+// their chunk's end exit at once; the hardware then keeps as many blocks
+// in flight as registers allow, as for a standalone kernel.
+template <bool FACES, class Fn>
+__device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, int grid, Fn fn) {
     __shared__ int64_t s_tile;
     // tiles claimed per atomic: the counters see tiles / kClaim requests
     constexpr int kClaim = 8;
     const int64_t T = FACES ? p->ftiles : p->ctiles;
+    if(grid) {
+        const int x = blockIdx.x % kXcd;
+        const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
+        const int64_t t = lo + blockIdx.x / kXcd;
+        if(t < hi) run_tile<FACES>(p, t, p->tile, fn);
+        return;
+    }
     // with fewer claims than blocks the first ones take them all
     if((int64_t)blockIdx.x * kClaim >= T) return;
     const int tile = p->tile;
@@ -94,23 +118,15 @@ __device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, Fn fn) {
             __syncthreads();
             if(t0 >= hi) break;
             const int64_t t1 = (t0 + kClaim < hi) ? t0 + kClaim : hi;
-            for(int64_t t = t0; t < t1; t++) {
-                const int s = find_slot<FACES>(*p, t);
-                const Slot& sl = p->slot[s];
-                const int64_t base = FACES ? sl.face0 + (t - sl.ftile0) * tile
-                                           : sl.cell0 + (t - sl.ctile0) * tile;
-                const int64_t end = FACES ? sl.face0 + sl.nface : sl.cell0 + sl.ncell;
-                const int64_t item = base + threadIdx.x;
-                if(threadIdx.x < tile && item < end) fn(sl, item);
-            }
+            for(int64_t t = t0; t < t1; t++) run_tile<FACES>(p, t, tile, fn);
         }
     }
 }
 
 template <int ND, typename T, typename R, bool DIFF>
 __global__ __launch_bounds__(256) void drain_gradients(flux::MeshView<T> m, T* __restrict__ grad,
-                                                       Plan* __restrict__ p) {
-    drain<false>(p, 0, [&](const Slot& sl, int64_t i) {
+                                                       Plan* __restrict__ p, int grid) {
+    drain<false>(p, 0, grid, [&](const Slot& sl, int64_t i) {
         flux::grad_cell<ND, T, R, DIFF>(m, i, grad, sl.gshift);
     });
 }
@@ -121,9 +137,9 @@ __global__ __launch_bounds__(256) void drain_faces(flux::MeshView<T> m, const T*
                                                    T* __restrict__ dQL,
                                                    uint8_t* __restrict__ fstatus,
                                                    unsigned long long* __restrict__ ctr,
-                                                   Plan* __restrict__ p) {
+                                                   Plan* __restrict__ p, int grid) {
     unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
-    drain<true>(p, 1, [&](const Slot& sl, int64_t f) {
+    drain<true>(p, 1, grid, [&](const Slot& sl, int64_t f) {
         flux::face_item<ND, KIND, MIXED, T, R>(m, f, grad, sl.gshift, cd, cf, dQL, sl.qshift,
                                                fstatus, cnt);
     });
@@ -137,8 +153,8 @@ __global__ __launch_bounds__(256) void drain_gather(const int64_t* __restrict__ 
                                                     const int32_t* __restrict__ twin,
                                                     const T* __restrict__ dQL,
                                                     double* __restrict__ dQ,
-                                                    Plan* __restrict__ p) {
-    drain<false>(p, 2, [&](const Slot& sl, int64_t i) {
+                                                    Plan* __restrict__ p, int grid) {
+    drain<false>(p, 2, grid, [&](const Slot& sl, int64_t i) {
         flux::gather_cell<ND, MIXED, T>(i, goff, gent, ghost, twin, dQL, sl.qshift, dQ);
     });
 }
@@ -150,9 +166,9 @@ __global__ __launch_bounds__(256) void drain_update(Consts cd, double* __restric
                                                     double* __restrict__ W,
                                                     uint8_t* __restrict__ cstatus,
                                                     unsigned long long* __restrict__ ctr,
-                                                    Plan* __restrict__ p) {
+                                                    Plan* __restrict__ p, int grid) {
     unsigned fail = 0;
-    drain<false>(p, 3, [&](const Slot&, int64_t i) {
+    drain<false>(p, 3, grid, [&](const Slot&, int64_t i) {
         fail += flux::update_cell<ND>(cd, i, Q, dQ, volume, W, cstatus);
     });
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
@@ -231,7 +247,8 @@ struct EnsArgs {
     unsigned long long* ctr;
     ens::Plan* plan;
     int block;    // threads per block = items per tile
-    int nblocks;  // persistent grid
+    int nblocks;  // persistent grid (0: one tile per block)
+    int64_t max_ctiles, max_ftiles;  // tile counts the arena allows (grid mode)
 };
 
 // ensemble_fp64.hip (T = double) and ensemble_mixed.hip (T = float)

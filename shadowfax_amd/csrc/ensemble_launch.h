@@ -11,21 +11,28 @@ namespace sfx {
 template <typename T, int MIXED>
 hipError_t ens_step_impl(int ndim, int kind, const EnsArgs<T>& a, hipStream_t s) {
     const mixed::CF cf = mixed::make_cf(a.c);
-    const dim3 g((unsigned)a.nblocks), b((unsigned)a.block);
+    const int grid = a.nblocks == 0;
+    // grid mode: a multiple of 8 blocks covering every XCD chunk of the
+    // largest tile count
+    auto cover = [](int64_t tiles) {
+        return (unsigned)(ens::kXcd * ((tiles + ens::kXcd - 1) / ens::kXcd + 1));
+    };
+    const dim3 gc(grid ? cover(a.max_ctiles) : (unsigned)a.nblocks),
+            gf(grid ? cover(a.max_ftiles) : (unsigned)a.nblocks), b((unsigned)a.block);
 #define SFX_ENS(ND)                                                                            \
     do {                                                                                       \
-        hipLaunchKernelGGL((ens::drain_gradients<ND, T, double, MIXED != 0>), g, b, 0, s, a.m,  \
-                           a.grad, a.plan);                                                     \
+        hipLaunchKernelGGL((ens::drain_gradients<ND, T, double, MIXED != 0>), gc, b, 0, s, a.m, \
+                           a.grad, a.plan, grid);                                               \
         if(kind == 0)                                                                          \
-            hipLaunchKernelGGL((ens::drain_faces<ND, 0, MIXED, T, double>), g, b, 0, s, a.m,    \
-                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan);               \
+            hipLaunchKernelGGL((ens::drain_faces<ND, 0, MIXED, T, double>), gf, b, 0, s, a.m,   \
+                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan, grid);         \
         else                                                                                   \
-            hipLaunchKernelGGL((ens::drain_faces<ND, 1, MIXED, T, double>), g, b, 0, s, a.m,    \
-                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan);               \
-        hipLaunchKernelGGL((ens::drain_gather<ND, MIXED, T>), g, b, 0, s, a.m.gather_off,          \
-                           a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, a.plan);         \
-        hipLaunchKernelGGL((ens::drain_update<ND>), g, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,     \
-                           a.W, a.cstatus, a.ctr, a.plan);                                      \
+            hipLaunchKernelGGL((ens::drain_faces<ND, 1, MIXED, T, double>), gf, b, 0, s, a.m,   \
+                               a.grad, a.c, cf, a.dQL, a.fstatus, a.ctr, a.plan, grid);         \
+        hipLaunchKernelGGL((ens::drain_gather<ND, MIXED, T>), gc, b, 0, s, a.m.gather_off,         \
+                           a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, a.plan, grid);   \
+        hipLaunchKernelGGL((ens::drain_update<ND>), gc, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,    \
+                           a.W, a.cstatus, a.ctr, a.plan, grid);                                \
     } while(0)
     if(ndim == 3)
         SFX_ENS(3);

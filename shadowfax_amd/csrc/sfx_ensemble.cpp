@@ -124,6 +124,9 @@ EnsArgs<T> ens_args(const sfx_ensemble* e) {
     a.plan = e->d_plan;
     a.block = e->block;
     a.nblocks = e->ncu * e->per_cu;
+    // a batch's tiles: ceil(items / block) per mesh, at most kMaxBatch meshes
+    a.max_ctiles = (e->arena_cells + e->block - 1) / e->block + ens::kMaxBatch;
+    a.max_ftiles = (e->arena_faces + e->block - 1) / e->block + ens::kMaxBatch;
     return a;
 }
 
@@ -503,7 +506,7 @@ int sfx_ensemble_read(sfx_ensemble* e, int32_t id, int what, double* host_out) {
 
 int sfx_ensemble_set_launch(sfx_ensemble* e, int block, int blocks_per_cu) {
     if(!e) return set_error(SFX_ERR_ARG, "sfx_ensemble_set_launch: NULL");
-    if(block < 64 || block > 256 || (block & 63) || blocks_per_cu < 1 || blocks_per_cu > 32)
+    if(block < 64 || block > 256 || (block & 63) || blocks_per_cu < 0 || blocks_per_cu > 32)
         return set_error(SFX_ERR_ARG, "sfx_ensemble_set_launch: block %d, blocks/CU %d", block,
                          blocks_per_cu);
     if(block != e->block || blocks_per_cu != e->per_cu) {
@@ -562,7 +565,7 @@ int sfx_ensemble_autotune(sfx_ensemble* e, const int32_t* ids, int32_t n, int re
     hipEventCreate(&t0);
     hipEventCreate(&t1);
     const int blocks[3] = {64, 128, 256};
-    const int per_cu[4] = {1, 2, 4, 8};
+    const int per_cu[5] = {0, 1, 2, 4, 8};  // 0: one block per tile
     double best = 1e300;
     int bb = e->block, bp = e->per_cu;
     e->h_batch[0] = n;

@@ -55,12 +55,14 @@ def check_meshes(ens, bs, kind, prec, dev, counts):
         assert np.abs(ens.read(mid, S.ENS_DQ)).max() == 0.0
 
 
+@pytest.mark.parametrize("per_cu", [4, 0])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("kind", [0, 1])
-def test_ensemble_steps_match_standalone(device, kind, prec):
-    print("case kind %d %s" % (kind, prec), flush=True)
+def test_ensemble_steps_match_standalone(device, kind, prec, per_cu):
+    print("case kind %d %s per_cu %d" % (kind, prec, per_cu), flush=True)
     bs = bundles(kind)
     ens, ids = make_ensemble(kind, prec, device, bs)
+    ens.set_launch(128, per_cu)  # persistent grid / one block per tile
     ens.step([0, 2, 3])
     ens.sync()
     ens.step([3, 1])          # a second replay of the graph, another batch
@@ -71,12 +73,14 @@ def test_ensemble_steps_match_standalone(device, kind, prec):
     check_meshes(ens, bs, kind, prec, device, [3, 2, 2, 3])
 
 
+@pytest.mark.parametrize("per_cu", [4, 0])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_serve_trace_applies_every_request_once(device, prec):
+def test_serve_trace_applies_every_request_once(device, prec, per_cu):
     kind = 1
     bs = bundles(kind)
     # an arena that holds only some meshes at a time, so batches split
     ens, ids = make_ensemble(kind, prec, device, bs, arena_cells=500)
+    ens.set_launch(64, per_cu)
     t = np.linspace(0.0, 2e-3, 24)
     mesh = np.array([0, 1, 0, 2, 3, 3, 1, 0, 2, 2, 1, 3, 0, 1, 2, 3, 0, 0, 1, 2, 3, 1, 1, 0],
                     np.int32)
@@ -101,7 +105,7 @@ def test_autotune_keeps_states(device):
     before = [ens.read(k, S.ENS_W) for k in range(4)]
     block, per_cu, ms = ens.autotune([0, 1, 2, 3], reps=2)
     print("autotune", block, per_cu, ms, flush=True)
-    assert block in (64, 128, 256) and per_cu in (1, 2, 4, 8) and ms > 0
+    assert block in (64, 128, 256) and per_cu in (0, 1, 2, 4, 8) and ms > 0
     for k in range(4):
         assert np.array_equal(ens.read(k, S.ENS_W), before[k])
     ens.step([0, 1, 2, 3])
