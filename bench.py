@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD per 4 cycles
+# (the FP64/FP32 FMA rate), 2.4 GHz peak engine clock
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 FP32_PEAK_TF = 157.3
 FP64_PEAK_TF = 78.6
 METRIC = "face-fluxes/sec at ≤1e-6 rel err vs FP64 plain GPU; p50/p99 served-step latency"
@@ -209,6 +212,17 @@ def measured_traffic(kernel, grid):
     return float(k["traffic_bytes"])
 
 
+def measured_valu(kernel):
+    """VALU wave-instructions per launch of a kernel from the committed
+    hardware-counter summary (profiles/valu_counters.json), if measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "valu_counters.json")) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return float(k["SQ_INSTS_VALU"]) if k else None
+
+
 def host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
@@ -381,7 +395,13 @@ def main():
             "kernel": kernels[0] + " (one step: the batched mixed exact solve)",
             "bytes_per_problem": bytes_per,
             "note": "exact solver is VALU-bound (Newton on FP32 transcendentals + one FP64 "
-                    "step); HBM fraction shown as the contract asks"}
+                    "step); HBM fraction shown as the contract asks, VALU issue below"}
+    vi = measured_valu(kernels[0])
+    if vi is not None:
+        va = vi / (ms * 1e-3)
+        roof["valu"] = {"insts_per_launch": vi, "achieved": va, "peak": VALU_PEAK_WAVE_INSTS,
+                        "unit": "wave64 VALU instructions/s", "frac": va / VALU_PEAK_WAVE_INSTS,
+                        "source": "profiles/valu_counters.md (rocprofv3 SQ_INSTS_VALU)"}
     out = {"metric": METRIC, "value": value, "unit": "face-fluxes/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
