@@ -34,6 +34,17 @@ namespace sfx {
 namespace mixed {
 
 // FP32 consts
+// The FP32 iteration stops once a Newton step moves p by at most this
+// fraction of (p_new + p_old); the FP64 step that follows squares the
+// remaining error.  Measured on the 4M-problem mixed batch and the vacuum,
+// shock and mixed families: the error against FP64 is the same (5.96e-8,
+// the FP32 output rounding) for 2e-6, 1e-5, 1e-4 and 1e-3; 1e-4 saves ~8%
+// of the solve and leaves margin where Newton converges only linearly
+// (p* near 0).
+#ifndef SFX_MIXED_STEP_TOL
+#define SFX_MIXED_STEP_TOL 1e-4f
+#endif
+
 struct CF {
     float gamma, gp1d2g, gm1d2g, gm1dgp1, tdgp1, tdgm1, gm1d2, tgdgm1, ginv, ginvm1;
     float gm1inv;      // 1/(gamma-1)
@@ -319,7 +330,7 @@ __device__ __forceinline__ void exact_iterate(const CF& c, ExactCase& s, int sto
             pn = (hi < __builtin_inff()) ? ((lo > 0.f) ? sqrtf_fast(lo) * sqrtf_fast(hi)
                                                        : 0.01f * hi)
                                          : 2.f * p;
-        const bool done = fabsf(pn - p) <= 2e-6f * (pn + p);
+        const bool done = fabsf(pn - p) <= SFX_MIXED_STEP_TOL * (pn + p);
         p = pn;
         if(done) {
             phase = EX_DONE;
