@@ -448,6 +448,11 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     }
     const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
     constexpr int KB = (ND == 2) ? SFX_GRAD_KB_2D : SFX_GRAD_KB;
+    // 2D: a cell's entries mostly fit one chunk; their second halves are
+    // then kept for the limiter loop instead of being read again
+    constexpr bool KEEP = ND == 2;
+    const bool one_chunk = KEEP && j1 - j0 <= KB;
+    float4 kept[KEEP ? KB : 1];
     for(int64_t jb = j1; jb > j0; jb -= KB) {
         float4 lo[KB], hi[KB];
         double Wj[KB][NC];
@@ -458,6 +463,10 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             lo[u] = r[0];
             hi[u] = r[1];
             if(j < j0) hi[u].w = hi[u].z = 0.f;  // padding: A = 0 (A sits at .z or .w)
+        }
+        if constexpr(KEEP) {
+#pragma unroll
+            for(int u = 0; u < KB; u++) kept[u] = hi[u];
         }
 #pragma unroll
         for(int u = 0; u < KB; u++) {
@@ -507,6 +516,13 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 #pragma unroll
         for(int u = 0; u < KL; u++) {
             const int64_t j = jb - 1 - u;
+            if constexpr(KEEP) {
+                // KL == KB in 2D: the first chunk's halves are still held
+                if(one_chunk) {
+                    hi[u] = kept[u];
+                    continue;
+                }
+            }
             hi[u] = reinterpret_cast<const float4*>(m.egeo + ((j >= j0) ? j : j0) * ER)[1];
             if(j < j0) hi[u].w = hi[u].z = 0.f;
         }
