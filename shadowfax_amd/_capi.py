@@ -37,7 +37,7 @@ class MeshDesc(ct.Structure):
                 ("left", vp), ("right", vp), ("ghost", vp), ("twin", vp),
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
                 ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
-                ("gather_entry", vp)]
+                ("gather_entry", vp), ("egeo_group", vp)]
 
 
 _lib = None
@@ -92,6 +92,7 @@ def lib():
     for name, args in _mesh_sigs().items():
         if hasattr(L, name):
             getattr(L, name).argtypes = args
+    L.sfx_mesh_entry_groups.restype = ct.c_int64
     _lib = L
     return L
 
@@ -110,6 +111,7 @@ def _mesh_sigs():
     return {
         "sfx_mesh_entry_geometry": [vp, md, i64, vp, vp],
         "sfx_mesh_entry_record": [ct.c_int, ct.c_int],
+        "sfx_mesh_entry_groups": [i64, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)],
         "sfx_gradients": [vp, md, vp, vp],
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],

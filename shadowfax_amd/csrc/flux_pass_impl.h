@@ -80,6 +80,9 @@ struct MeshView {
     const T* egeo;   // [cf_off[C]][entry_rec<ND, T>()]
     const int64_t* gather_off;
     const int32_t* gather_entry;
+    // FP32 records grouped by 64 consecutive cells (see grad_cell_f32), the
+    // first record row of every group; null: records in entry order
+    const int64_t* egrp = nullptr;
 };
 
 // Per-entry record of a cell's face, in the cell's walk order.  FP64
@@ -368,6 +371,35 @@ __device__ __forceinline__ void predict_side_offset(float gamma, const double* W
 
 // One lane per cell entry: the record of entry j (face f = e >> 1 seen from
 // side e & 1) from the face arrays; run once per tessellation.
+// The FP32 record of cell entry e = 2 face + side (entry_rec(ND, true)).
+template <int ND>
+__device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32_t e,
+                                                 float* rec) {
+    const int64_t F = m.F;
+    const int32_t f = ent_face(e);
+    const bool isR = ent_right(e);
+    // g = A (c - r_ij/2)/|r_ij| in FP64
+    double r[ND], r2 = 0.0;
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        const double rr = (double)m.rRL[l * F + f];
+        r[l] = isR ? rr : -rr;
+        r2 += r[l] * r[l];
+    }
+    const double s = (double)m.area[f] / sqrt(r2);
+    using RL = Rec32<ND>;
+#pragma unroll
+    for(int k = 0; k < 8; k++) rec[k] = 0.f;
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        rec[RL::G + l] = (float)(s * ((double)m.mc[l * F + f] - 0.5 * r[l]));
+        rec[RL::D + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+    }
+    const int32_t nb = isR ? m.left[f] : m.right[f];
+    rec[RL::NB] = index_bits(nb >= 0 ? nb : ~f, 0.f);
+    rec[RL::A] = m.area[f];
+}
+
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int64_t E,
                                                              T* __restrict__ egeo) {
@@ -379,27 +411,8 @@ __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int6
     const int32_t f = ent_face(e);
     const bool isR = ent_right(e);
     T rec[ER];
-    if(sizeof(T) == 4) {
-        // g = A (c - r_ij/2)/|r_ij| in FP64
-        double r[ND], r2 = 0.0;
-#pragma unroll
-        for(int l = 0; l < ND; l++) {
-            const double rr = (double)m.rRL[l * F + f];
-            r[l] = isR ? rr : -rr;
-            r2 += r[l] * r[l];
-        }
-        const double s = (double)m.area[f] / sqrt(r2);
-        using RL = Rec32<ND>;
-#pragma unroll
-        for(int k = 0; k < ER; k++) rec[k] = T(0);
-#pragma unroll
-        for(int l = 0; l < ND; l++) {
-            rec[RL::G + l] = (T)(s * ((double)m.mc[l * F + f] - 0.5 * r[l]));
-            rec[RL::D + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
-        }
-        const int32_t nb = isR ? m.left[f] : m.right[f];
-        rec[RL::NB] = index_bits(nb >= 0 ? nb : ~f, T(0));
-        rec[RL::A] = m.area[f];
+    if constexpr(sizeof(T) == 4) {
+        entry_record_f32<ND>(m, e, rec);
     } else {
         rec[0] = m.area[f];
 #pragma unroll
@@ -414,6 +427,39 @@ __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int6
     }
 #pragma unroll
     for(int k = 0; k < ER; k++) egeo[j * ER + k] = rec[k];
+}
+
+// The grouped FP32 layout: cells in groups of 64 (cell i: group i / 64,
+// lane i % 64); group g owns record rows egrp[g] .. egrp[g + 1] - 1, as
+// many as its cell with the most entries; row r holds the k-th entry
+// (k = r - egrp[g], walk order) of all 64 cells, lane beside lane, as two
+// planes of float4 (halves) -- float4 index (2 r + half) * 64 + lane.
+// Cells with fewer entries (and lanes past the last cell) get zero records
+// (area 0: skipped).  A wavefront of 64 consecutive cells then reads row
+// after row with coalesced 1 KB loads where the entry-ordered layout reads
+// 64 scattered records.  One lane per cell writes its column.
+template <int ND>
+__global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<float> m,
+                                                                     float* __restrict__ egeo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g = i >> 6;
+    const int lane = (int)(i & 63);
+    if(g >= (m.C + 63) / 64) return;
+    const int64_t r0 = m.egrp[g], K = m.egrp[g + 1] - r0;
+    const int64_t n = (i < m.C) ? m.cf_off[i + 1] - m.cf_off[i] : 0;
+    float4* out = reinterpret_cast<float4*>(egeo);
+    for(int64_t k = 0; k < K; k++) {
+        float rec[8];
+        if(k < n) {
+            entry_record_f32<ND>(m, m.cf_entry[m.cf_off[i] + k], rec);
+        } else {
+#pragma unroll
+            for(int q = 0; q < 8; q++) rec[q] = 0.f;
+        }
+        const int64_t r = r0 + k;
+        out[(2 * r) * 64 + lane] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+        out[(2 * r + 1) * 64 + lane] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+    }
 }
 
 // DIFF selects the form of the Green-Gauss sum.  The reference's
@@ -432,11 +478,10 @@ __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int6
 // a = Wmax - W, b = dp for dp > 0 and a = W - Wmin, b = -dp for dp < 0
 // (dp = 0 gives 1, which never lowers alpha <= 1); the smallest a/b is kept
 // as a pair, compared by cross products, and divided out once per component.
-template <int ND>
+template <int ND, bool GROUPED>
 __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t i,
                                               float* __restrict__ grad, int64_t gshift) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, float>();
-    using RL = Rec32<ND>;
     if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
     double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
     load_row<NC>(m.W, i, m.C, W);
@@ -446,23 +491,43 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 #pragma unroll
         for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
     }
-    const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
+    // the cell's records, visited in reverse walk order (src/VorCell.cpp:471)
+    // as positions n - 1 .. 0: entry j0 + p (entry order), or row r0 + p of
+    // the cell's group, lane i % 64 (grouped; rows past the cell's own
+    // entries hold area 0 and are skipped, so every lane of a group walks
+    // the same rows)
+    int64_t j0, n;
+    const float4* E4 = reinterpret_cast<const float4*>(m.egeo);
+    if constexpr(GROUPED) {
+        const int64_t g = i >> 6;
+        j0 = m.egrp[g];
+        n = m.egrp[g + 1] - j0;
+    } else {
+        j0 = m.cf_off[i];
+        n = m.cf_off[i + 1] - j0;
+    }
+    const int lane = (int)(i & 63);
+    auto half = [&](int64_t p, int h) -> float4 {
+        if constexpr(GROUPED)
+            return E4[(2 * (j0 + p) + h) * 64 + lane];
+        else
+            return E4[(j0 + p) * (ER / 4) + h];
+    };
     constexpr int KB = (ND == 2) ? SFX_GRAD_KB_2D : SFX_GRAD_KB;
     // 2D: a cell's entries mostly fit one chunk; their second halves are
     // then kept for the limiter loop instead of being read again
     constexpr bool KEEP = ND == 2;
-    const bool one_chunk = KEEP && j1 - j0 <= KB;
+    const bool one_chunk = KEEP && n <= KB;
     float4 kept[KEEP ? KB : 1];
-    for(int64_t jb = j1; jb > j0; jb -= KB) {
+    for(int64_t pb = n; pb > 0; pb -= KB) {
         float4 lo[KB], hi[KB];
         double Wj[KB][NC];
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            const int64_t j = jb - 1 - u;
-            const float4* r = reinterpret_cast<const float4*>(m.egeo + ((j >= j0) ? j : j0) * ER);
-            lo[u] = r[0];
-            hi[u] = r[1];
-            if(j < j0) hi[u].w = hi[u].z = 0.f;  // padding: A = 0 (A sits at .z or .w)
+            const int64_t p = pb - 1 - u;
+            lo[u] = half(p >= 0 ? p : 0, 0);
+            hi[u] = half(p >= 0 ? p : 0, 1);
+            if(p < 0) hi[u].w = hi[u].z = 0.f;  // padding: A = 0 (A sits at .z or .w)
         }
         if constexpr(KEEP) {
 #pragma unroll
@@ -489,7 +554,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 #pragma unroll
         for(int u = 0; u < KB; u++) {
             const float A = (ND == 3) ? hi[u].w : hi[u].z;
-            if(!(A != 0.f)) continue;  // also the padding past j0
+            if(!(A != 0.f)) continue;  // also the padding
             const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
                                   (ND == 3) ? (double)lo[u].z : 0.0};
 #pragma unroll
@@ -511,11 +576,11 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 #pragma unroll
     for(int k = 0; k < NC; k++) na[k] = nb[k] = 1.0;
     constexpr int KL = (ND == 2) ? SFX_GRAD_KB_2D : 4;
-    for(int64_t jb = j1; jb > j0; jb -= KL) {
+    for(int64_t pb = n; pb > 0; pb -= KL) {
         float4 hi[KL];
 #pragma unroll
         for(int u = 0; u < KL; u++) {
-            const int64_t j = jb - 1 - u;
+            const int64_t p = pb - 1 - u;
             if constexpr(KEEP) {
                 // KL == KB in 2D: the first chunk's halves are still held
                 if(one_chunk) {
@@ -523,8 +588,8 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
                     continue;
                 }
             }
-            hi[u] = reinterpret_cast<const float4*>(m.egeo + ((j >= j0) ? j : j0) * ER)[1];
-            if(j < j0) hi[u].w = hi[u].z = 0.f;
+            hi[u] = half(p >= 0 ? p : 0, 1);
+            if(p < 0) hi[u].w = hi[u].z = 0.f;
         }
 #pragma unroll
         for(int u = 0; u < KL; u++) {
@@ -561,7 +626,10 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
                                           int64_t gshift) {
     if constexpr(sizeof(T) == 4) {
         static_assert(DIFF, "FP32 records carry the difference-form weights");
-        grad_cell_f32<ND>(m, i, grad, gshift);
+        if(m.egrp)
+            grad_cell_f32<ND, true>(m, i, grad, gshift);
+        else
+            grad_cell_f32<ND, false>(m, i, grad, gshift);
         return;
     }
     const int NC = ND + 2, ER = entry_rec(ND, false);

@@ -56,6 +56,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.cf_off = m->cf_off;
     v.cf_entry = m->cf_entry;
     v.egeo = (const T*)m->egeo;
+    v.egrp = (sizeof(T) == 4) ? m->egeo_group : nullptr;
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
     return v;
@@ -74,6 +75,10 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
         return set_error(SFX_ERR_ARG, "%s: W must be 16-byte aligned", who);
     if(m->egeo && ((uintptr_t)m->egeo & 15))
         return set_error(SFX_ERR_ARG, "%s: egeo must be 16-byte aligned", who);
+    if(m->egeo_group && h->precision != SFX_PREC_MIXED)
+        return set_error(SFX_ERR_ARG, "%s: egeo_group is for the mixed records only", who);
+    if(m->egeo_group && !m->cf_off)
+        return set_error(SFX_ERR_ARG, "%s: egeo_group needs cf_off", who);
     if(gradients && (!m->W || !m->volume || !m->cf_off || !m->egeo))
         return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass (W, volume, "
                          "cf_off, egeo)", who);
@@ -218,6 +223,24 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left, const int32
         if(right[f] >= 0 && !(ghost && ghost[f])) gather_entry[fill[right[f]]++] = (int32_t)(2 * f + 1);
     }
     return SFX_OK;
+}
+
+int64_t sfx_mesh_entry_groups(int64_t C, const int64_t* cf_off, int64_t* egeo_group) {
+    if(C < 0 || !cf_off || !egeo_group) {
+        set_error(SFX_ERR_ARG, "sfx_mesh_entry_groups: bad arguments");
+        return -1;
+    }
+    const int64_t ng = (C + 63) / 64;
+    int64_t rows = 0;
+    for(int64_t g = 0; g < ng; g++) {
+        egeo_group[g] = rows;
+        int64_t k = 0;
+        for(int64_t i = 64 * g; i < std::min(C, 64 * g + 64); i++)
+            k = std::max(k, cf_off[i + 1] - cf_off[i]);
+        rows += k;
+    }
+    egeo_group[ng] = rows;
+    return rows;
 }
 
 int sfx_mesh_entry_record(int ndim, int precision) {

@@ -289,7 +289,9 @@ class DeviceMesh:
     *_soa() accessors return host copies in the component-major layout of
     the prepared bundle (and of the oracle)."""
 
-    def __init__(self, b, storage="fp64", device=None):
+    def __init__(self, b, storage="fp64", device=None, grouped=None):
+        """grouped (mixed storage; default on): the gradient records in the
+        grouped layout of sfx_mesh_entry_groups."""
         import torch
         self.torch = torch
         self.dev = device or torch.device("cuda:0")
@@ -317,7 +319,21 @@ class DeviceMesh:
         self.G = int(b["gather_off"][-1])
         ER = Cp.lib().sfx_mesh_entry_record(
             nd, Cp.SFX_PREC_FP64 if storage == "fp64" else Cp.SFX_PREC_MIXED)
-        self.egeo = torch.zeros((self.E, ER), dtype=tdt, device=self.dev)
+        self.grouped = (storage == "mixed") if grouped is None else bool(grouped)
+        if self.grouped and storage != "mixed":
+            raise ValueError("the grouped record layout is for mixed storage")
+        if self.grouped:
+            off = np.ascontiguousarray(b["cf_off"], np.int64)
+            grp = np.zeros((self.C + 63) // 64 + 1, np.int64)
+            rows = Cp.lib().sfx_mesh_entry_groups(
+                self.C, off.ctypes.data_as(ct.POINTER(ct.c_int64)),
+                grp.ctypes.data_as(ct.POINTER(ct.c_int64)))
+            if rows < 0:
+                raise Cp.SfxError(Cp.lib().sfx_last_error().decode())
+            self.arr["egeo_group"] = t(grp, torch.int64)
+            self.egeo = torch.zeros((rows * 64, ER), dtype=tdt, device=self.dev)
+        else:
+            self.egeo = torch.zeros((self.E, ER), dtype=tdt, device=self.dev)
         self.egeo_built = False
         self.grad = torch.zeros((self.C, nd, NC), dtype=tdt, device=self.dev)
         self.dQL = torch.zeros((self.F, NC), dtype=tdt, device=self.dev)
@@ -340,6 +356,7 @@ class DeviceMesh:
         d.cf_entry = p("cf_entry")
         d.egeo = ct.c_void_p(self.egeo.data_ptr())
         d.gather_entry = p("gather_entry")
+        d.egeo_group = p("egeo_group") if self.grouped else None
         d.twin = p("twin") if self.use_twin else None
         return d
 

@@ -132,3 +132,17 @@ def test_hilbert_order_is_a_hilbert_curve(nd, n):
     assert np.array_equal(np.sort(o), np.arange(20000))
     d = np.linalg.norm(np.diff(p[:, o], axis=1), axis=0)
     assert np.median(d) < 2.0 * 20000 ** (-1.0 / nd)
+
+
+def test_entry_groups_by_hand():
+    """sfx_mesh_entry_groups: groups of 64 consecutive cells, each as many
+    record rows as its cell with the most entries; the last group short."""
+    import ctypes as ct
+    from shadowfax_amd import _capi as C
+    counts = np.array([3] * 64 + [5] + [2] * 63 + [7, 1], np.int64)  # 130 cells
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    grp = np.zeros(4, np.int64)
+    p = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_int64))
+    rows = C.lib().sfx_mesh_entry_groups(130, p(off), p(grp))
+    assert rows == 3 + 5 + 7 and list(grp) == [0, 3, 8, 15]
+    assert C.lib().sfx_mesh_entry_groups(-1, p(off), p(grp)) == -1
