@@ -331,10 +331,10 @@ typedef struct {
                                   (sfx_mesh_entry_geometry) */
     const int64_t* gather_off; /* [C+1] */
     const int32_t* gather_entry; /* sfx_mesh_gather_index */
-    const int64_t* egeo_group; /* mixed precision only, may be NULL: the
-                                  grouped record layout (sfx_mesh_entry_groups)
-                                  -- egeo is then float [rows][2][64][4], rows
-                                  = egeo_group[ceil(C/64)] */
+    const int64_t* egeo_group; /* may be NULL: the grouped record layout
+                                  (sfx_mesh_entry_groups) -- egeo then holds
+                                  rows * 64 records, rows =
+                                  egeo_group[ceil(C/64)] */
 } sfx_mesh;
 
 /* Per-entry face records for the gradient pass, built once per
@@ -357,15 +357,17 @@ int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* e
  * -1 for other arguments. */
 int sfx_mesh_entry_record(int ndim, int precision);
 
-/* The grouped layout of the mixed records (host): cells in groups of 64
- * consecutive cells; group g gets record rows egeo_group[g] ..
+/* The grouped layout of the per-entry records (host): cells in groups of
+ * 64 consecutive cells; group g gets record rows egeo_group[g] ..
  * egeo_group[g+1]-1, one per entry of its cell with the most entries, and
  * row r holds every cell's entry r - egeo_group[g] (walk order) side by
- * side, zero records padding the shorter cells.  A wavefront of 64
- * consecutive cells then loads each row with coalesced 1 KB reads.
- * egeo_group: [ceil(C/64) + 1]; returns rows = egeo_group[ceil(C/64)]
- * (egeo then holds rows * 2 * 64 float4), or -1 on bad arguments.
- * sfx_mesh_entry_geometry fills it when sfx_mesh.egeo_group is set. */
+ * side as 16-byte planes of the record (plane q of row r, lane l at
+ * 16-byte index (r P + q) 64 + l, P = record bytes / 16), zero records
+ * padding the shorter cells.  A wavefront of 64 consecutive cells then
+ * loads each row with coalesced 1 KB reads.  egeo_group: [ceil(C/64) + 1];
+ * returns rows = egeo_group[ceil(C/64)] (egeo then holds rows * 64
+ * records), or -1 on bad arguments.  sfx_mesh_entry_geometry fills that
+ * layout when sfx_mesh.egeo_group is set. */
 int64_t sfx_mesh_entry_groups(int64_t C, const int64_t* cf_off, int64_t* egeo_group);
 
 /* VorCell::estimate_gradient for every active cell (src/VorCell.cpp:464-530,

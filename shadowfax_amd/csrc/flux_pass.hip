@@ -11,6 +11,18 @@ static inline dim3 grid_for(int64_t n, int bs) { return dim3((unsigned)((n + bs 
 template <>
 hipError_t launch_entry_geometry<double>(int ndim, const flux::MeshView<double>& m, int64_t E,
                                      double* egeo, hipStream_t s) {
+    if(m.egrp) {
+        // grouped layout: one lane per cell, the last group padded to 64
+        if(m.C <= 0) return hipSuccess;
+        const dim3 g((unsigned)(((m.C + 63) / 64 * 64 + 255) / 256));
+        if(ndim == 3)
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, double>), g, dim3(256), 0,
+                               s, m, egeo);
+        else
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, double>), g, dim3(256), 0,
+                               s, m, egeo);
+        return hipGetLastError();
+    }
     if(E <= 0) return hipSuccess;
     if(ndim == 3)
         hipLaunchKernelGGL((flux::entry_geometry_kernel<3, double>), grid_for(E, 256), dim3(256), 0,

@@ -400,47 +400,84 @@ __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32
     rec[RL::A] = m.area[f];
 }
 
+// The FP64 record of cell entry e (entry_rec(ND, false)).
+template <int ND>
+__device__ __forceinline__ void entry_record_f64(const MeshView<double>& m, int32_t e,
+                                                 double* rec) {
+    constexpr int ER = entry_rec(ND, false);
+    const int64_t F = m.F;
+    const int32_t f = ent_face(e);
+    const bool isR = ent_right(e);
+    rec[0] = m.area[f];
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        const double rr = m.rRL[l * F + f];
+        rec[1 + l] = m.mc[l * F + f];
+        rec[1 + ND + l] = isR ? rr : -rr;
+        rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+    }
+    rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], 0.0);
+    if(ER > 2 + 3 * ND) rec[ER - 1] = 0.0;
+}
+
+template <int ND, typename T>
+__device__ __forceinline__ void entry_record(const MeshView<T>& m, int32_t e, T* rec) {
+    if constexpr(sizeof(T) == 4)
+        entry_record_f32<ND>(m, e, rec);
+    else
+        entry_record_f64<ND>(m, e, rec);
+}
+
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void entry_geometry_kernel(MeshView<T> m, int64_t E,
                                                              T* __restrict__ egeo) {
     constexpr int ER = entry_rec<ND, T>();
     const int64_t j = xcd_block() * blockDim.x + threadIdx.x;
     if(j >= E) return;
-    const int64_t F = m.F;
-    const int32_t e = m.cf_entry[j];
-    const int32_t f = ent_face(e);
-    const bool isR = ent_right(e);
     T rec[ER];
-    if constexpr(sizeof(T) == 4) {
-        entry_record_f32<ND>(m, e, rec);
-    } else {
-        rec[0] = m.area[f];
-#pragma unroll
-        for(int l = 0; l < ND; l++) {
-            const T rr = m.rRL[l * F + f];
-            rec[1 + l] = m.mc[l * F + f];
-            rec[1 + ND + l] = isR ? rr : -rr;
-            rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
-        }
-        rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], T(0));
-        if(ER > 2 + 3 * ND) rec[ER - 1] = T(0);
-    }
+    entry_record<ND, T>(m, m.cf_entry[j], rec);
 #pragma unroll
     for(int k = 0; k < ER; k++) egeo[j * ER + k] = rec[k];
 }
 
-// The grouped FP32 layout: cells in groups of 64 (cell i: group i / 64,
+// The grouped record layout: cells in groups of 64 (cell i: group i / 64,
 // lane i % 64); group g owns record rows egrp[g] .. egrp[g + 1] - 1, as
 // many as its cell with the most entries; row r holds the k-th entry
-// (k = r - egrp[g], walk order) of all 64 cells, lane beside lane, as two
-// planes of float4 (halves) -- float4 index (2 r + half) * 64 + lane.
-// Cells with fewer entries (and lanes past the last cell) get zero records
-// (area 0: skipped).  A wavefront of 64 consecutive cells then reads row
-// after row with coalesced 1 KB loads where the entry-ordered layout reads
-// 64 scattered records.  One lane per cell writes its column.
-template <int ND>
-__global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<float> m,
-                                                                     float* __restrict__ egeo) {
+// (k = r - egrp[g], walk order) of all 64 cells, lane beside lane, as
+// P 16-byte planes of the record (2 for FP32, 4 / 6 for FP64 in 2D / 3D):
+// plane q of (row r, lane) at 16-byte index (r P + q) 64 + lane.  Cells
+// with fewer entries (and lanes past the last cell) get zero records (area
+// 0: skipped).  A wavefront of 64 consecutive cells then reads row after
+// row with coalesced 1 KB loads where the entry-ordered layout reads 64
+// scattered records.
+template <int ND, typename T>
+__host__ __device__ constexpr int rec_planes() {
+    return entry_rec<ND, T>() * (int)sizeof(T) / 16;
+}
+
+// Record at position p of a cell: entry base + p (entry order) or row
+// base + p, lane `lane` (grouped).
+template <int ND, typename T, bool GROUPED>
+__device__ __forceinline__ void load_record(const MeshView<T>& m, int64_t base, int64_t p,
+                                            int lane, T* out) {
+    constexpr int ER = entry_rec<ND, T>();
+    if constexpr(GROUPED) {
+        constexpr int P = rec_planes<ND, T>();
+        const float4* E = reinterpret_cast<const float4*>(m.egeo);
+#pragma unroll
+        for(int q = 0; q < P; q++)
+            reinterpret_cast<float4*>(out)[q] = E[((base + p) * P + q) * 64 + lane];
+    } else {
+#pragma unroll
+        for(int q = 0; q < ER; q++) out[q] = m.egeo[(base + p) * ER + q];
+    }
+}
+
+// One lane per cell writes its column of its group's rows.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T> m,
+                                                                     T* __restrict__ egeo) {
+    constexpr int ER = entry_rec<ND, T>(), P = rec_planes<ND, T>();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t g = i >> 6;
     const int lane = (int)(i & 63);
@@ -449,16 +486,16 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<fl
     const int64_t n = (i < m.C) ? m.cf_off[i + 1] - m.cf_off[i] : 0;
     float4* out = reinterpret_cast<float4*>(egeo);
     for(int64_t k = 0; k < K; k++) {
-        float rec[8];
+        alignas(16) T rec[ER];
         if(k < n) {
-            entry_record_f32<ND>(m, m.cf_entry[m.cf_off[i] + k], rec);
+            entry_record<ND, T>(m, m.cf_entry[m.cf_off[i] + k], rec);
         } else {
 #pragma unroll
-            for(int q = 0; q < 8; q++) rec[q] = 0.f;
+            for(int q = 0; q < ER; q++) rec[q] = T(0);
         }
-        const int64_t r = r0 + k;
-        out[(2 * r) * 64 + lane] = make_float4(rec[0], rec[1], rec[2], rec[3]);
-        out[(2 * r + 1) * 64 + lane] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+#pragma unroll
+        for(int q = 0; q < P; q++)
+            out[((r0 + k) * P + q) * 64 + lane] = reinterpret_cast<const float4*>(rec)[q];
     }
 }
 
@@ -621,18 +658,10 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 
 // Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
 // for a whole mesh, the arena offset of the cell's mesh in an ensemble).
-template <int ND, typename T, typename R, bool DIFF>
-__device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __restrict__ grad,
-                                          int64_t gshift) {
-    if constexpr(sizeof(T) == 4) {
-        static_assert(DIFF, "FP32 records carry the difference-form weights");
-        if(m.egrp)
-            grad_cell_f32<ND, true>(m, i, grad, gshift);
-        else
-            grad_cell_f32<ND, false>(m, i, grad, gshift);
-        return;
-    }
-    const int NC = ND + 2, ER = entry_rec(ND, false);
+template <int ND, typename T, typename R, bool DIFF, bool GROUPED>
+__device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
+                                              T* __restrict__ grad, int64_t gshift) {
+    constexpr int NC = ND + 2, ER = entry_rec<ND, T>();
     if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
     R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
 #pragma unroll
@@ -641,19 +670,31 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
 #pragma unroll
         for(int l = 0; l < ND; l++) delta[l][k] = R(0);
     }
-    const int64_t j0 = m.cf_off[i], j1 = m.cf_off[i + 1];
-    // faces in reverse (src/VorCell.cpp:471), KB at a time: the records and
-    // neighbour states of a chunk are loaded before any is used, so a lane
-    // has KB independent loads in flight instead of one dependent chain
+    // the cell's records as positions n - 1 .. 0 (reverse walk order,
+    // src/VorCell.cpp:471): entries base + p, or rows base + p of the
+    // cell's group (grouped; rows past the cell's own entries have area 0)
+    int64_t base, n;
+    if constexpr(GROUPED) {
+        base = m.egrp[i >> 6];
+        n = m.egrp[(i >> 6) + 1] - base;
+    } else {
+        base = m.cf_off[i];
+        n = m.cf_off[i + 1] - base;
+    }
+    const int lane = (int)(i & 63);
+    // KB records at a time: the records and neighbour states of a chunk are
+    // loaded before any is used, so a lane has KB independent loads in
+    // flight instead of one dependent chain
     constexpr int KB = SFX_GRAD_KB;
-    for(int64_t jb = j1; jb > j0; jb -= KB) {
+    for(int64_t pb = n; pb > 0; pb -= KB) {
         R A[KB], c[KB][ND], rij[KB][ND], Wj[KB][NC];
         int32_t o[KB];
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            const int64_t j = jb - 1 - u;
-            const bool ok = j >= j0;
-            const T* rec = m.egeo + (ok ? j : j0) * ER;
+            const int64_t p = pb - 1 - u;
+            const bool ok = p >= 0;
+            alignas(16) T rec[ER];
+            load_record<ND, T, GROUPED>(m, base, ok ? p : 0, lane, rec);
             A[u] = ok ? (R)rec[0] : R(0);
 #pragma unroll
             for(int l = 0; l < ND; l++) {
@@ -680,7 +721,7 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
         }
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            if(!(A[u] != R(0))) continue;  // also the padding past j0
+            if(!(A[u] != R(0))) continue;  // also the padding
             const R rnorm =
                     (ND == 3) ? sqrt(rij[u][0] * rij[u][0] + rij[u][1] * rij[u][1] +
                                      rij[u][ND - 1] * rij[u][ND - 1])
@@ -713,8 +754,9 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
     R alpha[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) alpha[k] = R(1);
-    for(int64_t j = j1; j-- > j0;) {
-        const T* rec = m.egeo + j * ER;
+    for(int64_t p = n; p-- > 0;) {
+        alignas(16) T rec[ER];
+        load_record<ND, T, GROUPED>(m, base, p, lane, rec);
         if(!((R)rec[0] != R(0))) continue;
         R dd[ND];
 #pragma unroll
@@ -733,6 +775,23 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
 #pragma unroll
         for(int k = 0; k < NC; k++)
             grad[((i + gshift) * ND + l) * NC + k] = (T)(delta[l][k] * alpha[k]);
+}
+
+template <int ND, typename T, typename R, bool DIFF>
+__device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __restrict__ grad,
+                                          int64_t gshift) {
+    if constexpr(sizeof(T) == 4) {
+        static_assert(DIFF, "FP32 records carry the difference-form weights");
+        if(m.egrp)
+            grad_cell_f32<ND, true>(m, i, grad, gshift);
+        else
+            grad_cell_f32<ND, false>(m, i, grad, gshift);
+    } else {
+        if(m.egrp)
+            grad_cell_f64<ND, T, R, DIFF, true>(m, i, grad, gshift);
+        else
+            grad_cell_f64<ND, T, R, DIFF, false>(m, i, grad, gshift);
+    }
 }
 
 template <int ND, typename T, typename R, bool DIFF>

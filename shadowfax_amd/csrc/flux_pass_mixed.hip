@@ -27,10 +27,10 @@ hipError_t launch_entry_geometry<float>(int ndim, const flux::MeshView<float>& m
         if(m.C <= 0) return hipSuccess;
         const dim3 g((unsigned)(((m.C + 63) / 64 * 64 + 255) / 256));
         if(ndim == 3)
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3>), g, dim3(256), 0, s, m,
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, float>), g, dim3(256), 0, s, m,
                                egeo);
         else
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2>), g, dim3(256), 0, s, m,
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, float>), g, dim3(256), 0, s, m,
                                egeo);
         return hipGetLastError();
     }

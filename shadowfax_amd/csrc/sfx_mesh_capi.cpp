@@ -56,7 +56,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.cf_off = m->cf_off;
     v.cf_entry = m->cf_entry;
     v.egeo = (const T*)m->egeo;
-    v.egrp = (sizeof(T) == 4) ? m->egeo_group : nullptr;
+    v.egrp = m->egeo_group;
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
     return v;
@@ -75,8 +75,6 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
         return set_error(SFX_ERR_ARG, "%s: W must be 16-byte aligned", who);
     if(m->egeo && ((uintptr_t)m->egeo & 15))
         return set_error(SFX_ERR_ARG, "%s: egeo must be 16-byte aligned", who);
-    if(m->egeo_group && h->precision != SFX_PREC_MIXED)
-        return set_error(SFX_ERR_ARG, "%s: egeo_group is for the mixed records only", who);
     if(m->egeo_group && !m->cf_off)
         return set_error(SFX_ERR_ARG, "%s: egeo_group needs cf_off", who);
     if(gradients && (!m->W || !m->volume || !m->cf_off || !m->egeo))

@@ -290,8 +290,8 @@ class DeviceMesh:
     the prepared bundle (and of the oracle)."""
 
     def __init__(self, b, storage="fp64", device=None, grouped=None):
-        """grouped (mixed storage; default on): the gradient records in the
-        grouped layout of sfx_mesh_entry_groups."""
+        """grouped (default on): the gradient records in the grouped layout
+        of sfx_mesh_entry_groups."""
         import torch
         self.torch = torch
         self.dev = device or torch.device("cuda:0")
@@ -319,9 +319,7 @@ class DeviceMesh:
         self.G = int(b["gather_off"][-1])
         ER = Cp.lib().sfx_mesh_entry_record(
             nd, Cp.SFX_PREC_FP64 if storage == "fp64" else Cp.SFX_PREC_MIXED)
-        self.grouped = (storage == "mixed") if grouped is None else bool(grouped)
-        if self.grouped and storage != "mixed":
-            raise ValueError("the grouped record layout is for mixed storage")
+        self.grouped = True if grouped is None else bool(grouped)
         if self.grouped:
             off = np.ascontiguousarray(b["cf_off"], np.int64)
             grp = np.zeros((self.C + 63) // 64 + 1, np.int64)

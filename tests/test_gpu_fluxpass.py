@@ -176,19 +176,20 @@ def test_sedov3d_fp64_vs_oracle(device):
     assert np.abs(g["dQ"].sum(1)).max() <= 1e-13 * np.abs(odQL).max()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("nd", [2, 3])
-def test_grouped_records_bit_identical(device, nd, kind):
-    """The mixed gradient records in the grouped layout (64 cells' k-th
-    entries side by side) give the entry-ordered layout's gradients, face
-    fluxes and dQ bit for bit."""
+def test_grouped_records_bit_identical(device, nd, kind, prec):
+    """The gradient records in the grouped layout (64 cells' k-th entries
+    side by side) give the entry-ordered layout's gradients, face fluxes and
+    dQ bit for bit."""
     import torch
-    print("case nd %d kind %d" % (nd, kind), flush=True)
+    print("case nd %d kind %d %s" % (nd, kind, prec), flush=True)
     b = kh_bundle(20, kind) if nd == 2 else sedov_bundle(1500, kind)
     res = []
     for grouped in (False, True):
-        s = RiemannSolver(kind, nd, G, 1e-8, -5.0, "mixed")
-        fp = FluxPass(s, M.DeviceMesh(b, "mixed", device, grouped=grouped))
+        s = RiemannSolver(kind, nd, G, 1e-8, -5.0, prec)
+        fp = FluxPass(s, M.DeviceMesh(b, prec, device, grouped=grouped))
         fp.run()
         torch.cuda.synchronize()
         res.append({"grad": fp.m.grad_soa(), "dQL": fp.m.dQL_soa(), "dQ": fp.m.dQ_soa()})
