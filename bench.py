@@ -4,7 +4,7 @@ Workload (the configuration the headline metric is quoted on): 4,194,304
 independent 3D problems, gamma = 5/3, rho and p log-uniform in [1e-3, 1e3],
 velocities U(-2, 2), normals uniform on the sphere, plus 5% near-vacuum faces
 (rho 1e-8, p 1e-10 on one side) and 5% strong shocks (pressure ratio 1e6);
-seeded, synthetic (shadowfax_amd.workloads.CONFIG1).
+seeded, synthetic (shadowfax_amd.workloads.RIEMANN_4M).
 
 One step = one solve_for_flux pass of the exact Riemann solver over the whole
 batch in the mixed-precision path (FP32 storage and bulk arithmetic, FP64
@@ -99,7 +99,7 @@ def bench_fluxpass(b, kind, steps, warmup, dev, alg_bytes):
 
 
 def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
-    """Config 2: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC."""
+    """The 2D pass: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC."""
     from shadowfax_amd import mesh as M
     t0 = time.perf_counter()
     # generators in Hilbert order, as the reference keeps its particles
@@ -118,7 +118,7 @@ def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
 
 
 def bench_fluxpass3d(steps, warmup, dev, n=2097152, kind=0):
-    """Config 3: 2,097,152 Poisson generators in the 3D periodic unit box,
+    """The 3D pass: 2,097,152 Poisson generators in the 3D periodic unit box,
     Sedov blast (E = 1 in the 32 cells nearest the centre), exact solver."""
     from shadowfax_amd import mesh as M
     t0 = time.perf_counter()
@@ -138,7 +138,7 @@ def bench_fluxpass3d(steps, warmup, dev, n=2097152, kind=0):
 
 
 def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
-    """Config 4: an ensemble of n_meshes 2D periodic meshes (cell counts
+    """Serving: an ensemble of n_meshes 2D periodic meshes (cell counts
     log-uniform in [1e3, 2.5e5], Sod / Kelvin-Helmholtz / Sedov states, HLLC)
     resident in HBM; requests (one flux pass + update of one mesh) arrive as
     a Poisson process and are served open loop.  Batched mixed serving (the
@@ -148,21 +148,21 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
     import torch
     from shadowfax_amd import serving as S
     from shadowfax_amd.riemann import RiemannSolver
-    w = S.config4(n_meshes)
+    w = S.ensemble_workload(n_meshes)
     out = {"workload": "%d periodic 2D meshes, cells log-uniform in [1e3, 2.5e5] (%d "
                        "jittered-grid size classes, each mesh a translated copy with its own "
                        "states), Sod/KH/Sedov, HLLC; Poisson arrivals, uniform mesh choice"
                        % (n_meshes, len(np.unique(w["cls"]))),
            "meshes": n_meshes, "requests": nreq, "load": load}
     t0 = time.perf_counter()
-    classes = S.config4_classes(w)
+    classes = S.ensemble_classes(w)
     out["tessellation_s"] = time.perf_counter() - t0
     for prec, mode, name in (("mixed", S.SERVE_BATCHED, "batched_mixed"),
                              ("fp64", S.SERVE_PLAIN, "plain_fp64")):
         t0 = time.perf_counter()
         solver = RiemannSolver(1, 2, 5.0 / 3.0, 1e-8, -5.0, prec)
         solver.enable_stats(False)
-        ens, faces = S.build_config4(solver, dev, w, classes, log=log)
+        ens, faces = S.build_ensemble(solver, dev, w, classes, log=log)
         r = {"build_s": time.perf_counter() - t0}
         tot_c = sum(ens.cells)
         r["cells"], r["faces"] = int(tot_c), int(faces.sum())
@@ -299,8 +299,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--no-fluxpass", action="store_true")
-    ap.add_argument("--no-3d", action="store_true", help="skip the 3D flux pass (config 3)")
-    ap.add_argument("--no-serve", action="store_true", help="skip serving (config 4)")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D flux pass (2,097,152-cell Sedov mesh)")
+    ap.add_argument("--no-serve", action="store_true", help="skip serving (2,000-mesh ensemble)")
     ap.add_argument("--serve-meshes", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--only", default=None,
@@ -311,7 +311,7 @@ def main():
                          "the scale reference for hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the face-flux path on a single "
+        print("bench.py measures one GPU: the face-flux path runs on a single "
               "MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -322,7 +322,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    cfg = dict(WK.CONFIG1)
+    cfg = dict(WK.RIEMANN_4M)
     if args.N:
         cfg["N"] = args.N
     N = cfg["N"]
@@ -405,20 +405,20 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "face-fluxes/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32+f64", "data": "synthetic (seeded, shadowfax_amd.workloads.CONFIG1)",
+           "dtype": "f32+f64", "data": "synthetic (seeded, shadowfax_amd.workloads.RIEMANN_4M)",
            "config": {"workload": "batched Riemann solve_for_flux, 4,194,304 3D problems, "
-                                  "gamma 5/3, 5% near-vacuum + 5% 1e6 shocks (config 1); "
+                                  "gamma 5/3, 5% near-vacuum + 5% 1e6 shocks; "
                                   "exact solver, mixed precision",
                       "N": N, "solver": "exact", "precision": "mixed", "seed": cfg["seed"]},
            "roofline": roof, "variants": variants, "errors": errors}
     if not args.no_fluxpass:
-        log("flux pass 2D (config 2)")
+        log("flux pass 2D (512x512 KH mesh)")
         out["fluxpass_2d_kh"] = bench_fluxpass2d(args.steps, args.warmup, dev)
         if not args.no_3d:
-            log("flux pass 3D (config 3)")
+            log("flux pass 3D (2,097,152-cell Sedov mesh)")
             out["fluxpass_3d_sedov"] = bench_fluxpass3d(args.steps, args.warmup, dev)
     if not args.no_serve:
-        log("serving (config 4)")
+        log("serving (2,000-mesh ensemble)")
         out["serving"] = bench_serving(dev, args.serve_meshes, args.serve_requests)
     if not args.no_cpu:
         log("cpu baseline")

@@ -1,5 +1,5 @@
-"""Serving an ensemble of independent meshes held in HBM (the config-4
-workload): ctypes wrapper of the sfx_ensemble_* C ABI, the ensemble builder
+"""Serving an ensemble of independent meshes held in HBM (the 2,000-mesh
+serving workload): ctypes wrapper of the sfx_ensemble_* C ABI, the ensemble builder
 and the arrival trace.
 
 Each request is one hydro step of one mesh: VorTessManager::hydro (gradients,
@@ -186,8 +186,8 @@ def ensemble_ics(kind_ic, pos, mesh, gamma, rng_u):
     return M.sedov_ics(mm, gamma=gamma, ncells=16)
 
 
-def config4(n_meshes=2000, seed=20240605, nclasses=64, gamma=5.0 / 3.0, kind=1):
-    """The config-4 ensemble: n_meshes periodic 2D meshes with cell counts
+def ensemble_workload(n_meshes=2000, seed=20240605, nclasses=64, gamma=5.0 / 3.0, kind=1):
+    """The serving ensemble: n_meshes periodic 2D meshes with cell counts
     log-uniform in [1e3, 2.5e5], quantised to nclasses log-spaced jittered
     grids (n x n generators in Hilbert order, tessellated once per class);
     each mesh is its class's generator set translated by its own random
@@ -231,19 +231,19 @@ def poisson_trace(nreq, rate, n_meshes, seed):
     return np.cumsum(gaps), mesh.astype(np.int32)
 
 
-def config4_classes(w):
+def ensemble_classes(w):
     """One tessellated, prepared mesh per size class in use."""
     return {int(k): class_bundle(w["sides"][k], w["seed"], w["gamma"], w["kind"])
             for k in np.unique(w["cls"])}
 
 
-def build_config4(solver, dev, w, classes=None, arena_cells=4_000_000, log=None):
-    """The config-4 ensemble on the device for one solver (precision): every
+def build_ensemble(solver, dev, w, classes=None, arena_cells=4_000_000, log=None):
+    """The serving ensemble on the device for one solver (precision): every
     mesh's own states on its class's tessellation.  Returns (Ensemble,
     per-mesh faces)."""
     import torch
     g, kind = w["gamma"], w["kind"]
-    classes = classes or config4_classes(w)
+    classes = classes or ensemble_classes(w)
     tmpl = {k: (b, M.DeviceMesh(b, solver.precision, dev)) for k, b in classes.items()}
     sizes = [(tmpl[k][0]["C"], tmpl[k][0]["F"], int(tmpl[k][0]["cf_off"][-1])) for k in w["cls"]]
     C = sum(x[0] for x in sizes)

@@ -142,8 +142,8 @@ def test_update_step_matches_oracle(device):
     assert int(fp.m.cstatus.max()) == 0
 
 
-def test_config2_full_mesh_fp64_vs_oracle(device):
-    """512^2 jittered periodic KH mesh (the config-2 workload), HLLC: the
+def test_kh512_full_mesh_fp64_vs_oracle(device):
+    """512^2 jittered periodic KH mesh (the 2D benchmark mesh), HLLC: the
     whole FP64 pass against the oracle."""
     b = kh_bundle(512, 1, seed=20240603)
     print("faces", b["F"], flush=True)
@@ -152,13 +152,13 @@ def test_config2_full_mesh_fp64_vs_oracle(device):
     assert np.array_equal(g["grad"], og)
     den = np.maximum(np.abs(odQL).max(0), face_scale(b, 1))
     err = (np.abs(g["dQL"] - odQL).max(0) / np.where(den > 0, den, 1.0)).max()
-    print("config2 fp64 face err %.3e" % err)
+    print("kh512 fp64 face err %.3e" % err)
     assert err <= 1e-12
     assert np.abs(g["dQ"] - odQ).max() <= 1e-12 * np.abs(odQ).max()
 
 
 def test_sedov3d_fp64_vs_oracle(device):
-    """3D periodic Poisson mesh with a Sedov blast (the config-3 workload at
+    """3D periodic Poisson mesh with a Sedov blast (the 3D benchmark workload at
     20,000 cells), exact solver: the whole FP64 pass against the oracle."""
     b = sedov_bundle(20000, 0, seed=20240604)
     print("cells", b["C"], "faces", b["F"], flush=True)

@@ -146,38 +146,38 @@ def test_scalar_entry_points(device):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_full_size_config0(device, kind):
-    """All 262,144 problems of the CPU-runnable configuration (rho, p
+def test_full_size_262k(device, kind):
+    """All 262,144 problems of the CPU-runnable batch (rho, p
     log-uniform over six decades, velocities in [-2, 2], normals uniform on
     the sphere): the FP64 path against the oracle (the reference's algorithm,
     bit-exact with the compiled reference on the golden vectors) to 1e-12,
     the mixed path against FP64 to 1e-6, and the vacuum count."""
-    cfg = WK.CONFIG0
+    cfg = WK.RIEMANN_262K
     b = WK.riemann_batch(cfg["N"], seed=cfg["seed"], family=cfg["family"], ndim=3)
-    print("config0 kind", kind, flush=True)
+    print("262k kind", kind, flush=True)
     F64, st64, s = gpu_flux(kind, 3, b["WL"], b["WR"], b["n"], None, None, device)
     Fo, sto, c = O.solve_for_flux(kind, 3, b["WL"], b["WR"], b["n"])
     ok = (st64 == 0) & (sto == 0)
     assert ok.mean() > 0.999 and np.array_equal(st64 != 0, sto != 0)
     err = summarize(normwise_error(F64[:, ok], Fo[:, ok], b["WL"][:, ok], b["WR"][:, ok], G))
-    print("config0 fp64 vs oracle", err, flush=True)
+    print("262k fp64 vs oracle", err, flush=True)
     assert err["max"] <= FP64_TOL
     assert s.stats()["nvacuum"] == c.nvacuum
     F32, st32, _ = gpu_flux(kind, 3, b["WL"], b["WR"], b["n"], None, None, device,
                             precision="mixed", f32=True)
     ok2 = ok & (st32 == 0)
     e2 = summarize(normwise_error(F32[:, ok2], F64[:, ok2], b["WL"][:, ok2], b["WR"][:, ok2], G))
-    print("config0 mixed vs fp64", e2, flush=True)
+    print("262k mixed vs fp64", e2, flush=True)
     assert e2["max"] <= MIXED_TOL
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_full_size_config1(device, kind):
+def test_full_size_4m(device, kind):
     """4,194,304 problems of the benchmark configuration: mixed (FP32
     storage) vs FP64 on the GPU, plus a 20,000-problem oracle spot check of
     the FP64 path."""
     import torch
-    cfg = WK.CONFIG1
+    cfg = WK.RIEMANN_4M
     b = WK.riemann_batch(cfg["N"], seed=cfg["seed"], family=cfg["family"], ndim=3)
     print("generated", flush=True)
     F64, st64, _ = gpu_flux(kind, 3, b["WL"], b["WR"], b["n"], None, None, device)
@@ -186,7 +186,7 @@ def test_full_size_config1(device, kind):
     ok = (st64 == 0) & (st32 == 0)
     assert ok.mean() > 0.999
     err = summarize(normwise_error(F32[:, ok], F64[:, ok], b["WL"][:, ok], b["WR"][:, ok], G))
-    print("config1 kind", kind, "mixed vs fp64", err)
+    print("4m kind", kind, "mixed vs fp64", err)
     assert err["max"] <= MIXED_TOL
     idx = np.arange(0, cfg["N"], cfg["N"] // 20000)[:20000]
     sub = {k: b[k][:, idx] for k in ("WL", "WR", "n")}

@@ -1,5 +1,5 @@
 """Serving, host side (CPU): FIFO admission by scratch footprint and the
-config-4 workload generator."""
+generator of the 2,000-mesh serving workload."""
 import numpy as np
 
 from shadowfax_amd import serving as S
@@ -30,8 +30,8 @@ def test_admit_defers_a_second_step_of_the_same_mesh():
     assert admit([], 1000) == []
 
 
-def test_config4_workload():
-    w = S.config4(n_meshes=2000)
+def test_ensemble_workload():
+    w = S.ensemble_workload(n_meshes=2000)
     sizes = S.mesh_sizes(2000, w["seed"])
     assert sizes.min() >= 1e3 and sizes.max() <= 2.5e5
     # log-uniform: the mean of log10 C is the midpoint of [3, log10 2.5e5]
