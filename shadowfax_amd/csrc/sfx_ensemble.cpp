@@ -24,6 +24,10 @@ struct sfx_ensemble {
     size_t tsz = 8;
     int64_t cap_mesh = 0, cap_cells = 0, cap_faces = 0, cap_ent = 0, cap_gent = 0;
     int64_t nmesh = 0, ncells = 0, nfaces = 0, nent = 0, ngent = 0;
+    // each mesh's first cell is a multiple of 64, so a mesh owns whole groups
+    // of the grouped record layout: the global cell arrays hold cap_alloc =
+    // cap_cells + 64 cap_mesh rows, cell_end is one past the last used row
+    int64_t cap_alloc = 0, cell_end = 0;
     // global arrays
     double *W = nullptr, *Q = nullptr, *dQ = nullptr, *mass = nullptr, *total_area = nullptr,
            *dt = nullptr, *volume = nullptr;
@@ -31,7 +35,8 @@ struct sfx_ensemble {
     uint8_t *active = nullptr, *ghost = nullptr, *fstatus = nullptr, *cstatus = nullptr;
     int32_t *left = nullptr, *right = nullptr, *twin = nullptr, *cf_entry = nullptr,
             *gather_entry = nullptr;
-    int64_t *cf_off = nullptr, *gather_off = nullptr;
+    int64_t *cf_off = nullptr, *gather_off = nullptr, *egrp = nullptr;
+    size_t egeo_bytes = 0;
     void *area = nullptr, *dL = nullptr, *dR = nullptr, *rRL = nullptr, *mc = nullptr,
          *rcent = nullptr, *egeo = nullptr;
     // mesh table
@@ -81,7 +86,7 @@ hipError_t dalloc(sfx_ensemble* e, P** p, size_t bytes) {
 template <typename T>
 flux::MeshView<T> ens_view(const sfx_ensemble* e) {
     flux::MeshView<T> v;
-    v.C = e->cap_cells;
+    v.C = e->cap_alloc;
     v.F = e->cap_faces;
     v.W = e->W;
     v.vel = (const T*)e->vel;
@@ -103,6 +108,7 @@ flux::MeshView<T> ens_view(const sfx_ensemble* e) {
     v.cf_off = e->cf_off;
     v.cf_entry = e->cf_entry;
     v.egeo = (const T*)e->egeo;
+    v.egrp = e->egrp;
     v.gather_off = e->gather_off;
     v.gather_entry = e->gather_entry;
     return v;
@@ -130,6 +136,13 @@ EnsArgs<T> ens_args(const sfx_ensemble* e) {
     return a;
 }
 
+void drop_graph(sfx_ensemble* e) {
+    if(e->exec) hipGraphExecDestroy(e->exec);
+    if(e->graph) hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
 // plan + drains for the batch already in d_batch
 hipError_t launch_batch(sfx_ensemble* e, hipStream_t s) {
     hipError_t err = ens_launch_plan(e->d_batch, e->d_c0, e->d_nc, e->d_f0, e->d_nf, e->block,
@@ -140,13 +153,45 @@ hipError_t launch_batch(sfx_ensemble* e, hipStream_t s) {
                    : ens_launch_step<float>(e->nd, e->h->kind, ens_args<float>(e), s);
 }
 
+// The gradient records in the grouped layout (groups of 64 global cells,
+// one row per entry of the group's fullest cell; flux_pass_impl.h) over
+// cells 0 .. cell_end - 1, the alignment gaps included (no entries).
 int build_entries(sfx_ensemble* e) {
     if(e->entries_built) return SFX_OK;
-    hipError_t err = e->prec == SFX_PREC_FP64
-                             ? ens_launch_entries<double>(e->nd, ens_view<double>(e), e->nent,
-                                                          (double*)e->egeo, e->stream)
-                             : ens_launch_entries<float>(e->nd, ens_view<float>(e), e->nent,
-                                                         (float*)e->egeo, e->stream);
+    const int64_t C = e->cell_end, ng = (C + 63) / 64;
+    std::vector<int64_t> off(C + 1), grp(ng + 1);
+    hipError_t err = hipStreamSynchronize(e->stream);
+    if(err == hipSuccess)
+        err = hipMemcpy(off.data(), e->cf_off, 8 * (C + 1), hipMemcpyDeviceToHost);
+    if(err != hipSuccess) return hip_error(err, "ensemble entry groups");
+    const int64_t rows = sfx_mesh_entry_groups(C, off.data(), grp.data());
+    if(rows < 0) return SFX_ERR_ARG;
+    const size_t bytes = (size_t)rows * 64 * flux::entry_rec(e->nd, e->tsz == 4) * e->tsz;
+    if(bytes > e->egeo_bytes) {
+        e->allocs.erase(std::find(e->allocs.begin(), e->allocs.end(), e->egeo));
+        hipFree(e->egeo);
+        e->egeo = nullptr;
+        e->egeo_bytes = 0;
+        err = dalloc(e, &e->egeo, bytes);
+        if(err != hipSuccess)
+            return err == hipErrorOutOfMemory
+                           ? set_error(SFX_ERR_NOMEM, "ensemble entry records: out of device memory")
+                           : hip_error(err, "ensemble entry records");
+        e->egeo_bytes = bytes;
+        drop_graph(e);
+    }
+    err = hipMemcpy(e->egrp, grp.data(), 8 * (ng + 1), hipMemcpyHostToDevice);
+    if(err == hipSuccess) {
+        if(e->prec == SFX_PREC_FP64) {
+            flux::MeshView<double> v = ens_view<double>(e);
+            v.C = C;
+            err = ens_launch_entries<double>(e->nd, v, e->nent, (double*)e->egeo, e->stream);
+        } else {
+            flux::MeshView<float> v = ens_view<float>(e);
+            v.C = C;
+            err = ens_launch_entries<float>(e->nd, v, e->nent, (float*)e->egeo, e->stream);
+        }
+    }
     if(err != hipSuccess) return hip_error(err, "ensemble entry records");
     e->entries_built = true;
     return SFX_OK;
@@ -174,13 +219,6 @@ int capture(sfx_ensemble* e) {
     }
     e->graph = g;
     return SFX_OK;
-}
-
-void drop_graph(sfx_ensemble* e) {
-    if(e->exec) hipGraphExecDestroy(e->exec);
-    if(e->graph) hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
 }
 
 int check_batch(const sfx_ensemble* e, const int32_t* ids, int32_t n) {
@@ -278,7 +316,8 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     if(max_meshes <= 0 || max_cells <= 0 || max_faces <= 0 || max_entries <= 0 ||
        arena_cells <= 0 || arena_faces <= 0)
         return set_error(SFX_ERR_ARG, "sfx_ensemble_create: sizes must be > 0");
-    if(max_cells > INT32_MAX || max_faces > (INT32_MAX >> 1) || max_entries > INT32_MAX)
+    if(max_cells + 64 * max_meshes > INT32_MAX || max_faces > (INT32_MAX >> 1) ||
+       max_entries > INT32_MAX)
         return set_error(SFX_ERR_ARG, "sfx_ensemble_create: too large for 32-bit indices");
     int rc = bind_device(h);
     if(rc) return rc;
@@ -295,7 +334,8 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     e->cap_gent = max_entries;
     e->arena_cells = arena_cells;
     e->arena_faces = arena_faces;
-    const int64_t C = max_cells, F = max_faces, E = max_entries, nd = e->nd, NC = e->NC;
+    e->cap_alloc = max_cells + 64 * max_meshes;
+    const int64_t C = e->cap_alloc, F = max_faces, E = max_entries, nd = e->nd, NC = e->NC;
     const size_t t = e->tsz;
     const int ER = flux::entry_rec(e->nd, t == 4);
     hipError_t err = hipSuccess;
@@ -327,7 +367,8 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     A(&e->mc, t * F * nd);
     A(&e->cf_entry, 4 * E);
     A(&e->gather_entry, 4 * E);
-    A(&e->egeo, t * E * ER);
+    A(&e->egeo, t * E * ER);  // grown by build_entries when the groups' padding needs it
+    A(&e->egrp, 8 * ((C + 63) / 64 + 1));
     A(&e->d_c0, 8 * max_meshes);
     A(&e->d_nc, 8 * max_meshes);
     A(&e->d_f0, 8 * max_meshes);
@@ -336,6 +377,7 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     A(&e->dQL, t * arena_faces * NC);
     A(&e->d_batch, sizeof(int32_t) * (1 + ens::kMaxBatch));
     A(&e->d_plan, sizeof(ens::Plan));
+    e->egeo_bytes = std::max<size_t>(t * E * ER, 256);
     if(err == hipSuccess) err = hipHostMalloc((void**)&e->h_batch,
                                               sizeof(int32_t) * (1 + ens::kMaxBatch));
     if(err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -381,11 +423,22 @@ int sfx_ensemble_add_mesh(sfx_ensemble* e, const sfx_mesh* m, const double* Q, i
                          (long long)C, (long long)F);
     int rc = bind_device(e->h);
     if(rc) return rc;
-    const int64_t c0 = e->ncells, f0 = e->nfaces, e0 = e->nent, g0 = e->ngent;
+    const int64_t c0 = (e->cell_end + 63) / 64 * 64, f0 = e->nfaces, e0 = e->nent,
+                  g0 = e->ngent;
     const int nd = e->nd, NC = e->NC;
     const size_t t = e->tsz;
     hipStream_t s = e->stream;
     hipError_t err = hipSuccess;
+    // the gap cells cell_end .. c0 - 1 have no entries: their offsets repeat
+    // the previous mesh's end (read by the host before the stream syncs below)
+    const std::vector<int64_t> gap_e(c0 - e->cell_end, e0), gap_g(c0 - e->cell_end, g0);
+    if(c0 > e->cell_end) {
+        err = hipMemcpyAsync(e->cf_off + e->cell_end + 1, gap_e.data(), 8 * gap_e.size(),
+                             hipMemcpyHostToDevice, s);
+        if(err == hipSuccess)
+            err = hipMemcpyAsync(e->gather_off + e->cell_end + 1, gap_g.data(),
+                                 8 * gap_g.size(), hipMemcpyHostToDevice, s);
+    }
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         if(err == hipSuccess && bytes)
             err = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
@@ -436,6 +489,7 @@ int sfx_ensemble_add_mesh(sfx_ensemble* e, const sfx_mesh* m, const double* Q, i
     e->f0.push_back(f0);
     e->nf.push_back(F);
     e->ncells += C;
+    e->cell_end = c0 + C;
     e->nfaces += F;
     e->nent += E;
     e->ngent += G;
