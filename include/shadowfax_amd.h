@@ -315,8 +315,10 @@ typedef struct {
     const int32_t* left;       /* [F] */
     const int32_t* right;      /* [F] -1: mirror */
     const uint8_t* ghost;      /* [F] */
-    const int32_t* twin;       /* [F] or NULL (mixed pass: one evaluation per
-                                  periodic pair when given) */
+    const int32_t* twin;       /* [F] or NULL: the other copy of a periodic
+                                  face, -1 on every other face (mixed pass:
+                                  one evaluation per periodic pair when
+                                  given) */
     const void* area;          /* T [F] */
     const void* dL;            /* T [ndim][F]  (sfx_face_geometry) */
     const void* dR;            /* T [ndim][F] */
@@ -379,7 +381,9 @@ int sfx_gradients(sfx_solver* h, const sfx_mesh* m, void* grad, void* stream);
 /* VorFace::set_v + calculate_flux for every face (src/VorFace.cpp:586-866):
  * dQL: T [F][ndim+2] the flux exchanged through each face, dt*A*F after the
  * mass-flux limiter, seen from the left cell (the right cell receives -dQL);
- * face_status: [F] or NULL. */
+ * face_status: [F] or NULL.  Mixed precision with twin: each periodic pair is
+ * evaluated once, by its lower-index copy, which also writes the other
+ * copy's row (negated: seen from that copy's left cell). */
 int sfx_face_fluxes(sfx_solver* h, const sfx_mesh* m, const void* grad,
                     void* dQL, uint8_t* face_status, void* stream);
 

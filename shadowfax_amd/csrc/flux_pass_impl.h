@@ -47,6 +47,9 @@
 #ifndef SFX_GATHER_KB
 #define SFX_GATHER_KB 8
 #endif
+#ifndef SFX_GATHER_KB_F64  // FP64 rows: the same registers as 8 FP32 rows
+#define SFX_GATHER_KB_F64 4
+#endif
 
 namespace sfx {
 namespace flux {
@@ -106,10 +109,12 @@ struct Rec32 {
 };
 
 // Row o of an item-major FP64 array of NC = 4 or 5 values per item
-// ([n][NC], 16-byte aligned base) with 16-byte loads: for NC = 5 the aligned
-// window of six values that holds the row is read and the row picked out of
-// it by the row's parity; the last row, whose window would end past the
-// array, is read value by value.
+// ([n][NC], 16-byte aligned base) with 16-byte loads.  For NC = 5 the row
+// starts at s = 5 o; from the aligned base b = s & ~1 two 16-byte loads
+// hold values b .. b + 3 and two 8-byte loads the values b + 4 and (odd
+// rows) b + 5, so no load reads past the row's last value s + 4; the row is
+// picked out by the parity of s.  No branch: the loads of several rows go
+// out back to back and nothing waits for them before their values are used.
 template <int NC>
 __device__ __forceinline__ void load_row(const double* __restrict__ a, int64_t o, int64_t n,
                                          double* out) {
@@ -122,20 +127,18 @@ __device__ __forceinline__ void load_row(const double* __restrict__ a, int64_t o
         out[2] = y.x;
         out[3] = y.y;
     } else {
+        (void)n;
         const int64_t s = o * 5, base = s & ~int64_t(1);
-        if(base + 6 <= n * 5) {
-            const double2* p = reinterpret_cast<const double2*>(a + base);
-            const double2 x = p[0], y = p[1], z = p[2];
-            const bool odd = (s & 1) != 0;
-            out[0] = odd ? x.y : x.x;
-            out[1] = odd ? y.x : x.y;
-            out[2] = odd ? y.y : y.x;
-            out[3] = odd ? z.x : y.y;
-            out[4] = odd ? z.y : z.x;
-        } else {
-#pragma unroll
-            for(int k = 0; k < 5; k++) out[k] = a[s + k];
-        }
+        const bool odd = (s & 1) != 0;
+        const double2* p = reinterpret_cast<const double2*>(a + base);
+        const double2 x = p[0], y = p[1];
+        // (base + 4 + (s & 1): an index, not a choice between two loads)
+        const double z0 = a[base + 4], z1 = a[base + 4 + (s & 1)];
+        out[0] = odd ? x.y : x.x;
+        out[1] = odd ? y.x : x.y;
+        out[2] = odd ? y.y : y.x;
+        out[3] = odd ? z0 : y.y;
+        out[4] = odd ? z1 : z0;
     }
 }
 template <int ND, typename T>
@@ -515,26 +518,17 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
 // a = Wmax - W, b = dp for dp > 0 and a = W - Wmin, b = -dp for dp < 0
 // (dp = 0 gives 1, which never lowers alpha <= 1); the smallest a/b is kept
 // as a pair, compared by cross products, and divided out once per component.
-template <int ND, bool GROUPED>
+template <int ND, bool GROUPED, bool ACT>
 __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t i,
                                               float* __restrict__ grad, int64_t gshift) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, float>();
-    if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
-    double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
-    load_row<NC>(m.W, i, m.C, W);
-#pragma unroll
-    for(int k = 0; k < NC; k++) {
-        Wmax[k] = Wmin[k] = W[k];
-#pragma unroll
-        for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
-    }
-    // the cell's records, visited in reverse walk order (src/VorCell.cpp:471)
-    // as positions n - 1 .. 0: entry j0 + p (entry order), or row r0 + p of
-    // the cell's group, lane i % 64 (grouped; rows past the cell's own
-    // entries hold area 0 and are skipped, so every lane of a group walks
-    // the same rows)
+    // The loads that depend on nothing go out together, before any wait:
+    // the activity flag (ACT: m.active is set), the record range, the cell's
+    // state and volume.  Inactive cells (which keep their gradients) walk no
+    // records and store nothing; a branch on the flag here would put its
+    // load alone in front of all the others.
+    const bool act = !ACT || m.active[i] != 0;
     int64_t j0, n;
-    const float4* E4 = reinterpret_cast<const float4*>(m.egeo);
     if constexpr(GROUPED) {
         const int64_t g = i >> 6;
         j0 = m.egrp[g];
@@ -543,6 +537,22 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         j0 = m.cf_off[i];
         n = m.cf_off[i + 1] - j0;
     }
+    double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
+    load_row<NC>(m.W, i, m.C, W);
+    const double Vinv = mixed::rcp_nr(m.volume[i]);
+    if(!act) n = 0;
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        Wmax[k] = Wmin[k] = W[k];
+#pragma unroll
+        for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
+    }
+    // the cell's records, visited in reverse walk order (src/VorCell.cpp:471)
+    // as positions n - 1 .. 0: entry j0 + p (entry order), or row j0 + p of
+    // the cell's group, lane i % 64 (grouped; rows past the cell's own
+    // entries hold area 0 and are skipped, so every lane of a group walks
+    // the same rows)
+    const float4* E4 = reinterpret_cast<const float4*>(m.egeo);
     const int lane = (int)(i & 63);
     auto half = [&](int64_t p, int h) -> float4 {
         if constexpr(GROUPED)
@@ -551,47 +561,61 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             return E4[(j0 + p) * (ER / 4) + h];
     };
     constexpr int KB = (ND == 2) ? SFX_GRAD_KB_2D : SFX_GRAD_KB;
-    // 2D: a cell's entries mostly fit one chunk; their second halves are
-    // then kept for the limiter loop instead of being read again
-    constexpr bool KEEP = ND == 2;
-    const bool one_chunk = KEEP && n <= KB;
-    float4 kept[KEEP ? KB : 1];
-    for(int64_t pb = n; pb > 0; pb -= KB) {
-        float4 lo[KB], hi[KB];
-        double Wj[KB][NC];
+    // positions pb - 1 .. pb - KB; those below 0 read position 0 (n > 0)
+    // and are skipped where used (u >= pb), not masked here: a register
+    // write right after the load would wait for it
+    auto load_chunk = [&](int64_t pb, float4* L, float4* H) {
 #pragma unroll
         for(int u = 0; u < KB; u++) {
             const int64_t p = pb - 1 - u;
-            lo[u] = half(p >= 0 ? p : 0, 0);
-            hi[u] = half(p >= 0 ? p : 0, 1);
-            if(p < 0) hi[u].w = hi[u].z = 0.f;  // padding: A = 0 (A sits at .z or .w)
+            L[u] = half(p >= 0 ? p : 0, 0);
+            H[u] = half(p >= 0 ? p : 0, 1);
         }
-        if constexpr(KEEP) {
-#pragma unroll
-            for(int u = 0; u < KB; u++) kept[u] = hi[u];
-        }
+    };
+    // 2D: a cell's entries fit one chunk (at most 8 in the meshes here): the
+    // records stay in registers for the limiter loop.  3D: chunk after
+    // chunk, the next chunk's records in flight while this chunk's
+    // neighbour states load.
+    constexpr bool KEEP = ND == 2;
+    constexpr bool PIPE = ND == 3;
+    const bool one_chunk = KEEP && n > 0 && n <= KB;
+    float4 lo[KB], hi[KB];
+    if(n > 0) load_chunk(n, lo, hi);
+    for(int64_t pb = n; pb > 0; pb -= KB) {
+        // every neighbour row is loaded unconditionally (a mirror neighbour
+        // is the cell's own row, reflected below), so the loads go out back
+        // to back
+        double Wj[KB][NC];
+        int32_t nbo[KB];
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            const float nbf = (ND == 3) ? lo[u].w : lo[u].z;
-            const int32_t o = bits_index(nbf);
-            if(o >= 0) {
-                load_row<NC>(m.W, o, m.C, Wj[u]);
-            } else {
-                // mirror neighbour (src/VorCell.cpp:482-489) of face ~o;
-                // only a left cell has one, so rRL is seen from it
-                const int32_t f = ~o;
+            nbo[u] = bits_index((ND == 3) ? lo[u].w : lo[u].z);
+            load_row<NC>(m.W, nbo[u] >= 0 ? (int64_t)nbo[u] : i, m.C, Wj[u]);
+        }
+        float4 nlo[PIPE ? KB : 1], nhi[PIPE ? KB : 1];
+        if constexpr(PIPE) load_chunk(pb - KB, nlo, nhi);
+        // mirror neighbours (src/VorCell.cpp:482-489) of face ~o: only a left
+        // cell has one, so rRL is seen from it
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            if(nbo[u] < 0) {
+                const int32_t f = ~nbo[u];
                 double rr[ND];
 #pragma unroll
                 for(int l = 0; l < ND; l++) rr[l] = (double)m.rRL[l * m.F + f];
-#pragma unroll
-                for(int k = 0; k < NC; k++) Wj[u][k] = W[k];
                 mirror_velocity<ND, double>(Wj[u] + 1, rr);
             }
         }
+        if constexpr(KEEP) {
+            // the limiter's halves must be in registers by now, not read
+            // again after this loop
+#pragma unroll
+            for(int u = 0; u < KB; u++) asm volatile("" ::"v"(hi[u].x), "v"(hi[u].y));
+        }
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            const float A = (ND == 3) ? hi[u].w : hi[u].z;
-            if(!(A != 0.f)) continue;  // also the padding
+            const float A = (ND == 3) ? hi[u].w : hi[u].z;  // 0: a padding row
+            if(u >= pb || !(A != 0.f)) continue;
             const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
                                   (ND == 3) ? (double)lo[u].z : 0.0};
 #pragma unroll
@@ -603,8 +627,16 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
                 Wmin[k] = sminr<double>(Wmin[k], Wj[u][k]);
             }
         }
+        if constexpr(PIPE) {
+#pragma unroll
+            for(int u = 0; u < KB; u++) {
+                lo[u] = nlo[u];
+                hi[u] = nhi[u];
+            }
+        } else if(!one_chunk && pb > KB) {
+            load_chunk(pb - KB, lo, hi);
+        }
     }
-    const double Vinv = mixed::rcp_nr(m.volume[i]);
 #pragma unroll
     for(int l = 0; l < ND; l++)
 #pragma unroll
@@ -612,41 +644,48 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     double na[NC], nb[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) na[k] = nb[k] = 1.0;
-    constexpr int KL = (ND == 2) ? SFX_GRAD_KB_2D : 4;
-    for(int64_t pb = n; pb > 0; pb -= KL) {
-        float4 hi[KL];
+    // the limiter over the records' second halves (d and A)
+    auto limit = [&](const float4& h, bool valid) {
+        const float A = (ND == 3) ? h.w : h.z;
+        if(!valid || !(A != 0.f)) return;
+        const double dd[3] = {(double)h.x, (double)h.y, (ND == 3) ? (double)h.z : 0.0};
 #pragma unroll
-        for(int u = 0; u < KL; u++) {
-            const int64_t p = pb - 1 - u;
-            if constexpr(KEEP) {
-                // KL == KB in 2D: the first chunk's halves are still held
-                if(one_chunk) {
-                    hi[u] = kept[u];
-                    continue;
-                }
+        for(int k = 0; k < NC; k++) {
+            double dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
+            if(ND == 3) dp += delta[ND - 1][k] * dd[ND - 1];
+            const double a = (dp > 0.0) ? Wmax[k] - W[k] : W[k] - Wmin[k];
+            const double b = fabs(dp);
+            if(dp != 0.0 && a * nb[k] < na[k] * b) {
+                na[k] = a;
+                nb[k] = b;
             }
-            hi[u] = half(p >= 0 ? p : 0, 1);
-            if(p < 0) hi[u].w = hi[u].z = 0.f;
         }
+    };
+    if(one_chunk) {
 #pragma unroll
-        for(int u = 0; u < KL; u++) {
-            const float A = (ND == 3) ? hi[u].w : hi[u].z;
-            if(!(A != 0.f)) continue;
-            const double dd[3] = {(double)hi[u].x, (double)hi[u].y,
-                                  (ND == 3) ? (double)hi[u].z : 0.0};
+        for(int u = 0; u < KB; u++) limit(hi[u], u < n);
+    } else if(n > 0) {
+        constexpr int KL = (ND == 2) ? SFX_GRAD_KB_2D : 4;
+        float4 h[KL];
+        auto load_halves = [&](int64_t pb, float4* H) {
 #pragma unroll
-            for(int k = 0; k < NC; k++) {
-                double dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
-                if(ND == 3) dp += delta[ND - 1][k] * dd[ND - 1];
-                const double a = (dp > 0.0) ? Wmax[k] - W[k] : W[k] - Wmin[k];
-                const double b = fabs(dp);
-                if(dp != 0.0 && a * nb[k] < na[k] * b) {
-                    na[k] = a;
-                    nb[k] = b;
-                }
+            for(int u = 0; u < KL; u++) {
+                const int64_t p = pb - 1 - u;
+                H[u] = half(p >= 0 ? p : 0, 1);
             }
+        };
+        load_halves(n, h);
+        for(int64_t pb = n; pb > 0; pb -= KL) {
+            // the next chunk's halves load while this one is used
+            float4 nh[KL];
+            load_halves(pb - KL, nh);
+#pragma unroll
+            for(int u = 0; u < KL; u++) limit(h[u], u < pb);
+#pragma unroll
+            for(int u = 0; u < KL; u++) h[u] = nh[u];
         }
     }
+    if(!act) return;
 #pragma unroll
     for(int k = 0; k < NC; k++) {
         const double alpha = na[k] * mixed::rcp_nr(nb[k]);
@@ -658,21 +697,15 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 
 // Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
 // for a whole mesh, the arena offset of the cell's mesh in an ensemble).
-template <int ND, typename T, typename R, bool DIFF, bool GROUPED>
+template <int ND, typename T, typename R, bool DIFF, bool GROUPED, bool ACT>
 __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
                                               T* __restrict__ grad, int64_t gshift) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, T>();
-    if(m.active && !m.active[i]) return;  // inactive cells keep their gradients
-    R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
-#pragma unroll
-    for(int k = 0; k < NC; k++) {
-        W[k] = Wmax[k] = Wmin[k] = (R)m.W[i * NC + k];
-#pragma unroll
-        for(int l = 0; l < ND; l++) delta[l][k] = R(0);
-    }
-    // the cell's records as positions n - 1 .. 0 (reverse walk order,
-    // src/VorCell.cpp:471): entries base + p, or rows base + p of the
-    // cell's group (grouped; rows past the cell's own entries have area 0)
+    // the loads that depend on nothing go out together: the activity flag
+    // (ACT: m.active is set), the record range, the cell's state and
+    // volume; inactive cells (which keep their gradients) walk no records
+    // and store nothing
+    const bool act = !ACT || m.active[i] != 0;
     int64_t base, n;
     if constexpr(GROUPED) {
         base = m.egrp[i >> 6];
@@ -681,10 +714,25 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
         base = m.cf_off[i];
         n = m.cf_off[i + 1] - base;
     }
+    R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        W[k] = Wmax[k] = Wmin[k] = (R)m.W[i * NC + k];
+#pragma unroll
+        for(int l = 0; l < ND; l++) delta[l][k] = R(0);
+    }
+    const R V = (R)m.volume[i];
+    if(!act) n = 0;
+    // the cell's records as positions n - 1 .. 0 (reverse walk order,
+    // src/VorCell.cpp:471): entries base + p, or rows base + p of the
+    // cell's group (grouped; rows past the cell's own entries have area 0);
+    // positions below 0 of a chunk read position 0 and are skipped where
+    // used (u >= pb)
     const int lane = (int)(i & 63);
-    // KB records at a time: the records and neighbour states of a chunk are
-    // loaded before any is used, so a lane has KB independent loads in
-    // flight instead of one dependent chain
+    // KB records at a time: the records, then the neighbour states of a
+    // chunk are loaded before any is used (every neighbour row
+    // unconditionally: a mirror neighbour is the cell's own row, reflected
+    // afterwards), so a lane has KB independent loads in flight
     constexpr int KB = SFX_GRAD_KB;
     for(int64_t pb = n; pb > 0; pb -= KB) {
         R A[KB], c[KB][ND], rij[KB][ND], Wj[KB][NC];
@@ -692,10 +740,9 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
 #pragma unroll
         for(int u = 0; u < KB; u++) {
             const int64_t p = pb - 1 - u;
-            const bool ok = p >= 0;
             alignas(16) T rec[ER];
-            load_record<ND, T, GROUPED>(m, base, ok ? p : 0, lane, rec);
-            A[u] = ok ? (R)rec[0] : R(0);
+            load_record<ND, T, GROUPED>(m, base, p >= 0 ? p : 0, lane, rec);
+            A[u] = (R)rec[0];
 #pragma unroll
             for(int l = 0; l < ND; l++) {
                 c[u][l] = (R)rec[1 + l];
@@ -705,23 +752,24 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
         }
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            if(o[u] >= 0) {
+            const int64_t src = o[u] >= 0 ? (int64_t)o[u] : i;
 #pragma unroll
-                for(int k = 0; k < NC; k++) Wj[u][k] = (R)m.W[(int64_t)o[u] * NC + k];
-            } else {
+            for(int k = 0; k < NC; k++) Wj[u][k] = (R)m.W[src * NC + k];
+        }
+#pragma unroll
+        for(int u = 0; u < KB; u++) {
+            if(o[u] < 0) {
                 // mirror neighbour (:482-489); only a left cell has one, so
                 // rRL = -r_ij
                 R rr[ND];
 #pragma unroll
                 for(int l = 0; l < ND; l++) rr[l] = -rij[u][l];
-#pragma unroll
-                for(int k = 0; k < NC; k++) Wj[u][k] = W[k];
                 mirror_velocity<ND, R>(Wj[u] + 1, rr);
             }
         }
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            if(!(A[u] != R(0))) continue;  // also the padding
+            if(u >= pb || !(A[u] != R(0))) continue;  // also the padding
             const R rnorm =
                     (ND == 3) ? sqrt(rij[u][0] * rij[u][0] + rij[u][1] * rij[u][1] +
                                      rij[u][ND - 1] * rij[u][ND - 1])
@@ -746,7 +794,6 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
             }
         }
     }
-    const R V = (R)m.volume[i];
 #pragma unroll
     for(int l = ND; l--;)
 #pragma unroll
@@ -754,22 +801,33 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
     R alpha[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) alpha[k] = R(1);
-    for(int64_t p = n; p-- > 0;) {
-        alignas(16) T rec[ER];
-        load_record<ND, T, GROUPED>(m, base, p, lane, rec);
-        if(!((R)rec[0] != R(0))) continue;
-        R dd[ND];
+    // the limiter, KL records a time (their loads out together), same order
+    constexpr int KL = 4;
+    for(int64_t pb = n; pb > 0; pb -= KL) {
+        R Ah[KL], dd[KL][ND];
 #pragma unroll
-        for(int l = 0; l < ND; l++) dd[l] = (R)rec[1 + 2 * ND + l];
+        for(int u = 0; u < KL; u++) {
+            const int64_t p = pb - 1 - u;
+            alignas(16) T rec[ER];
+            load_record<ND, T, GROUPED>(m, base, p >= 0 ? p : 0, lane, rec);
+            Ah[u] = (R)rec[0];
 #pragma unroll
-        for(int k = 0; k < NC; k++) {
-            R dp = delta[0][k] * dd[0] + delta[1][k] * dd[1];
-            if(ND == 3) dp += delta[ND - 1][k] * dd[ND - 1];
-            R q1 = (dp != R(0)) ? (Wmax[k] - W[k]) / dp : R(1);
-            R q2 = (dp != R(0)) ? (Wmin[k] - W[k]) / dp : R(1);
-            alpha[k] = sminr<R>(alpha[k], smaxr<R>(q1, q2));
+            for(int l = 0; l < ND; l++) dd[u][l] = (R)rec[1 + 2 * ND + l];
+        }
+#pragma unroll
+        for(int u = 0; u < KL; u++) {
+            if(u >= pb || !(Ah[u] != R(0))) continue;
+#pragma unroll
+            for(int k = 0; k < NC; k++) {
+                R dp = delta[0][k] * dd[u][0] + delta[1][k] * dd[u][1];
+                if(ND == 3) dp += delta[ND - 1][k] * dd[u][ND - 1];
+                R q1 = (dp != R(0)) ? (Wmax[k] - W[k]) / dp : R(1);
+                R q2 = (dp != R(0)) ? (Wmin[k] - W[k]) / dp : R(1);
+                alpha[k] = sminr<R>(alpha[k], smaxr<R>(q1, q2));
+            }
         }
     }
+    if(!act) return;
 #pragma unroll
     for(int l = 0; l < ND; l++)
 #pragma unroll
@@ -782,15 +840,29 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
                                           int64_t gshift) {
     if constexpr(sizeof(T) == 4) {
         static_assert(DIFF, "FP32 records carry the difference-form weights");
-        if(m.egrp)
-            grad_cell_f32<ND, true>(m, i, grad, gshift);
-        else
-            grad_cell_f32<ND, false>(m, i, grad, gshift);
+        if(m.egrp) {
+            if(m.active)
+                grad_cell_f32<ND, true, true>(m, i, grad, gshift);
+            else
+                grad_cell_f32<ND, true, false>(m, i, grad, gshift);
+        } else {
+            if(m.active)
+                grad_cell_f32<ND, false, true>(m, i, grad, gshift);
+            else
+                grad_cell_f32<ND, false, false>(m, i, grad, gshift);
+        }
     } else {
-        if(m.egrp)
-            grad_cell_f64<ND, T, R, DIFF, true>(m, i, grad, gshift);
-        else
-            grad_cell_f64<ND, T, R, DIFF, false>(m, i, grad, gshift);
+        if(m.egrp) {
+            if(m.active)
+                grad_cell_f64<ND, T, R, DIFF, true, true>(m, i, grad, gshift);
+            else
+                grad_cell_f64<ND, T, R, DIFF, true, false>(m, i, grad, gshift);
+        } else {
+            if(m.active)
+                grad_cell_f64<ND, T, R, DIFF, false, true>(m, i, grad, gshift);
+            else
+                grad_cell_f64<ND, T, R, DIFF, false, false>(m, i, grad, gshift);
+        }
     }
 }
 
@@ -813,6 +885,9 @@ __device__ __forceinline__ void load_grad(const T* grad, int64_t C, int64_t cell
         for(int k = 0; k < NC; k++) g[l][k] = (R)grad[(cell * ND + l) * NC + k];
 }
 
+// The activity flag read for every cell of a mesh without an activity array.
+static __device__ const uint8_t kAllActive = 1;
+
 // MIXED: 0 = FP64 parity solvers, 1 = mixed solvers (and, for a periodic
 // face whose other copy has a lower index, no evaluation: the gather applies
 // that copy's flux with the opposite sign).
@@ -830,26 +905,77 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
     unsigned iters = 0, brent = 0, vac = 0, fail = 0, solved = 0;
     {
         const int64_t C = m.C, F = m.F;
+        (void)C;
+        // Every load goes out in two waves before the arithmetic: the face's
+        // own values, then its two cells' (a face without a right cell reads
+        // its left cell's in their place, unused).  A branch between loads
+        // (on activity, on a missing right cell) would make each later load
+        // wait for the earlier ones.
         const int32_t L = m.left[f], Rc = m.right[f];
         const bool has_r = Rc >= 0;
+        const int64_t Rs = has_r ? (int64_t)Rc : (int64_t)L;
+        const R A = (R)m.area[f];
+        R rr[ND], mcv[ND], dLf[ND], dRf[ND];
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            rr[l] = (R)m.rRL[l * F + f];
+            mcv[l] = (R)m.mc[l * F + f];
+            dLf[l] = (R)m.dL[l * F + f];
+            dRf[l] = (R)m.dR[l * F + f];
+        }
+        const T rcent_f = m.rcent[f];
+        // twin >= 0 marks a periodic copy (-1 on every other face); the left
+        // indices stand in for a missing twin array (never used)
+        const int32_t twin_f = (m.twin ? m.twin : m.left)[f];
+        const uint8_t actLb = *(m.active ? m.active + L : &kAllActive);
+        const uint8_t actRb = *(m.active ? m.active + Rs : &kAllActive);
+        R vl[ND], vr[ND];
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            vl[l] = (R)m.vel[(int64_t)L * ND + l];
+            vr[l] = (R)m.vel[Rs * ND + l];
+        }
+        const double dtL = m.dt[L], dtR = m.dt[Rs];
+        const double massL = m.mass[L], massR = m.mass[Rs];
+        const double areaL = m.total_area[L], areaR = m.total_area[Rs];
+        R WL[NC], WR[NC];
+#pragma unroll
+        for(int k = 0; k < NC; k++) {
+            WL[k] = (R)m.W[(int64_t)L * NC + k];
+            WR[k] = (R)m.W[Rs * NC + k];
+        }
+        using GT = std::conditional_t<MIXED != 0, float, R>;
+        GT gL[ND][NC], gR[ND][NC];
+        load_grad<ND, T, GT>(grad, C, L + gshift, gL);
+        load_grad<ND, T, GT>(grad, C, Rs + gshift, gR);
+        // the skip test below needs only the flags, but a branch on them
+        // would let the compiler move the other loads behind it
+#pragma unroll
+        for(int k = 0; k < NC; k++) asm volatile("" ::"v"(WL[k]), "v"(WR[k]));
+#pragma unroll
+        for(int l = 0; l < ND; l++)
+#pragma unroll
+            for(int k = 0; k < NC; k++) asm volatile("" ::"v"(gL[l][k]), "v"(gR[l][k]));
+        asm volatile("" ::"v"(dtL), "v"(dtR), "v"(massL), "v"(massR), "v"(areaL), "v"(areaR));
         uint8_t st = 0;
         double out[NC];
 #pragma unroll
         for(int k = 0; k < NC; k++) out[k] = 0.0;
-        const R A = (R)m.area[f];
-        const bool actL = m.active ? m.active[L] != 0 : true;
-        const bool actR = has_r ? (m.active ? m.active[Rc] != 0 : true) : false;
+        const bool actL = actLb != 0;
+        const bool actR = has_r && actRb != 0;
         bool skip = !(A != R(0)) || (!has_r && !actL) || (has_r && !actR && !actL);
-        if(MIXED && m.twin && m.ghost[f] && m.twin[f] >= 0 && m.twin[f] < f) skip = true;
+        // the lower-index copy of a periodic pair is evaluated and writes
+        // both rows (the other one negated: that copy's left cell is this
+        // one's right cell); the higher-index copy writes nothing
+        const bool twin_skip = MIXED && m.twin && twin_f >= 0 && twin_f < f;
+        const bool twin_write = MIXED && m.twin && twin_f > f;
+        if(twin_skip) skip = true;
         if(!skip) {
             solved = 1;
-            R rr[ND], mcv[ND], vl[ND], vr[ND], vface[ND];
+            R vface[ND];
+            if(!has_r) {
 #pragma unroll
-            for(int l = 0; l < ND; l++) {
-                rr[l] = (R)m.rRL[l * F + f];
-                mcv[l] = (R)m.mc[l * F + f];
-                vl[l] = (R)m.vel[(int64_t)L * ND + l];
-                vr[l] = has_r ? (R)m.vel[(int64_t)Rc * ND + l] : R(0);
+                for(int l = 0; l < ND; l++) vr[l] = R(0);
             }
             // VorFace::set_v (src/VorFace.cpp:586-660)
             if(has_r || ND == 2) {
@@ -874,53 +1000,33 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                 for(int l = 0; l < ND; l++) vface[l] = R(0);
             }
             // dt, :695-699
-            double dt = m.dt[L];
-            if(has_r) dt = smin(dt, m.dt[Rc]);
-            R WL[NC], WR[NC];
+            double dt = dtL;
+            if(has_r) dt = smin(dt, dtR);
+            if(!has_r) {
 #pragma unroll
-            for(int k = 0; k < NC; k++) {
-                WL[k] = (R)m.W[(int64_t)L * NC + k];
-                WR[k] = has_r ? (R)m.W[(int64_t)Rc * NC + k] : R(0);
+                for(int k = 0; k < NC; k++) WR[k] = R(0);
             }
 #pragma unroll
             for(int l = 0; l < ND; l++) {
                 WL[1 + l] -= vface[l];
                 WR[1 + l] -= vface[l];
             }
-            const R rc = has_r ? (R)m.rcent[f] : R(0);
+            const R rc = has_r ? (R)rcent_f : R(0);
             R WRrec[NC], WLrec[NC];
             const R gam = (R)cd.gamma;
             if constexpr(MIXED) {
-                const float gf = (float)cd.gamma, rcf = (float)m.rcent[f];
-                if(has_r) {
-                    float g[ND][NC], d[ND];
-                    load_grad<ND, T, float>(grad, C, Rc + gshift, g);
+                const float gf = (float)cd.gamma, rcf = (float)rcent_f;
+                float dRs[ND], dLs[ND];
 #pragma unroll
-                    for(int l = 0; l < ND; l++) d[l] = (float)m.dR[l * F + f];
-                    predict_side_offset<ND>(gf, WR, WL, g, d, rcf, true, dt, WRrec);
+                for(int l = 0; l < ND; l++) {
+                    dRs[l] = (float)dRf[l];
+                    dLs[l] = (float)dLf[l];
                 }
-                {
-                    float g[ND][NC], d[ND];
-                    load_grad<ND, T, float>(grad, C, L + gshift, g);
-#pragma unroll
-                    for(int l = 0; l < ND; l++) d[l] = (float)m.dL[l * F + f];
-                    predict_side_offset<ND>(gf, WL, WR, g, d, rcf, has_r, dt, WLrec);
-                }
+                if(has_r) predict_side_offset<ND>(gf, WR, WL, gR, dRs, rcf, true, dt, WRrec);
+                predict_side_offset<ND>(gf, WL, WR, gL, dLs, rcf, has_r, dt, WLrec);
             } else {
-                if(has_r) {
-                    R g[ND][NC], d[ND];
-                    load_grad<ND, T, R>(grad, C, Rc + gshift, g);
-#pragma unroll
-                    for(int l = 0; l < ND; l++) d[l] = (R)m.dR[l * F + f];
-                    predict_side<ND, R>(gam, WR, WL, g, d, rc, true, (R)dt, WRrec);
-                }
-                {
-                    R g[ND][NC], d[ND];
-                    load_grad<ND, T, R>(grad, C, L + gshift, g);
-#pragma unroll
-                    for(int l = 0; l < ND; l++) d[l] = (R)m.dL[l * F + f];
-                    predict_side<ND, R>(gam, WL, WR, g, d, rc, has_r, (R)dt, WLrec);
-                }
+                if(has_r) predict_side<ND, R>(gam, WR, WL, gR, dRf, rc, true, (R)dt, WRrec);
+                predict_side<ND, R>(gam, WL, WR, gL, dLf, rc, has_r, (R)dt, WLrec);
             }
             if(WL[0] != R(0)) {
 #pragma unroll
@@ -997,23 +1103,23 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
             };
             double timefac = 1.;
             if(has_r) {
-                double dtmax = m.dt[L];
-                dtmax = smax(dtmax, m.dt[Rc]);
+                double dtmax = dtL;
+                dtmax = smax(dtmax, dtR);
                 timefac = quot(dt, dtmax);
             }
-            double areafac = quot((double)A, m.total_area[L]);
+            double areafac = quot((double)A, areaL);
             areafac *= timefac;
-            if(dQ[0] > areafac * m.mass[L]) {
-                double corfac = fabs(quot(areafac * m.mass[L], dQ[0]));
+            if(dQ[0] > areafac * massL) {
+                double corfac = fabs(quot(areafac * massL, dQ[0]));
 #pragma unroll
                 for(int k = 0; k < NC; k++) dQ[k] *= corfac;
             }
             if(has_r) {
-                double af = quot((double)A, m.total_area[Rc]);
+                double af = quot((double)A, areaR);
                 af *= timefac;
                 // dQR[0] = -dQL[0]
-                if(-dQ[0] > af * m.mass[Rc]) {
-                    double corfac = fabs(quot(af * m.mass[Rc], -dQ[0]));
+                if(-dQ[0] > af * massR) {
+                    double corfac = fabs(quot(af * massR, -dQ[0]));
 #pragma unroll
                     for(int k = 0; k < NC; k++) dQ[k] = -(-dQ[k] * corfac);
                 }
@@ -1023,8 +1129,14 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
             for(int k = 0; k < NC; k++) out[k] = dQ[k];
             fail = st != 0;
         }
+        if(!twin_skip) {
 #pragma unroll
-        for(int k = 0; k < NC; k++) dQL[(f + qshift) * NC + k] = (T)out[k];
+            for(int k = 0; k < NC; k++) dQL[(f + qshift) * NC + k] = (T)out[k];
+        }
+        if(twin_write) {
+#pragma unroll
+            for(int k = 0; k < NC; k++) dQL[((int64_t)twin_f + qshift) * NC + k] = (T)(-out[k]);
+        }
         if(fstatus) fstatus[f] = st;
     }
     cnt[CNT_NEVAL] += solved;
@@ -1058,6 +1170,10 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
                                             const T* __restrict__ dQL, int64_t qshift,
                                             double* __restrict__ dQ) {
     const int NC = ND + 2;
+    // every row of dQL holds its face's flux seen from the left cell (the
+    // mixed face pass writes both copies of a periodic pair): no twins here
+    (void)ghost;
+    (void)twin;
     double acc[NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) acc[k] = 0.0;
@@ -1065,7 +1181,7 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
     // SFX_GATHER_KB entries at a time: their indices, then their rows, are
     // loaded before any is added, so a lane has that many independent loads
     // in flight; the additions keep ascending face order
-    constexpr int KB = SFX_GATHER_KB;
+    constexpr int KB = sizeof(T) == 8 ? SFX_GATHER_KB_F64 : SFX_GATHER_KB;
     for(int64_t jb = j0; jb < j1; jb += KB) {
         int32_t f[KB];
         bool neg[KB];
@@ -1075,28 +1191,22 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
             f[u] = ent_face(e);
             neg[u] = ent_right(e);
         }
-        if(MIXED && twin) {
-#pragma unroll
-            for(int u = 0; u < KB; u++) {
-                if(jb + u < j1 && ghost[f[u]]) {
-                    const int32_t t = twin[f[u]];
-                    if(t >= 0 && t < f[u]) {
-                        f[u] = t;          // the other copy carries this face's flux
-                        neg[u] = !neg[u];  // seen from its right cell
-                    }
-                }
-            }
-        }
         double x[KB][NC];
 #pragma unroll
         for(int u = 0; u < KB; u++)
 #pragma unroll
             for(int k = 0; k < NC; k++) x[u][k] = (double)dQL[((int64_t)f[u] + qshift) * NC + k];
+        // entries past the cell's last keep acc by a select, not a branch:
+        // a branch here would let each row's load sink behind the previous
+        // row's addition
 #pragma unroll
         for(int u = 0; u < KB; u++) {
-            if(jb + u >= j1) break;
+            const bool valid = jb + u < j1;
 #pragma unroll
-            for(int k = 0; k < NC; k++) acc[k] += neg[u] ? -x[u][k] : x[u][k];
+            for(int k = 0; k < NC; k++) {
+                const double sum = acc[k] + (neg[u] ? -x[u][k] : x[u][k]);
+                acc[k] = valid ? sum : acc[k];
+            }
         }
     }
 #pragma unroll

@@ -71,8 +71,9 @@ def fluxpass_errors(b, gamma, dQL_ref, dQ_ref, dQL, dQ, once_per_pair=False):
         cancel, so |dQ| itself is no scale);
       cell_vs_maxdQ: max |dQ - dQ_ref| / max |dQ_ref|;
       conservation: max_k |sum_i dQ_ki| / max |dQL_ref|.
-    once_per_pair: the pass evaluated each periodic face pair once and left
-    the second copy's dQL unused; it is compared through its twin."""
+    once_per_pair: the pass evaluated each periodic face pair once; the
+    second copy's dQL is taken as its twin's, negated (what the mixed pass
+    writes there)."""
     dQL_ref, dQ_ref, dQL, dQ = map(_np, (dQL_ref, dQ_ref, dQL, dQ))
     F = dQL.shape[1]
     if once_per_pair:

@@ -89,13 +89,20 @@ def test_fp64_pass_matches_oracle(device, name, kind):
     assert errc <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("kind", [0, 1])
-def test_gather_is_ordered_sum_of_gpu_face_fluxes(device, kind):
+def test_gather_is_ordered_sum_of_gpu_face_fluxes(device, kind, prec):
     """dQ on the GPU equals, bit for bit, the sum of the GPU's own dQL in the
-    reference's face-loop order (left += dQL, right += -dQL)."""
+    reference's face-loop order (left += dQL, right += -dQL), summed in FP64.
+    The mixed pass evaluates each periodic pair once and writes the other
+    copy's row as the exact negation."""
     b = kh_bundle(32, kind)
-    g = run_gpu(b, kind, "fp64", device)
-    dQL = g["dQL"]
+    g = run_gpu(b, kind, prec, device)
+    dQL = np.asarray(g["dQL"], np.float64)
+    if prec == "mixed":
+        gh = np.where(b["ghost"] == 1)[0]
+        assert len(gh) > 0
+        assert np.array_equal(dQL[:, gh], -dQL[:, b["twin"][gh]])
     acc = np.zeros_like(g["dQ"])
     for f in range(b["F"]):
         acc[:, b["left"][f]] += dQL[:, f]

@@ -66,7 +66,7 @@ def test_volumes_closure_and_face_contract(nd):
     assert np.all((owner == L[cf]) | ((owner == R[cf]) & (gh[cf] == 0)))
     assert np.all(L[gh == 0] < R[gh == 0])
     g = np.where(gh == 1)[0]
-    assert np.all(tw[g] >= 0)
+    assert np.all(tw[g] >= 0) and np.all(tw[gh == 0] == -1)
     assert np.array_equal(tw[tw[g]], g)
     assert np.array_equal(L[tw[g]], R[g]) and np.array_equal(R[tw[g]], L[g])
     assert np.array_equal(m["shift"][:, tw[g]], -m["shift"][:, g])
