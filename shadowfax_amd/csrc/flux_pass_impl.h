@@ -44,6 +44,18 @@
 #ifndef SFX_GRAD_KB_2D  // FP32-record gradients of 2D meshes (about 6 entries a cell)
 #define SFX_GRAD_KB_2D 8
 #endif
+#ifndef SFX_GRAD_KN_2D  // FP32-record gradients of 2D meshes: neighbour rows in flight
+#define SFX_GRAD_KN_2D 4
+#endif
+#ifndef SFX_GRAD_KEEP_2D  // 2D: hold the records' second halves for the limiter
+#define SFX_GRAD_KEEP_2D 0
+#endif
+#ifndef SFX_GRAD_KL_3D  // FP32-record limiter loop in 3D: records per chunk
+#define SFX_GRAD_KL_3D 4
+#endif
+#ifndef SFX_GRAD_PIPE  // FP32-record gradients: next chunk's records in flight
+#define SFX_GRAD_PIPE 1
+#endif
 #ifndef SFX_GATHER_KB
 #define SFX_GATHER_KB 8
 #endif
@@ -102,10 +114,12 @@ struct MeshView {
 __host__ __device__ constexpr int entry_rec(int ND, bool f32) {
     return f32 ? 8 : (ND == 2 ? 8 : 12);
 }
-// field offsets of the FP32 record
+// field offsets of the FP32 record: g, the neighbour, d, A; a 2D record
+// carries A a second time in its first half (A2), so the gradient loop reads
+// only first halves
 template <int ND>
 struct Rec32 {
-    static constexpr int G = 0, NB = ND, D = 4, A = 4 + ND;
+    static constexpr int G = 0, NB = ND, D = 4, A = 4 + ND, A2 = (ND == 2) ? 3 : A;
 };
 
 // Row o of an item-major FP64 array of NC = 4 or 5 values per item
@@ -216,6 +230,18 @@ __device__ __forceinline__ void mirror_velocity(R* vel, const R* rRL) {
     }
 }
 
+// a / b: IEEE division in FP64 (the parity pass); in FP32 (the mixed pass's
+// offsets) the hardware reciprocal, about 1 ulp, without the denormal
+// scaling the compiler wraps around an FP32 divide
+template <typename R>
+__device__ __forceinline__ R quot_r(R a, R b) {
+    return a / b;
+}
+template <>
+__device__ __forceinline__ float quot_r<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 // get_time_derivative (ExactRiemannSolver.cpp:892-934 / HLLCRiemannSolver.cpp:
 // 415-450); g[d][k] = dW_k/dx_d.  The 2D dWdt[2] uses gradients[1].vy()
 // twice, as the reference does (:920 / :443).
@@ -227,17 +253,17 @@ __device__ __forceinline__ void time_derivative(R gamma, const R* W, const R (*g
         const R* gy = g[1];
         const R* gz = g[ND - 1];
         dWdt[0] = -W[0] * (gx[1] + gy[2] + gz[3]) - W[1] * gx[0] - W[2] * gy[0] - W[3] * gz[0];
-        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - W[3] * gz[1] - gx[4] / W[0];
-        dWdt[2] = -W[1] * gx[2] - W[2] * gy[2] - W[3] * gz[2] - gy[4] / W[0];
-        dWdt[3] = -W[1] * gx[3] - W[2] * gy[3] - W[3] * gz[3] - gz[4] / W[0];
+        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - W[3] * gz[1] - quot_r(gx[4], W[0]);
+        dWdt[2] = -W[1] * gx[2] - W[2] * gy[2] - W[3] * gz[2] - quot_r(gy[4], W[0]);
+        dWdt[3] = -W[1] * gx[3] - W[2] * gy[3] - W[3] * gz[3] - quot_r(gz[4], W[0]);
         dWdt[ND + 1] = -gamma * W[4] * (gx[1] + gy[2] + gz[3]) - W[1] * gx[4] - W[2] * gy[4] -
                        W[3] * gz[4];
     } else {
         const R* gx = g[0];
         const R* gy = g[1];
         dWdt[0] = -W[1] * gx[0] - W[2] * gy[0] - W[0] * (gx[1] + gy[2]);
-        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - gx[3] / W[0];
-        dWdt[2] = -W[1] * gy[2] - W[2] * gy[2] - gy[3] / W[0];
+        dWdt[1] = -W[1] * gx[1] - W[2] * gy[1] - quot_r(gx[3], W[0]);
+        dWdt[2] = -W[1] * gy[2] - W[2] * gy[2] - quot_r(gy[3], W[0]);
         dWdt[ND + 1] = -gamma * W[3] * (gx[1] + gy[2]) - W[1] * gx[3] - W[2] * gy[3];
     }
 }
@@ -345,12 +371,12 @@ __device__ __forceinline__ void predict_side_offset(float gamma, const double* W
             if(phiL < phiR) {
                 const float om = ((phiL - d1d) * phiL > 0.0) ? -d1
                                  : (pl == 0.f)               ? 0.f
-                                                             : -(d1 * pl) / (pl + d1);
+                                                             : -(d1 * pl) * mixed::rcpf(pl + d1);
                 off[k] = fmaxf(om, fminf((float)(t * Dd) + d2, off[k]));
             } else {
                 const float op = ((phiL + d1d) * phiL > 0.0) ? d1
                                  : (pl == 0.f)               ? 0.f
-                                                             : (d1 * -pl) / (d1 - pl);
+                                                             : (d1 * -pl) * mixed::rcpf(d1 - pl);
                 off[k] = fminf(op, fmaxf((float)(t * Dd) - d2, off[k]));
             }
         }
@@ -401,6 +427,7 @@ __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32
     const int32_t nb = isR ? m.left[f] : m.right[f];
     rec[RL::NB] = index_bits(nb >= 0 ? nb : ~f, 0.f);
     rec[RL::A] = m.area[f];
+    rec[RL::A2] = m.area[f];
 }
 
 // The FP64 record of cell entry e (entry_rec(ND, false)).
@@ -576,8 +603,8 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     // records stay in registers for the limiter loop.  3D: chunk after
     // chunk, the next chunk's records in flight while this chunk's
     // neighbour states load.
-    constexpr bool KEEP = ND == 2;
-    constexpr bool PIPE = ND == 3;
+    constexpr bool KEEP = ND == 2 && KB >= 8 && SFX_GRAD_KEEP_2D;
+    constexpr bool PIPE = ND == 3 && SFX_GRAD_PIPE;
     const bool one_chunk = KEEP && n > 0 && n <= KB;
     float4 lo[KB], hi[KB];
     if(n > 0) load_chunk(n, lo, hi);
@@ -585,47 +612,66 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         // every neighbour row is loaded unconditionally (a mirror neighbour
         // is the cell's own row, reflected below), so the loads go out back
         // to back
-        double Wj[KB][NC];
-        int32_t nbo[KB];
-#pragma unroll
-        for(int u = 0; u < KB; u++) {
-            nbo[u] = bits_index((ND == 3) ? lo[u].w : lo[u].z);
-            load_row<NC>(m.W, nbo[u] >= 0 ? (int64_t)nbo[u] : i, m.C, Wj[u]);
-        }
+        // the neighbour rows in batches of KN (all KB at once in 3D; two
+        // batches of 4 in 2D, which holds the registers to four waves per
+        // SIMD at the cost of one more load trip)
+        constexpr int KN = (ND == 2) ? SFX_GRAD_KN_2D : KB;
         float4 nlo[PIPE ? KB : 1], nhi[PIPE ? KB : 1];
-        if constexpr(PIPE) load_chunk(pb - KB, nlo, nhi);
-        // mirror neighbours (src/VorCell.cpp:482-489) of face ~o: only a left
-        // cell has one, so rRL is seen from it
 #pragma unroll
-        for(int u = 0; u < KB; u++) {
-            if(nbo[u] < 0) {
-                const int32_t f = ~nbo[u];
-                double rr[ND];
+        for(int u0 = 0; u0 < KB; u0 += KN) {
+            // every neighbour row is loaded unconditionally (a mirror
+            // neighbour is the cell's own row, reflected below), so the
+            // loads go out back to back
+            double Wj[KN][NC];
+            int32_t nbo[KN];
 #pragma unroll
-                for(int l = 0; l < ND; l++) rr[l] = (double)m.rRL[l * m.F + f];
-                mirror_velocity<ND, double>(Wj[u] + 1, rr);
+            for(int v = 0; v < KN; v++) {
+                const int u = u0 + v;
+                nbo[v] = bits_index((ND == 3) ? lo[u].w : lo[u].z);
+                load_row<NC>(m.W, nbo[v] >= 0 ? (int64_t)nbo[v] : i, m.C, Wj[v]);
             }
-        }
-        if constexpr(KEEP) {
-            // the limiter's halves must be in registers by now, not read
-            // again after this loop
-#pragma unroll
-            for(int u = 0; u < KB; u++) asm volatile("" ::"v"(hi[u].x), "v"(hi[u].y));
-        }
-#pragma unroll
-        for(int u = 0; u < KB; u++) {
-            const float A = (ND == 3) ? hi[u].w : hi[u].z;  // 0: a padding row
-            if(u >= pb || !(A != 0.f)) continue;
-            const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
-                                  (ND == 3) ? (double)lo[u].z : 0.0};
-#pragma unroll
-            for(int k = 0; k < NC; k++) {
-                const double dk = Wj[u][k] - W[k];
-#pragma unroll
-                for(int l = 0; l < ND; l++) delta[l][k] += dk * gl[l];
-                Wmax[k] = smaxr<double>(Wmax[k], Wj[u][k]);
-                Wmin[k] = sminr<double>(Wmin[k], Wj[u][k]);
+            if constexpr(PIPE) {
+                if(u0 == 0) load_chunk(pb - KB, nlo, nhi);
             }
+            // mirror neighbours (src/VorCell.cpp:482-489) of face ~o: only a
+            // left cell has one, so rRL is seen from it
+#pragma unroll
+            for(int v = 0; v < KN; v++) {
+                if(nbo[v] < 0) {
+                    const int32_t f = ~nbo[v];
+                    double rr[ND];
+#pragma unroll
+                    for(int l = 0; l < ND; l++) rr[l] = (double)m.rRL[l * m.F + f];
+                    mirror_velocity<ND, double>(Wj[v] + 1, rr);
+                }
+            }
+            if constexpr(KEEP) {
+                // the limiter's halves must be in registers by now, not
+                // read again after this loop
+                if(u0 == 0) {
+#pragma unroll
+                    for(int u = 0; u < KB; u++) asm volatile("" ::"v"(hi[u].x), "v"(hi[u].y));
+                }
+            }
+#pragma unroll
+            for(int v = 0; v < KN; v++) {
+                const int u = u0 + v;
+                // A (0: a padding row) from the first half in 2D
+                const float A = (ND == 3) ? hi[u].w : KEEP ? hi[u].z : lo[u].w;
+                if(u >= pb || !(A != 0.f)) continue;
+                const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
+                                      (ND == 3) ? (double)lo[u].z : 0.0};
+#pragma unroll
+                for(int k = 0; k < NC; k++) {
+                    const double dk = Wj[v][k] - W[k];
+#pragma unroll
+                    for(int l = 0; l < ND; l++) delta[l][k] += dk * gl[l];
+                    Wmax[k] = smaxr<double>(Wmax[k], Wj[v][k]);
+                    Wmin[k] = sminr<double>(Wmin[k], Wj[v][k]);
+                }
+            }
+            // the next batch's loads stay behind this batch's use
+            if constexpr(KN < KB) asm volatile("" ::: "memory");
         }
         if constexpr(PIPE) {
 #pragma unroll
@@ -664,25 +710,20 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     if(one_chunk) {
 #pragma unroll
         for(int u = 0; u < KB; u++) limit(hi[u], u < n);
-    } else if(n > 0) {
-        constexpr int KL = (ND == 2) ? SFX_GRAD_KB_2D : 4;
-        float4 h[KL];
-        auto load_halves = [&](int64_t pb, float4* H) {
+    } else {
+        // KL halves a time, their loads out together (a pipelined loop, the
+        // next chunk loading while this one is used, saves little here: the
+        // work per chunk is short next to a load's latency)
+        constexpr int KL = (ND == 2) ? 8 : SFX_GRAD_KL_3D;
+        for(int64_t pb = n; pb > 0; pb -= KL) {
+            float4 h[KL];
 #pragma unroll
             for(int u = 0; u < KL; u++) {
                 const int64_t p = pb - 1 - u;
-                H[u] = half(p >= 0 ? p : 0, 1);
+                h[u] = half(p >= 0 ? p : 0, 1);
             }
-        };
-        load_halves(n, h);
-        for(int64_t pb = n; pb > 0; pb -= KL) {
-            // the next chunk's halves load while this one is used
-            float4 nh[KL];
-            load_halves(pb - KL, nh);
 #pragma unroll
             for(int u = 0; u < KL; u++) limit(h[u], u < pb);
-#pragma unroll
-            for(int u = 0; u < KL; u++) h[u] = nh[u];
         }
     }
     if(!act) return;
