@@ -12,9 +12,9 @@ template <>
 hipError_t launch_entry_geometry<double>(int ndim, const flux::MeshView<double>& m, int64_t E,
                                      double* egeo, hipStream_t s) {
     if(m.egrp) {
-        // grouped layout: one lane per cell, the last group padded to 64
+        // grouped layout: a block per group of 64 cells (the last one padded)
         if(m.C <= 0) return hipSuccess;
-        const dim3 g((unsigned)(((m.C + 63) / 64 * 64 + 255) / 256));
+        const dim3 g((unsigned)((m.C + 63) / 64));  // one block per group
         if(ndim == 3)
             hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, double>), g, dim3(256), 0,
                                s, m, egeo);

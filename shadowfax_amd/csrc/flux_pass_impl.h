@@ -503,29 +503,49 @@ __device__ __forceinline__ void load_record(const MeshView<T>& m, int64_t base, 
     }
 }
 
-// One lane per cell writes its column of its group's rows.
+// One block per group of 64 cells (XCD-aware order).  The block's threads
+// walk the group's entries in entry order (the face reads of neighbouring
+// threads then fall on neighbouring faces, as for entry_geometry_kernel):
+// entry j's cell is the last l with off[l] <= j among the group's 65 offsets
+// (held in LDS; a cell without entries never matches), its row
+// r0 + j - off[l], its lane l.  A second loop writes zeros to the rows past
+// each cell's last entry.
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T> m,
                                                                      T* __restrict__ egeo) {
     constexpr int ER = entry_rec<ND, T>(), P = rec_planes<ND, T>();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t g = i >> 6;
-    const int lane = (int)(i & 63);
-    if(g >= (m.C + 63) / 64) return;
+    __shared__ int64_t off[65];
+    const int64_t g = xcd_block();
+    const int64_t c0 = g * 64;
+    const int nc = (int)((m.C - c0) < 64 ? (m.C - c0) : 64);
+    if((int)threadIdx.x <= nc) off[threadIdx.x] = m.cf_off[c0 + threadIdx.x];
+    __syncthreads();
     const int64_t r0 = m.egrp[g], K = m.egrp[g + 1] - r0;
-    const int64_t n = (i < m.C) ? m.cf_off[i + 1] - m.cf_off[i] : 0;
+    const int64_t j0 = off[0], j1 = off[nc];
     float4* out = reinterpret_cast<float4*>(egeo);
-    for(int64_t k = 0; k < K; k++) {
-        alignas(16) T rec[ER];
-        if(k < n) {
-            entry_record<ND, T>(m, m.cf_entry[m.cf_off[i] + k], rec);
-        } else {
-#pragma unroll
-            for(int q = 0; q < ER; q++) rec[q] = T(0);
+    for(int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        int lo = 0, hi = nc;  // off[lo] <= j < off[hi]
+        while(hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if(off[mid] <= j)
+                lo = mid;
+            else
+                hi = mid;
         }
+        alignas(16) T rec[ER];
+        entry_record<ND, T>(m, m.cf_entry[j], rec);
+        const int64_t r = r0 + (j - off[lo]);
 #pragma unroll
         for(int q = 0; q < P; q++)
-            out[((r0 + k) * P + q) * 64 + lane] = reinterpret_cast<const float4*>(rec)[q];
+            out[(r * P + q) * 64 + lo] = reinterpret_cast<const float4*>(rec)[q];
+    }
+    for(int64_t sl = threadIdx.x; sl < K * 64; sl += blockDim.x) {
+        const int64_t k = sl >> 6;
+        const int lane = (int)(sl & 63);
+        const int64_t n = lane < nc ? off[lane + 1] - off[lane] : 0;
+        if(k < n) continue;
+#pragma unroll
+        for(int q = 0; q < P; q++) out[((r0 + k) * P + q) * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
