@@ -502,6 +502,56 @@ int sfx_ensemble_serve(sfx_ensemble* e, int mode, int64_t nreq,
                        const double* arrival_s, const int32_t* mesh_id,
                        double* latency_s, int32_t* batch_of, sfx_serve_stats* st);
 
+/* ---- snapshots: HDF5 files of Gadget-style initial conditions ------- */
+
+/* Replaces the HDF5-library reads of GadgetSnapshotReader::read_snapshot
+ * (src/GadgetSnapshotReader.cpp:79-380, through src/io/HDF5tools.hpp) with a
+ * self-contained reader of the structures such files use: superblock
+ * version 0, symbol-table groups, version 1 object headers, contiguous or
+ * compact datasets, attributes.  Chunked or filtered datasets are refused
+ * with an error.  Host only; paths are absolute ("/PartType0/Coordinates"). */
+typedef struct sfx_h5 sfx_h5;
+int sfx_h5_open(const char* path, sfx_h5** out);
+int sfx_h5_close(sfx_h5* f);
+/* 1 if the object (and, attr != NULL, its attribute) exists, 0 if not */
+int sfx_h5_exists(const sfx_h5* f, const char* path, const char* attr);
+/* A dataset (attr NULL) or an attribute of the object at path:
+ * type_class 0 integer, 1 floating point, 8 enumeration (boolean),
+ * -1 other; type_size in bytes; rank and dims[rank] (rank <= 8). */
+int sfx_h5_info(const sfx_h5* f, const char* path, const char* attr, int* type_class,
+                int* type_size, int* rank, int64_t* dims);
+/* All n values (n must equal the element count) converted to double or
+ * int64. */
+int sfx_h5_read_f64(const sfx_h5* f, const char* path, const char* attr, int64_t n,
+                    double* out);
+int sfx_h5_read_i64(const sfx_h5* f, const char* path, const char* attr, int64_t n,
+                    int64_t* out);
+/* The members of the group at path and its attributes ("@name"), one per
+ * line, NUL-terminated; returns the bytes needed when cap is too small. */
+int sfx_h5_list(const sfx_h5* f, const char* path, char* names, int64_t cap);
+
+/* Writing such files (what GadgetSnapshotWriter writes through the HDF5
+ * library, src/GadgetSnapshotWriter.cpp:220-500): build a tree of groups,
+ * contiguous datasets and attributes in memory, then save it in the same
+ * structures the reader above reads (superblock version 0, symbol-table
+ * groups of at most 256 members, version 1 object headers).  Types:
+ * SFX_H5_UINT / SFX_H5_INT (1, 2, 4 or 8 bytes), SFX_H5_FLOAT (4 or 8);
+ * data little-endian, row-major. */
+#define SFX_H5_UINT 0
+#define SFX_H5_FLOAT 1
+#define SFX_H5_INT 2
+typedef struct sfx_h5w sfx_h5w;
+int sfx_h5w_create(sfx_h5w** out);
+int sfx_h5w_destroy(sfx_h5w* w);
+/* a group at an absolute path (its parent must exist) */
+int sfx_h5w_group(sfx_h5w* w, const char* path);
+int sfx_h5w_dataset(sfx_h5w* w, const char* path, int type, int type_size, int rank,
+                    const int64_t* dims, const void* data);
+/* an attribute of the group or dataset at path (rank 0: a scalar) */
+int sfx_h5w_attr(sfx_h5w* w, const char* path, const char* name, int type, int type_size,
+                 int rank, const int64_t* dims, const void* data);
+int sfx_h5w_save(sfx_h5w* w, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,7 @@ SOURCES = [
     ("ensemble_fp64.hip", ["-ffp-contract=off"]),
     ("ensemble_mixed.hip", ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sfx_ensemble.cpp", []),
+    ("snapshot_hdf5.cpp", []),
 ]
 
 

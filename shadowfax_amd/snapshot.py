@@ -1,0 +1,317 @@
+"""Gadget-style HDF5 snapshots and initial conditions.
+
+read_gadget_snapshot mirrors GadgetSnapshotReader::read_snapshot
+(src/GadgetSnapshotReader.cpp:79-560, GadgetSnapshotReader.hpp:49-58) and
+write_gadget_snapshot the files GadgetSnapshotWriter writes
+(src/GadgetSnapshotWriter.cpp:220-500).  The HDF5 structures are parsed and
+written by the native sfx_h5 / sfx_h5w entry points (include/shadowfax_amd.h);
+there is no dependency on an HDF5 library.
+
+Values are returned in the file's own units; the file's /Units block (cgs
+factors), when present, is returned alongside (the reference converts to its
+run's units through UnitConverter, which is outside this package).
+"""
+import ctypes as ct
+
+import numpy as np
+
+from . import _capi as C
+
+vp = ct.c_void_p
+i64 = ct.c_int64
+_ip = ct.POINTER(ct.c_int)
+_i64p = ct.POINTER(ct.c_int64)
+
+SFX_H5_UINT, SFX_H5_FLOAT, SFX_H5_INT = 0, 1, 2
+PAQ_NAMES = ("Dye", "Fe", "Mg")  # src/StateVector.hpp:39-40
+UNIT_NAMES = {"U_I": "Unit current in cgs (U_I)", "U_L": "Unit length in cgs (U_L)",
+              "U_M": "Unit mass in cgs (U_M)", "U_T": "Unit temperature in cgs (U_T)",
+              "U_t": "Unit time in cgs (U_t)"}
+
+
+def _lib():
+    L = C.lib()
+    if getattr(L, "_h5_sigs", False):
+        return L
+    L.sfx_h5_open.argtypes = [ct.c_char_p, ct.POINTER(vp)]
+    L.sfx_h5_close.argtypes = [vp]
+    L.sfx_h5_exists.argtypes = [vp, ct.c_char_p, ct.c_char_p]
+    L.sfx_h5_info.argtypes = [vp, ct.c_char_p, ct.c_char_p, _ip, _ip, _ip, _i64p]
+    L.sfx_h5_read_f64.argtypes = [vp, ct.c_char_p, ct.c_char_p, i64, vp]
+    L.sfx_h5_read_i64.argtypes = [vp, ct.c_char_p, ct.c_char_p, i64, vp]
+    L.sfx_h5_list.argtypes = [vp, ct.c_char_p, ct.c_char_p, i64]
+    L.sfx_h5w_create.argtypes = [ct.POINTER(vp)]
+    L.sfx_h5w_destroy.argtypes = [vp]
+    L.sfx_h5w_group.argtypes = [vp, ct.c_char_p]
+    L.sfx_h5w_dataset.argtypes = [vp, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, _i64p, vp]
+    L.sfx_h5w_attr.argtypes = [vp, ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int,
+                               _i64p, vp]
+    L.sfx_h5w_save.argtypes = [vp, ct.c_char_p]
+    L._h5_sigs = True
+    return L
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+class H5File:
+    """A read-only HDF5 file (superblock version 0, contiguous or compact
+    datasets).  Paths are absolute: "/PartType0/Coordinates"."""
+
+    def __init__(self, path):
+        self.L = _lib()
+        self.h = vp()
+        C.check(self.L.sfx_h5_open(str(path).encode(), ct.byref(self.h)), "sfx_h5_open")
+
+    def close(self):
+        if self.h:
+            self.L.sfx_h5_close(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def exists(self, path, attr=None):
+        rc = self.L.sfx_h5_exists(self.h, _b(path), _b(attr))
+        if rc < 0:
+            C.check(rc, "sfx_h5_exists")
+        return rc == 1
+
+    def info(self, path, attr=None):
+        """(type class, type size, shape): class 0 integer, 1 floating point,
+        8 enumeration, -1 other."""
+        cls, size, rank = ct.c_int(), ct.c_int(), ct.c_int()
+        dims = (ct.c_int64 * 8)()
+        C.check(self.L.sfx_h5_info(self.h, _b(path), _b(attr), ct.byref(cls), ct.byref(size),
+                                   ct.byref(rank), dims), "sfx_h5_info")
+        return cls.value, size.value, tuple(dims[d] for d in range(rank.value))
+
+    def read(self, path, attr=None, integer=None):
+        """The values as float64 (or int64 for integer types, or when integer
+        is True), in the stored shape."""
+        cls, _, shape = self.info(path, attr)
+        n = int(np.prod(shape)) if shape else 1
+        as_int = (cls in (0, 8)) if integer is None else integer
+        out = np.empty(n, np.int64 if as_int else np.float64)
+        fn = self.L.sfx_h5_read_i64 if as_int else self.L.sfx_h5_read_f64
+        C.check(fn(self.h, _b(path), _b(attr), n, out.ctypes.data), "sfx_h5_read")
+        return out.reshape(shape)
+
+    def list(self, path="/"):
+        """(members, attribute names) of the object at path."""
+        cap = 1 << 12
+        while True:
+            buf = ct.create_string_buffer(cap)
+            rc = self.L.sfx_h5_list(self.h, _b(path), buf, cap)
+            if rc == 0:
+                break
+            if rc < 0:
+                C.check(rc, "sfx_h5_list")
+            cap = rc
+        names = [s for s in buf.value.decode().split("\n") if s]
+        return ([s for s in names if not s.startswith("@")],
+                [s[1:] for s in names if s.startswith("@")])
+
+
+class H5Writer:
+    """Builds an HDF5 file in memory; save() writes it."""
+
+    def __init__(self):
+        self.L = _lib()
+        self.h = vp()
+        C.check(self.L.sfx_h5w_create(ct.byref(self.h)), "sfx_h5w_create")
+
+    def close(self):
+        if self.h:
+            self.L.sfx_h5w_destroy(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def group(self, path):
+        C.check(self.L.sfx_h5w_group(self.h, _b(path)), "sfx_h5w_group")
+
+    @staticmethod
+    def _typed(values, dtype):
+        a = np.array(values, dtype=dtype, order="C")  # keeps a 0-d scalar 0-d
+        kind = {"f": SFX_H5_FLOAT, "u": SFX_H5_UINT, "i": SFX_H5_INT}[a.dtype.kind]
+        return a, kind, a.dtype.itemsize
+
+    def dataset(self, path, values, dtype):
+        a, kind, size = self._typed(values, dtype)
+        dims = (ct.c_int64 * max(a.ndim, 1))(*a.shape)
+        C.check(self.L.sfx_h5w_dataset(self.h, _b(path), kind, size, a.ndim, dims,
+                                       a.ctypes.data if a.size else None), "sfx_h5w_dataset")
+
+    def attr(self, path, name, values, dtype):
+        a, kind, size = self._typed(values, dtype)
+        dims = (ct.c_int64 * max(a.ndim, 1))(*a.shape)
+        C.check(self.L.sfx_h5w_attr(self.h, _b(path), _b(name), kind, size, a.ndim, dims,
+                                    a.ctypes.data), "sfx_h5w_attr")
+
+    def save(self, path):
+        C.check(self.L.sfx_h5w_save(self.h, str(path).encode()), "sfx_h5w_save")
+
+
+def _vector(f, path, n, ndim=3):
+    """An [n][3] (or [n][1]) dataset as float64 [n][3] / [n]."""
+    a = f.read(path, integer=False)
+    return a.reshape(n, -1)
+
+
+def _scalar(f, path, n):
+    return f.read(path, integer=False).reshape(n)
+
+
+def keep_inside(pos, box, ndim):
+    """GadgetSnapshotReader::keep_inside (GadgetSnapshotReader.hpp:49-58):
+    one box length added below the origin, subtracted past the far side."""
+    pos = pos.copy()
+    for i in range(ndim):
+        pos[:, i] = np.where(pos[:, i] < 0.0, pos[:, i] + box[i], pos[:, i])
+        pos[:, i] = np.where(pos[:, i] > box[i], pos[:, i] - box[i], pos[:, i])
+    return pos
+
+
+def read_gadget_snapshot(path, ndim=3, read_mass=False):
+    """GadgetSnapshotReader::read_snapshot (src/GadgetSnapshotReader.cpp:79-560)
+    on one rank: the header (box from BoxSize, padded to 3 values; particle
+    counts NumPart_ThisFile[0, 1, 4]; Time as float or double), the run-time
+    flags (/RuntimePars: PeriodicBoundariesOn; GravityOn, SofteningLength
+    defaulting to false and 0.03; without the group: not periodic, no
+    gravity), and the particles:
+      gas   (PartType0): positions (kept inside the box), velocities, density
+            (or masses with read_mass), pressure = u 2 rho / 3 (gamma 5/3, as
+            the reference; with read_mass the internal energy u itself), ids,
+            the passive scalars present (Dye, Fe, Mg);
+      dm    (PartType1) and stars (PartType4): positions, velocities, masses,
+            ids.
+    Values in the file's units; "units" holds the /Units cgs factors or None
+    (the reference then assumes SI)."""
+    out = {}
+    with H5File(path) as f:
+        box = list(np.atleast_1d(f.read("/Header", "BoxSize", integer=False)))
+        while len(box) < 3:
+            box.append(box[0])
+        out["box"] = np.array(box[:3], np.float64)
+        numpart = np.atleast_1d(f.read("/Header", "NumPart_ThisFile", integer=True))
+        out["numpart"] = numpart
+        out["time"] = float(np.atleast_1d(f.read("/Header", "Time", integer=False))[0])
+        if f.exists("/Units"):
+            out["units"] = {k: float(np.atleast_1d(f.read("/Units", v, integer=False))[0])
+                            for k, v in UNIT_NAMES.items() if f.exists("/Units", v)}
+        else:
+            out["units"] = None
+        if f.exists("/RuntimePars"):
+            rp = "/RuntimePars"
+            out["periodic"] = (bool(np.atleast_1d(f.read(rp, "PeriodicBoundariesOn",
+                                                         integer=True))[0])
+                               if f.exists(rp, "PeriodicBoundariesOn") else False)
+            out["gravity"] = (bool(np.atleast_1d(f.read(rp, "GravityOn", integer=True))[0])
+                              if f.exists(rp, "GravityOn") else False)
+            out["hsoft"] = (float(np.atleast_1d(f.read(rp, "SofteningLength",
+                                                       integer=False))[0])
+                            if f.exists(rp, "SofteningLength") else 0.03)
+        else:
+            out["periodic"], out["gravity"], out["hsoft"] = False, False, 0.03
+        ngas = int(numpart[0])
+        if ngas:
+            g = "/PartType0/"
+            gas = {"positions": keep_inside(_vector(f, g + "Coordinates", ngas), out["box"],
+                                            ndim),
+                   "velocities": _vector(f, g + "Velocities", ngas)}
+            u = _scalar(f, g + "InternalEnergy", ngas)
+            if read_mass:
+                gas["masses"] = _scalar(f, g + "Masses", ngas)
+                gas["pressure"] = u
+            else:
+                rho = _scalar(f, g + "Density", ngas)
+                gas["density"] = rho
+                gas["pressure"] = u * 2.0 * rho / 3.0
+            gas["ids"] = f.read(g + "ParticleIDs", integer=True).reshape(ngas)
+            gas["paqs"] = {p: _scalar(f, g + p, ngas) for p in PAQ_NAMES if f.exists(g + p)}
+            out["gas"] = gas
+        for key, grp, idx in (("dm", "/PartType1/", 1), ("stars", "/PartType4/", 4)):
+            n = int(numpart[idx]) if len(numpart) > idx else 0
+            if not n:
+                continue
+            out[key] = {"positions": keep_inside(_vector(f, grp + "Coordinates", n),
+                                                 out["box"], ndim),
+                        "velocities": _vector(f, grp + "Velocities", n),
+                        "masses": _scalar(f, grp + "Masses", n),
+                        "ids": f.read(grp + "ParticleIDs", integer=True).reshape(n)}
+    return out
+
+
+def gas_primitives(snap, ndim):
+    """The primitive rows [rho, v (ndim), p] of the gas, in the reference's
+    StateVector order (GadgetSnapshotReader.cpp:395-404)."""
+    g = snap["gas"]
+    return np.column_stack([g["density"], g["velocities"][:, :ndim], g["pressure"]])
+
+
+def write_gadget_snapshot(path, box, time, positions, density, velocities, pressure, ids=None,
+                          masses=None, paqs=None, periodic=True, gravity=False, hsoft=0.03,
+                          units=None):
+    """The gas part of GadgetSnapshotWriter's output
+    (src/GadgetSnapshotWriter.cpp:224-400): /Header (BoxSize, Flag_Entropy_ICs,
+    MassTable, NumFilesPerSnapshot, NumPart_ThisFile, NumPart_Total,
+    NumPart_Total_HighWord, Time as float), /RuntimePars
+    (PeriodicBoundariesOn, GravityOn, SofteningLength with gravity), /Units
+    (cgs factors; SI when not given) and /PartType0 with Coordinates
+    (double, [n][3]), Density (double), Velocities (float, [n][3]),
+    InternalEnergy = 1.5 p / rho (0 where rho = 0), ParticleIDs (uint64),
+    the passive scalars given and, with masses, Masses."""
+    pos = np.asarray(positions, np.float64)
+    n, nd = pos.shape
+    p3 = np.zeros((n, 3))
+    p3[:, :nd] = pos
+    v3 = np.zeros((n, 3))
+    vel = np.asarray(velocities, np.float64)
+    v3[:, :vel.shape[1]] = vel
+    rho = np.asarray(density, np.float64)
+    prs = np.asarray(pressure, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = np.where(rho != 0.0, 1.5 * prs / np.where(rho != 0.0, rho, 1.0), 0.0)
+    ids = np.arange(n, dtype=np.uint64) if ids is None else np.asarray(ids, np.uint64)
+    units = units or {"U_I": 1.0, "U_L": 100.0, "U_M": 1000.0, "U_T": 1.0, "U_t": 1.0}
+    num = np.zeros(6, np.uint32)
+    num[0] = n
+    with H5Writer() as w:
+        w.group("/Header")
+        w.attr("/Header", "BoxSize", np.asarray(box, np.float64)[:nd], np.float64)
+        w.attr("/Header", "Flag_Entropy_ICs", np.zeros(6, np.uint32), np.uint32)
+        w.attr("/Header", "MassTable", np.zeros(6), np.float64)
+        w.attr("/Header", "NumFilesPerSnapshot", np.array([1], np.int32), np.int32)
+        w.attr("/Header", "NumPart_ThisFile", num, np.uint32)
+        w.attr("/Header", "NumPart_Total", num, np.uint32)
+        w.attr("/Header", "NumPart_Total_HighWord", np.zeros(6, np.uint32), np.uint32)
+        w.attr("/Header", "Time", np.array([time], np.float32), np.float32)
+        w.group("/RuntimePars")
+        w.attr("/RuntimePars", "PeriodicBoundariesOn", np.array([int(periodic)], np.uint32),
+               np.uint32)
+        w.attr("/RuntimePars", "GravityOn", np.array([int(gravity)], np.uint32), np.uint32)
+        if gravity:
+            w.attr("/RuntimePars", "SofteningLength", np.array([hsoft]), np.float64)
+        w.group("/Units")
+        for k, name in UNIT_NAMES.items():
+            w.attr("/Units", name, np.array([units[k]]), np.float64)
+        w.group("/PartType0")
+        w.dataset("/PartType0/Coordinates", p3, np.float64)
+        w.dataset("/PartType0/Density", rho, np.float64)
+        w.dataset("/PartType0/Velocities", v3, np.float32)
+        w.dataset("/PartType0/InternalEnergy", u, np.float64)
+        w.dataset("/PartType0/ParticleIDs", ids, np.uint64)
+        for name, vals in (paqs or {}).items():
+            w.dataset("/PartType0/" + name, np.asarray(vals, np.float64), np.float64)
+        if masses is not None:
+            w.dataset("/PartType0/Masses", np.asarray(masses, np.float64), np.float64)
+        w.save(path)

@@ -11,6 +11,10 @@ moving: random face velocity and 25% reflective faces):
 plus time-derivative and get_W/get_Q vectors.  gamma = 5/3, tol = 1e-8,
 cutoff = -5 (the configuration the benchmarks use) and a gamma = 1.4 set.
 
+tests/golden/ic_cosmo3d.hdf5, ic_nbody3d.hdf5: the initial-condition files of
+the reference test suite (rundir/testsuite/cosmo, nbody), copied unchanged as
+data for the snapshot reader tests (not generated here).
+
 tests/golden/mesh_{2,3}d.npz: small periodic and mirror-walled meshes from
 the host tessellator with states, and the reference classes' gradients
 (VorCell::estimate_gradient), face velocities and normals (VorFace::set_v,
