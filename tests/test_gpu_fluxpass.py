@@ -202,3 +202,30 @@ def test_grouped_records_bit_identical(device, nd, kind, prec):
         res.append({"grad": fp.m.grad_soa(), "dQL": fp.m.dQL_soa(), "dQ": fp.m.dQ_soa()})
     for k in ("grad", "dQL", "dQ"):
         assert np.array_equal(res[0][k], res[1][k]), k
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (2, 3), (2, 5), (3, 20), (3, 40)])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_tiny_meshes(device, shape, kind):
+    """Meshes of 4 to 40 cells (one partial group of 64, cells that meet
+    several periodic images of one neighbour): the FP64 pass against the
+    oracle, the mixed pass within 1e-6 of it and conservative."""
+    nd, n = shape
+    print("case nd %d n %d kind %d" % (nd, n, kind), flush=True)
+    if nd == 2:
+        b = kh_bundle(n, kind)
+    else:
+        m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(n, seed=4)))
+        b = M.prepare(m, M.sedov_ics(m, ncells=2), G, kind=kind)
+    g = run_gpu(b, kind, "fp64", device)
+    og, odQ, odQL, ost = oracle_pass(b, kind)
+    assert np.array_equal(g["grad"], og), "gradients must be bit-exact"
+    assert np.array_equal(g["fstatus"] != 0, ost != 0)
+    den = np.maximum(np.abs(odQL).max(0), face_scale(b, kind))
+    den = np.where(den > 0, den, 1.0)
+    assert (np.abs(g["dQL"] - odQL).max(0) / den).max() <= 1e-12
+    assert np.abs(g["dQ"] - odQ).max() <= 1e-12 * np.abs(odQ).max()
+    gm = run_gpu(b, kind, "mixed", device)
+    errs = mixed_errors(b, kind, g, gm)
+    assert errs["face_max"] <= 1e-6 and errs["cell_gross_max"] <= 1e-6
+    assert errs["conservation"] <= 1e-13

@@ -264,3 +264,37 @@ def test_compacted_exact_equals_one_kernel(device, f32):
         assert np.array_equal(F1, F0), split
         assert np.array_equal(st1, st0), split
         assert c1 == c0, split
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_empty_and_ragged_batches(device, kind, precision):
+    """N = 0 launches nothing and succeeds; batch sizes that leave a partial
+    wavefront or block (1, 63, 65, 257, 65,537) give the oracle's fluxes to
+    the tolerance of their path, and the partial tail writes nothing past N."""
+    import torch
+    s = RiemannSolver(kind, 3, G, 1e-8, -5.0, precision)
+    dt = torch.float32 if precision == "mixed" else torch.float64
+    z = torch.zeros((5, 0), dtype=dt, device=device)
+    F = s.solve_for_flux_batch(z, z.clone(), torch.zeros((3, 0), dtype=dt, device=device))
+    torch.cuda.synchronize()
+    assert tuple(F.shape) == (5, 0)
+    b = WK.riemann_batch(65537, seed=11, family="mixed", ndim=3)
+    for N in (1, 63, 65, 257, 65537):
+        WL, WR, n = b["WL"][:, :N], b["WR"][:, :N], b["n"][:, :N]
+        # outputs are the heads of larger buffers whose tails hold a
+        # sentinel: the kernels write only the first 5 N fluxes, N flags
+        Fbuf = torch.full((5 * N + 64,), 7.25, dtype=dt, device=device)
+        sbuf = torch.full((N + 64,), 0xAB, dtype=torch.uint8, device=device)
+        F = Fbuf[:5 * N].view(5, N)
+        st = sbuf[:N]
+        st.zero_()
+        s.solve_for_flux_batch(_t(WL, dt, device), _t(WR, dt, device), _t(n, dt, device),
+                               F=F, status=st)
+        torch.cuda.synchronize()
+        assert torch.all(Fbuf[5 * N:] == 7.25) and torch.all(sbuf[N:] == 0xAB)
+        Fo, sto, _ = O.solve_for_flux(O.EXACT if kind == 0 else O.HLLC, 3, WL, WR, n)
+        ok = (sto == 0) & (st.cpu().numpy() == 0)
+        err = summarize(normwise_error(F.double().cpu().numpy()[:, ok], Fo[:, ok], WL[:, ok],
+                                       WR[:, ok], G))
+        assert err["max"] <= (FP64_TOL if precision == "fp64" else MIXED_TOL), (N, err)
