@@ -337,6 +337,9 @@ typedef struct {
                                   (sfx_mesh_entry_groups) -- egeo then holds
                                   rows * 64 records, rows =
                                   egeo_group[ceil(C/64)] */
+    int64_t egeo_rows;         /* that rows value, or 0 if not given (with it,
+                                  and records under 4 GB, a lane past its
+                                  cell's entries reads no padding) */
 } sfx_mesh;
 
 /* Per-entry face records for the gradient pass, built once per

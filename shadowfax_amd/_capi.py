@@ -37,7 +37,7 @@ class MeshDesc(ct.Structure):
                 ("left", vp), ("right", vp), ("ghost", vp), ("twin", vp),
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
                 ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
-                ("gather_entry", vp), ("egeo_group", vp)]
+                ("gather_entry", vp), ("egeo_group", vp), ("egeo_rows", i64)]
 
 
 _lib = None

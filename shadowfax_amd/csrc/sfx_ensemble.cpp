@@ -36,7 +36,8 @@ struct sfx_ensemble {
     int32_t *left = nullptr, *right = nullptr, *twin = nullptr, *cf_entry = nullptr,
             *gather_entry = nullptr;
     int64_t *cf_off = nullptr, *gather_off = nullptr, *egrp = nullptr;
-    size_t egeo_bytes = 0;
+    size_t egeo_bytes = 0;  // allocated
+    uint64_t rec_bytes = 0;  // the grouped records built in it
     void *area = nullptr, *dL = nullptr, *dR = nullptr, *rRL = nullptr, *mc = nullptr,
          *rcent = nullptr, *egeo = nullptr;
     // mesh table
@@ -109,6 +110,7 @@ flux::MeshView<T> ens_view(const sfx_ensemble* e) {
     v.cf_entry = e->cf_entry;
     v.egeo = (const T*)e->egeo;
     v.egrp = e->egrp;
+    v.egeo_bytes = e->rec_bytes;
     v.gather_off = e->gather_off;
     v.gather_entry = e->gather_entry;
     return v;
@@ -180,6 +182,7 @@ int build_entries(sfx_ensemble* e) {
         e->egeo_bytes = bytes;
         drop_graph(e);
     }
+    e->rec_bytes = bytes;
     err = hipMemcpy(e->egrp, grp.data(), 8 * (ng + 1), hipMemcpyHostToDevice);
     if(err == hipSuccess) {
         if(e->prec == SFX_PREC_FP64) {

@@ -57,6 +57,9 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.cf_entry = m->cf_entry;
     v.egeo = (const T*)m->egeo;
     v.egrp = m->egeo_group;
+    if(m->egeo_group && m->egeo_rows > 0)
+        v.egeo_bytes = (uint64_t)m->egeo_rows * 64 * flux::entry_rec(m->ndim, sizeof(T) == 4) *
+                       sizeof(T);
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
     return v;
@@ -77,6 +80,7 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
         return set_error(SFX_ERR_ARG, "%s: egeo must be 16-byte aligned", who);
     if(m->egeo_group && !m->cf_off)
         return set_error(SFX_ERR_ARG, "%s: egeo_group needs cf_off", who);
+    if(m->egeo_rows < 0) return set_error(SFX_ERR_ARG, "%s: egeo_rows < 0", who);
     if(gradients && (!m->W || !m->volume || !m->cf_off || !m->egeo))
         return set_error(SFX_ERR_ARG, "%s: missing array for the gradient pass (W, volume, "
                          "cf_off, egeo)", who);

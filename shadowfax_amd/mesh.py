@@ -329,6 +329,7 @@ class DeviceMesh:
             if rows < 0:
                 raise Cp.SfxError(Cp.lib().sfx_last_error().decode())
             self.arr["egeo_group"] = t(grp, torch.int64)
+            self.egeo_rows = int(rows)
             self.egeo = torch.zeros((rows * 64, ER), dtype=tdt, device=self.dev)
         else:
             self.egeo = torch.zeros((self.E, ER), dtype=tdt, device=self.dev)
@@ -355,6 +356,7 @@ class DeviceMesh:
         d.egeo = ct.c_void_p(self.egeo.data_ptr())
         d.gather_entry = p("gather_entry")
         d.egeo_group = p("egeo_group") if self.grouped else None
+        d.egeo_rows = self.egeo_rows if self.grouped else 0
         d.twin = p("twin") if self.use_twin else None
         return d
 
