@@ -1045,8 +1045,18 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
             vr[l] = (R)m.vel[Rs * ND + l];
         }
         const double dtL = m.dt[L], dtR = m.dt[Rs];
-        const double massL = m.mass[L], massR = m.mass[Rs];
-        const double areaL = m.total_area[L], areaR = m.total_area[Rs];
+        // 3D: the cells' masses and total areas load with the rest; 2D: after
+        // the solve (fewer registers held through it: 2D mixed faces
+        // 26.9 -> 25.3 us, while 3D lost 2% that way)
+        constexpr bool LATE = ND == 2;
+        double massL = 0.0, massR = 0.0, areaL = 0.0, areaR = 0.0;
+        if constexpr(!LATE) {
+            massL = m.mass[L];
+            massR = m.mass[Rs];
+            areaL = m.total_area[L];
+            areaR = m.total_area[Rs];
+            asm volatile("" ::"v"(massL), "v"(massR), "v"(areaL), "v"(areaR));
+        }
         R WL[NC], WR[NC];
 #pragma unroll
         for(int k = 0; k < NC; k++) {
@@ -1065,7 +1075,7 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
         for(int l = 0; l < ND; l++)
 #pragma unroll
             for(int k = 0; k < NC; k++) asm volatile("" ::"v"(gL[l][k]), "v"(gR[l][k]));
-        asm volatile("" ::"v"(dtL), "v"(dtR), "v"(massL), "v"(massR), "v"(areaL), "v"(areaR));
+        asm volatile("" ::"v"(dtL), "v"(dtR));
         uint8_t st = 0;
         double out[NC];
 #pragma unroll
@@ -1203,6 +1213,12 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
 #pragma unroll
             for(int k = 0; k < NC; k++) dQ[k] = dA * flux[k];
             // mass flux limiter, :840-866; dQR stays -dQL bitwise
+            if constexpr(LATE) {
+                massL = m.mass[L];
+                massR = m.mass[Rs];
+                areaL = m.total_area[L];
+                areaR = m.total_area[Rs];
+            }
             // the mixed pass takes its quotients through rcp_nr
             auto quot = [](double a, double b) {
                 if constexpr(MIXED)
