@@ -47,9 +47,6 @@
 #ifndef SFX_GRAD_KN_2D  // FP32-record gradients of 2D meshes: neighbour rows in flight
 #define SFX_GRAD_KN_2D 4
 #endif
-#ifndef SFX_GRAD_KEEP_2D  // 2D: hold the records' second halves for the limiter
-#define SFX_GRAD_KEEP_2D 0
-#endif
 #ifndef SFX_REC_BUFFER  // grouped records through a bounded buffer descriptor
 #define SFX_REC_BUFFER 1
 #endif
@@ -508,22 +505,16 @@ __device__ __forceinline__ float4 rec_plane(__amdgpu_buffer_rsrc_t rs, int64_t i
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
 }
 
-template <int ND, typename T, bool GROUPED, bool BUF = false>
+template <int ND, typename T, bool GROUPED>
 __device__ __forceinline__ void load_record(const MeshView<T>& m, int64_t base, int64_t p,
-                                            int lane, T* out, bool own,
-                                            __amdgpu_buffer_rsrc_t rs) {
+                                            int lane, T* out) {
     constexpr int ER = entry_rec<ND, T>();
     if constexpr(GROUPED) {
         constexpr int P = rec_planes<ND, T>();
         const float4* E = reinterpret_cast<const float4*>(m.egeo);
 #pragma unroll
-        for(int q = 0; q < P; q++) {
-            const int64_t idx = ((base + p) * P + q) * 64 + lane;
-            if constexpr(BUF)
-                reinterpret_cast<float4*>(out)[q] = rec_plane(rs, idx, own);
-            else
-                reinterpret_cast<float4*>(out)[q] = E[idx];
-        }
+        for(int q = 0; q < P; q++)
+            reinterpret_cast<float4*>(out)[q] = E[((base + p) * P + q) * 64 + lane];
     } else {
 #pragma unroll
         for(int q = 0; q < ER; q++) out[q] = m.egeo[(base + p) * ER + q];
@@ -650,22 +641,16 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             H[u] = half(p >= 0 ? p : 0, 1);
         }
     };
-    // 2D: a cell's entries fit one chunk (at most 8 in the meshes here): the
-    // records stay in registers for the limiter loop.  3D: chunk after
-    // chunk, the next chunk's records in flight while this chunk's
-    // neighbour states load.
-    constexpr bool KEEP = ND == 2 && KB >= 8 && SFX_GRAD_KEEP_2D;
+    // 2D: a cell's entries fit one chunk of 8 (the records' second halves
+    // are read again for the limiter); 3D: chunk after chunk, the next
+    // chunk's records in flight while this chunk's neighbour states load
     constexpr bool PIPE = ND == 3 && SFX_GRAD_PIPE;
-    const bool one_chunk = KEEP && n > 0 && n <= KB;
     float4 lo[KB], hi[KB];
     if(n > 0) load_chunk(n, lo, hi);
     for(int64_t pb = n; pb > 0; pb -= KB) {
-        // every neighbour row is loaded unconditionally (a mirror neighbour
-        // is the cell's own row, reflected below), so the loads go out back
-        // to back
         // the neighbour rows in batches of KN (all KB at once in 3D; two
-        // batches of 4 in 2D, which holds the registers to four waves per
-        // SIMD at the cost of one more load trip)
+        // batches of 4 in 2D: fewer registers in flight, measured as fast as
+        // one batch of 8)
         constexpr int KN = (ND == 2) ? SFX_GRAD_KN_2D : KB;
         float4 nlo[PIPE ? KB : 1], nhi[PIPE ? KB : 1];
 #pragma unroll
@@ -696,19 +681,11 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
                     mirror_velocity<ND, double>(Wj[v] + 1, rr);
                 }
             }
-            if constexpr(KEEP) {
-                // the limiter's halves must be in registers by now, not
-                // read again after this loop
-                if(u0 == 0) {
-#pragma unroll
-                    for(int u = 0; u < KB; u++) asm volatile("" ::"v"(hi[u].x), "v"(hi[u].y));
-                }
-            }
 #pragma unroll
             for(int v = 0; v < KN; v++) {
                 const int u = u0 + v;
                 // A (0: a padding row) from the first half in 2D
-                const float A = (ND == 3) ? hi[u].w : KEEP ? hi[u].z : lo[u].w;
+                const float A = (ND == 3) ? hi[u].w : lo[u].w;
                 if(u >= pb || !(A != 0.f)) continue;
                 const double gl[3] = {(double)lo[u].x, (double)lo[u].y,
                                       (ND == 3) ? (double)lo[u].z : 0.0};
@@ -730,7 +707,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
                 lo[u] = nlo[u];
                 hi[u] = nhi[u];
             }
-        } else if(!one_chunk && pb > KB) {
+        } else if(pb > KB) {
             load_chunk(pb - KB, lo, hi);
         }
     }
@@ -758,10 +735,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             }
         }
     };
-    if(one_chunk) {
-#pragma unroll
-        for(int u = 0; u < KB; u++) limit(hi[u], u < n);
-    } else {
+    {
         // KL halves a time, their loads out together (a pipelined loop, the
         // next chunk loading while this one is used, saves little here: the
         // work per chunk is short next to a load's latency)
@@ -789,7 +763,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 
 // Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
 // for a whole mesh, the arena offset of the cell's mesh in an ensemble).
-template <int ND, typename T, typename R, bool DIFF, bool GROUPED, bool ACT, bool BUF>
+template <int ND, typename T, typename R, bool DIFF, bool GROUPED, bool ACT>
 __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
                                               T* __restrict__ grad, int64_t gshift) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, T>();
@@ -798,16 +772,14 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
     // volume; inactive cells (which keep their gradients) walk no records
     // and store nothing
     const bool act = !ACT || m.active[i] != 0;
-    int64_t base, n, own = 0;
+    int64_t base, n;
     if constexpr(GROUPED) {
         base = m.egrp[i >> 6];
         n = m.egrp[(i >> 6) + 1] - base;
-        if constexpr(BUF) own = m.cf_off[i + 1] - m.cf_off[i];  // the cell's own rows
     } else {
         base = m.cf_off[i];
         n = m.cf_off[i + 1] - base;
     }
-    const __amdgpu_buffer_rsrc_t rs = rec_rsrc(m.egeo, BUF ? m.egeo_bytes : 0);
     R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) {
@@ -835,7 +807,7 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
         for(int u = 0; u < KB; u++) {
             const int64_t p = pb - 1 - u;
             alignas(16) T rec[ER];
-            load_record<ND, T, GROUPED, BUF>(m, base, p >= 0 ? p : 0, lane, rec, p < own, rs);
+            load_record<ND, T, GROUPED>(m, base, p >= 0 ? p : 0, lane, rec);
             A[u] = (R)rec[0];
 #pragma unroll
             for(int l = 0; l < ND; l++) {
@@ -903,7 +875,7 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
         for(int u = 0; u < KL; u++) {
             const int64_t p = pb - 1 - u;
             alignas(16) T rec[ER];
-            load_record<ND, T, GROUPED, BUF>(m, base, p >= 0 ? p : 0, lane, rec, p < own, rs);
+            load_record<ND, T, GROUPED>(m, base, p >= 0 ? p : 0, lane, rec);
             Ah[u] = (R)rec[0];
 #pragma unroll
             for(int l = 0; l < ND; l++) dd[u][l] = (R)rec[1 + 2 * ND + l];
@@ -963,14 +935,14 @@ __device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __
     } else {
         if(m.egrp) {
             if(m.active)
-                grad_cell_f64<ND, T, R, DIFF, true, true, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, true, true>(m, i, grad, gshift);
             else
-                grad_cell_f64<ND, T, R, DIFF, true, false, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, true, false>(m, i, grad, gshift);
         } else {
             if(m.active)
-                grad_cell_f64<ND, T, R, DIFF, false, true, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, false, true>(m, i, grad, gshift);
             else
-                grad_cell_f64<ND, T, R, DIFF, false, false, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, false, false>(m, i, grad, gshift);
         }
     }
 }
