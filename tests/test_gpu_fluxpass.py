@@ -229,3 +229,32 @@ def test_tiny_meshes(device, shape, kind):
     errs = mixed_errors(b, kind, g, gm)
     assert errs["face_max"] <= 1e-6 and errs["cell_gross_max"] <= 1e-6
     assert errs["conservation"] <= 1e-13
+
+
+def test_sedov3d_full_size_mixed_errors(device):
+    """The 3D benchmark workload at full size (2,097,152 Poisson generators,
+    Sedov blast, exact solver), mixed pass against the FP64 pass: every cell
+    within 1e-6 of its gross flux, 99% of faces within 1e-8, conservation to
+    FP64 round-off.  The largest face errors (3.1e-6 measured, a few faces
+    of the blast's hot core) sit above 1e-6; storing the gradients in FP64
+    moved them (to 5.2e-6) rather than removing them, so they are bounded
+    here at 1e-5 rather than claimed below 1e-6."""
+    nthreads = min(os.cpu_count() or 1, 16)
+    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(2097152, seed=20240604)),
+                           nthreads=nthreads)
+    b = M.prepare(m, M.sedov_ics(m), G, kind=0)
+    del m
+    print("cells", b["C"], "faces", b["F"], flush=True)
+    g64 = run_gpu(b, 0, "fp64", device)
+    gmx = run_gpu(b, 0, "mixed", device)
+    errs = mixed_errors(b, 0, g64, gmx)
+    den = np.maximum(np.abs(g64["dQL"]).max(0), face_scale(b, 0))
+    ef = np.abs(np.asarray(gmx["dQL"], np.float64) - g64["dQL"]).max(0) / np.where(den > 0, den, 1.0)
+    nbad = int((ef > 1e-6).sum())
+    print("sedov3d full size mixed vs fp64: %s; faces above 1e-6: %d of %d" % (errs, nbad, b["F"]),
+          flush=True)
+    assert nbad <= 20
+    assert errs["cell_gross_max"] <= 1e-6
+    assert errs["face_p99"] <= 1e-8
+    assert errs["face_max"] <= 1e-5
+    assert errs["conservation"] <= 1e-13
