@@ -188,6 +188,7 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
         r["open_loop"] = {"rate_per_s": rate, "p50_ms": st["p50_s"] * 1e3,
                           "p99_ms": st["p99_s"] * 1e3, "mean_ms": st["mean_s"] * 1e3,
                           "max_ms": st["max_s"] * 1e3, "mean_batch": st["mean_batch"],
+                          "steps": st["nbatch"],
                           "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"]}
         out[name] = r
         log("  %s: capacity %.0f req/s, p50 %.3f ms p99 %.3f ms" %
