@@ -1,9 +1,12 @@
-"""Gadget-style HDF5 snapshots and initial conditions.
+"""HDF5 snapshots and initial conditions in the Gadget and Shadowfax formats.
 
 read_gadget_snapshot mirrors GadgetSnapshotReader::read_snapshot
 (src/GadgetSnapshotReader.cpp:79-560, GadgetSnapshotReader.hpp:49-58) and
 write_gadget_snapshot the files GadgetSnapshotWriter writes
-(src/GadgetSnapshotWriter.cpp:220-500).  The HDF5 structures are parsed and
+(src/GadgetSnapshotWriter.cpp:220-500); read_shadowfax_snapshot and
+write_shadowfax_snapshot do the same for ShadowfaxSnapshotReader and
+ShadowfaxSnapshotWriter (src/ShadowfaxSnapshotReader.cpp,
+src/ShadowfaxSnapshotWriter.cpp).  The HDF5 structures are parsed and
 written by the native sfx_h5 / sfx_h5w entry points (include/shadowfax_amd.h);
 there is no dependency on an HDF5 library.
 
@@ -315,3 +318,151 @@ def write_gadget_snapshot(path, box, time, positions, density, velocities, press
         if masses is not None:
             w.dataset("/PartType0/Masses", np.asarray(masses, np.float64), np.float64)
         w.save(path)
+
+
+# ------------------------------------------------------ Shadowfax format --
+
+def _cells_names(ndim):
+    """The datasets of the /cells block in the order the reference writes
+    them (src/ShadowfaxSnapshotWriter.cpp:101-132)."""
+    ax = "xyz"[:ndim]
+    return (["id", "density"] + ["velocity_" + a for a in ax] + ["pressure", "timestep"] +
+            ["acceleration_" + a for a in ax] + ["mass"])
+
+
+DM_NAMES = ("id", "mass", "x", "y", "z", "vx", "vy", "vz", "process")
+
+
+def write_shadowfax_snapshot(path, ndim, box, time, gas=None, dm=None, origin=None,
+                             periodic=True, second_order=True, static=False,
+                             global_timestep=False, gamma=5.0 / 3.0, gravity=False, hsoft=0.0):
+    """The file ShadowfaxSnapshotWriter::write_snapshot writes
+    (src/ShadowfaxSnapshotWriter.cpp:59-277, through FileOutput::write_header
+    and FileOutput::write, src/io/FileOutput.cpp:73-254):
+      /Header attributes: npart (uint32 [2]: gas, DM), ndim (uint32), box
+        (double [2 ndim]: origin, then side lengths), time (double),
+        periodic, "second order", static, "global timestep", gravity
+        (booleans stored as uint32), "adiabatic index" (double), and hsoft
+        (double) when there are DM particles and gravity is on;
+      /cells: id, density, velocity_x..., pressure, timestep,
+        acceleration_x..., mass; /grid: x, y, z (z = 0 in 2D);
+      /DM: id, mass, x, y, z, vx, vy, vz, process (rank, 0 here);
+    every dataset a one-dimensional double array.  gas: positions [n][ndim],
+    density, velocities [n][ndim], pressure, and optionally ids (default
+    0..n-1), timestep, acceleration [n][ndim], masses (default 0).  dm:
+    positions, velocities, masses, ids.  Values are written as given: the
+    reference converts to its output units and tags each dataset with
+    unit_quantity / unit_name (variable-length strings) and unit_SI_value;
+    those attributes are not written, and the reference reader, finding no
+    unit_quantity, takes the values in its own units
+    (src/io/FileInput.cpp:100-113)."""
+    if ndim not in (2, 3):
+        raise ValueError("ndim must be 2 or 3")
+    sides = np.asarray(box, np.float64).reshape(-1)[:ndim]
+    org = np.zeros(ndim) if origin is None else np.asarray(origin, np.float64)[:ndim]
+    ngas = 0 if gas is None else len(gas["density"])
+    ndm = 0 if dm is None else len(dm["masses"])
+    b = lambda v: np.array(int(bool(v)), np.uint32)
+    with H5Writer() as w:
+        w.group("/Header")
+        w.attr("/Header", "npart", np.array([ngas, ndm], np.uint32), np.uint32)
+        w.attr("/Header", "ndim", np.array(ndim, np.uint32), np.uint32)
+        w.attr("/Header", "box", np.concatenate([org, sides]), np.float64)
+        w.attr("/Header", "time", np.array(float(time)), np.float64)
+        w.attr("/Header", "periodic", b(periodic), np.uint32)
+        w.attr("/Header", "second order", b(second_order), np.uint32)
+        w.attr("/Header", "static", b(static), np.uint32)
+        w.attr("/Header", "global timestep", b(global_timestep), np.uint32)
+        w.attr("/Header", "adiabatic index", np.array(float(gamma)), np.float64)
+        w.attr("/Header", "gravity", b(gravity), np.uint32)
+        if ndm and gravity:
+            w.attr("/Header", "hsoft", np.array(float(hsoft)), np.float64)
+        if ngas:
+            pos = np.asarray(gas["positions"], np.float64).reshape(ngas, -1)
+            vel = np.asarray(gas["velocities"], np.float64).reshape(ngas, -1)
+            acc = np.asarray(gas.get("acceleration", np.zeros((ngas, ndim))),
+                             np.float64).reshape(ngas, -1)
+            cols = [np.asarray(gas.get("ids", np.arange(ngas)), np.float64),
+                    np.asarray(gas["density"], np.float64)]
+            cols += [vel[:, d] for d in range(ndim)]
+            cols += [np.asarray(gas["pressure"], np.float64),
+                     np.asarray(gas.get("timestep", np.zeros(ngas)), np.float64)]
+            cols += [acc[:, d] for d in range(ndim)]
+            cols.append(np.asarray(gas.get("masses", np.zeros(ngas)), np.float64))
+            w.group("/cells")
+            for name, col in zip(_cells_names(ndim), cols):
+                w.dataset("/cells/" + name, np.ascontiguousarray(col), np.float64)
+            w.group("/grid")
+            for d, name in enumerate("xyz"):
+                col = pos[:, d] if d < ndim else np.zeros(ngas)
+                w.dataset("/grid/" + name, np.ascontiguousarray(col), np.float64)
+        if ndm:
+            pos = np.asarray(dm["positions"], np.float64).reshape(ndm, -1)
+            vel = np.asarray(dm["velocities"], np.float64).reshape(ndm, -1)
+            pad = lambda a, d: a[:, d] if d < a.shape[1] else np.zeros(ndm)
+            cols = [np.asarray(dm.get("ids", np.arange(ndm)), np.float64),
+                    np.asarray(dm["masses"], np.float64)]
+            cols += [pad(pos, d) for d in range(3)] + [pad(vel, d) for d in range(3)]
+            cols.append(np.zeros(ndm))
+            w.group("/DM")
+            for name, col in zip(DM_NAMES, cols):
+                w.dataset("/DM/" + name, np.ascontiguousarray(col), np.float64)
+        w.save(path)
+
+
+def read_shadowfax_snapshot(path, ndim=3, gamma=None):
+    """ShadowfaxSnapshotReader::read_snapshot (src/ShadowfaxSnapshotReader.cpp:
+    79-199) with FileInput::read_header (src/io/FileInput.cpp:163-199) on
+    one rank.  The header must have ndim equal to the ndim asked for, and,
+    when gamma is given, that adiabatic index (Header::check_makeflags,
+    src/io/Header.cpp:277-317, stops the run otherwise); hsoft is read only
+    with DM particles and gravity (else 0).  Gas: positions from /grid (x, y
+    and, in 3D, z), density, velocities, pressure and ids from /cells (ids
+    stored as doubles); DM: ids, masses, positions, velocities.  Values as
+    stored; "units" maps each dataset that carries unit_SI_value to it (the
+    reference converts such datasets to its run's units)."""
+    out = {}
+    with H5File(path) as f:
+        hd = "/Header"
+        rd = lambda name, integer: np.atleast_1d(f.read(hd, name, integer=integer))
+        fdim = int(rd("ndim", True)[0])
+        if fdim != ndim:
+            raise ValueError("dimensions of the snapshot (ndim=%d) and of the run (ndim=%d) "
+                             "do not match" % (fdim, ndim))
+        out["ndim"] = fdim
+        out["numpart"] = rd("npart", True)
+        out["box"] = rd("box", False)
+        out["time"] = float(rd("time", False)[0])
+        for key, name in (("periodic", "periodic"), ("second_order", "second order"),
+                          ("static", "static"), ("global_timestep", "global timestep"),
+                          ("gravity", "gravity")):
+            out[key] = bool(rd(name, True)[0])
+        out["gamma"] = float(rd("adiabatic index", False)[0])
+        if gamma is not None and out["gamma"] != gamma:
+            raise ValueError("the snapshot has gamma=%r, the run gamma=%r" % (out["gamma"], gamma))
+        ngas = int(out["numpart"][0])
+        ndm = int(out["numpart"][1]) if len(out["numpart"]) > 1 else 0
+        out["hsoft"] = (float(rd("hsoft", False)[0]) if ndm and out["gravity"] else 0.0)
+        units = {}
+
+        def col(group, name, n):
+            p = "/%s/%s" % (group, name)
+            if f.exists(p, "unit_SI_value"):
+                units[p] = float(np.atleast_1d(f.read(p, "unit_SI_value", integer=False))[0])
+            return f.read(p, integer=False).reshape(-1)[:n]
+
+        ax = "xyz"[:ndim]
+        if ngas:
+            out["gas"] = {
+                "positions": np.column_stack([col("grid", a, ngas) for a in ax]),
+                "density": col("cells", "density", ngas),
+                "velocities": np.column_stack([col("cells", "velocity_" + a, ngas) for a in ax]),
+                "pressure": col("cells", "pressure", ngas),
+                "ids": col("cells", "id", ngas).astype(np.int64)}
+        if ndm:
+            out["dm"] = {"ids": col("DM", "id", ndm).astype(np.int64),
+                         "masses": col("DM", "mass", ndm),
+                         "positions": np.column_stack([col("DM", a, ndm) for a in ax]),
+                         "velocities": np.column_stack([col("DM", "v" + a, ndm) for a in ax])}
+        out["units"] = units
+    return out

@@ -183,3 +183,94 @@ def test_ic_to_mesh_state(tmp_path):
     assert np.allclose(Wr[:, 3], W[3], rtol=4e-16, atol=0)
     m2 = M.voronoi_periodic(np.ascontiguousarray(s["gas"]["positions"][:, :2].T))
     assert m2["F"] == m["F"] and np.array_equal(m2["volume"], m["volume"])
+
+
+# ---------------------------------------------------------- Shadowfax format
+
+def shadowfax_gas(n, ndim, seed=11):
+    rng = np.random.default_rng(seed)
+    return dict(positions=rng.uniform(0, 3.0, (n, ndim)), density=rng.uniform(0.5, 2.0, n),
+                velocities=rng.normal(size=(n, ndim)), pressure=rng.uniform(0.1, 3.0, n),
+                ids=rng.permutation(n) + 100, timestep=rng.uniform(0, 1e-3, n),
+                acceleration=rng.normal(size=(n, ndim)), masses=rng.uniform(1, 2, n))
+
+
+def test_shadowfax_round_trip_3d(tmp_path):
+    n, m = 300, 40
+    gas = shadowfax_gas(n, 3)
+    rng = np.random.default_rng(2)
+    dm = dict(positions=rng.uniform(0, 3, (m, 3)), velocities=rng.normal(size=(m, 3)),
+              masses=rng.uniform(1, 2, m), ids=np.arange(m) + 5000)
+    path = str(tmp_path / "sfx3.hdf5")
+    S.write_shadowfax_snapshot(path, 3, [3.0, 3.0, 3.0], 0.25, gas=gas, dm=dm, gravity=True,
+                               hsoft=0.07, origin=[-1.0, 0.0, 0.5])
+    s = S.read_shadowfax_snapshot(path, ndim=3, gamma=5.0 / 3.0)
+    assert list(s["numpart"]) == [n, m] and s["ndim"] == 3 and s["time"] == 0.25
+    assert np.array_equal(s["box"], [-1.0, 0.0, 0.5, 3.0, 3.0, 3.0])
+    assert s["periodic"] and s["second_order"] and not s["static"]
+    assert not s["global_timestep"] and s["gravity"] and s["hsoft"] == 0.07
+    g = s["gas"]
+    for k in ("positions", "density", "velocities", "pressure"):
+        assert np.array_equal(g[k], gas[k]), k
+    assert np.array_equal(g["ids"], gas["ids"])
+    d = s["dm"]
+    for k in ("positions", "velocities", "masses", "ids"):
+        assert np.array_equal(d[k], dm[k]), k
+    assert s["units"] == {}
+    # the columns the reader does not use are stored as given
+    with S.H5File(path) as f:
+        assert np.array_equal(f.read("/cells/timestep"), gas["timestep"])
+        assert np.array_equal(f.read("/cells/acceleration_z"), gas["acceleration"][:, 2])
+        assert np.array_equal(f.read("/cells/mass"), gas["masses"])
+        assert np.array_equal(f.read("/DM/process"), np.zeros(m))
+
+
+def test_shadowfax_format_facts_2d(tmp_path):
+    """Header attribute types and shapes (booleans and counts as uint32,
+    scalars with a scalar dataspace, box as origin then sides), the
+    datasets of each block, z = 0 in 2D, hsoft only with DM and gravity, and
+    the reader's ndim and gamma checks."""
+    n = 50
+    path = str(tmp_path / "sfx2.hdf5")
+    S.write_shadowfax_snapshot(path, 2, [1.0, 2.0], 1.5, gas=shadowfax_gas(n, 2), gravity=True,
+                               gamma=1.4)
+    with S.H5File(path) as f:
+        groups, _ = f.list("/")
+        assert groups == ["Header", "cells", "grid"]
+        _, attrs = f.list("/Header")
+        assert sorted(attrs) == sorted(["npart", "ndim", "box", "time", "periodic",
+                                        "second order", "static", "global timestep",
+                                        "adiabatic index", "gravity"])
+        assert f.info("/Header", "npart") == (0, 4, (2,))
+        assert f.info("/Header", "ndim") == (0, 4, ())
+        assert f.info("/Header", "periodic") == (0, 4, ())
+        assert f.info("/Header", "box") == (1, 8, (4,))
+        assert f.info("/Header", "time") == (1, 8, ())
+        assert f.list("/cells")[0] == sorted(["id", "density", "velocity_x", "velocity_y",
+                                              "pressure", "timestep", "acceleration_x",
+                                              "acceleration_y", "mass"])
+        assert f.list("/grid")[0] == ["x", "y", "z"]
+        assert f.info("/cells/density") == (1, 8, (n,))
+        assert np.all(f.read("/grid/z") == 0.0)
+    s = S.read_shadowfax_snapshot(path, ndim=2)
+    assert np.array_equal(s["box"], [0.0, 0.0, 1.0, 2.0]) and s["gamma"] == 1.4
+    assert s["gas"]["positions"].shape == (n, 2) and s["hsoft"] == 0.0 and "dm" not in s
+    with pytest.raises(ValueError, match="ndim=2.*ndim=3"):
+        S.read_shadowfax_snapshot(path, ndim=3)
+    with pytest.raises(ValueError, match="gamma"):
+        S.read_shadowfax_snapshot(path, ndim=2, gamma=5.0 / 3.0)
+
+
+def test_gadget_ic_to_shadowfax_format(tmp_path):
+    """The reference's cosmological IC (Gadget format, DM only) rewritten in
+    the Shadowfax format and read back: the same particles."""
+    g = S.read_gadget_snapshot(COSMO)
+    dm = g["dm"]
+    path = str(tmp_path / "cosmo_sfx.hdf5")
+    S.write_shadowfax_snapshot(path, 3, g["box"], g["time"], dm=dm, periodic=g["periodic"])
+    s = S.read_shadowfax_snapshot(path)
+    assert list(s["numpart"]) == [0, 4096] and "gas" not in s
+    assert s["time"] == g["time"] and np.array_equal(s["box"][3:], g["box"])
+    assert s["periodic"] and not s["gravity"] and s["hsoft"] == 0.0
+    for k in ("positions", "velocities", "masses", "ids"):
+        assert np.array_equal(s["dm"][k], dm[k].astype(s["dm"][k].dtype)), k
