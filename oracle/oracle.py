@@ -93,7 +93,7 @@ def lib():
         L.sfxo_exact_solve_batch.restype = ct.c_uint
         L.sfxo_exact_solve_batch.argtypes = [
             ct.POINTER(SolverParams), ct.c_int64, _dp, _dp, _dp, _u8p, _dp,
-            _u8p, ct.POINTER(Counters)]
+            _dp, _u8p, ct.POINTER(Counters)]
         L.sfxo_time_derivative.argtypes = [ct.POINTER(SolverParams), _dp, _dp,
                                            _dp]
         L.sfxo_limit.restype = ct.c_double
@@ -158,17 +158,20 @@ def solve_for_flux(kind, ndim, WL, WR, n, v=None, refl=None, gamma=5.0 / 3.0,
 
 def exact_solve(ndim, WL, WR, n, refl=None, gamma=5.0 / 3.0, tol=1e-8,
                 cutoff=-5.0):
+    """Batched ExactRiemannSolver::solve.  Returns (Whalf, status, mach),
+    mach starting from 0 per problem."""
     s = params(EXACT, ndim, gamma, tol, cutoff)
     WL, WR, n = _c(WL), _c(WR), _c(n)
     N = WL.shape[1]
     refl = None if refl is None else _c(refl, np.uint8)
     out = np.zeros((ndim + 2, N))
+    mach = np.zeros(N)
     st = np.zeros(N, np.uint8)
     c = Counters()
     lib().sfxo_exact_solve_batch(ct.byref(s), N, _p(WL), _p(WR), _p(n),
-                                 _p(refl, _u8p), _p(out), _p(st, _u8p),
+                                 _p(refl, _u8p), _p(out), _p(mach), _p(st, _u8p),
                                  ct.byref(c))
-    return out, st, c
+    return out, st, mach
 
 
 def time_derivative(kind, ndim, W, grad, gamma=5.0 / 3.0):

@@ -741,8 +741,8 @@ unsigned sfxo_solve_for_flux_batch(const sfxo_solver* s, int64_t N,
 unsigned sfxo_exact_solve_batch(const sfxo_solver* s, int64_t N,
                                 const double* WL, const double* WR,
                                 const double* n, const uint8_t* refl,
-                                double* Whalf, uint8_t* status,
-                                sfxo_counters* c) {
+                                double* Whalf, double* mach_io,
+                                uint8_t* status, sfxo_counters* c) {
     int nd = s->ndim, NC = nd + 2;
     unsigned all = 0;
     for(int64_t i = 0; i < N; i++) {
@@ -752,9 +752,10 @@ unsigned sfxo_exact_solve_batch(const sfxo_solver* s, int64_t N,
             wr[k] = WR[k * N + i];
         }
         for(int d = 0; d < nd; d++) nn[d] = n[d * N + i];
-        double mach = 0.;
+        double mach = mach_io ? mach_io[i] : 0.;
         unsigned st = sfxo_exact_solve(s, wl, wr, nn, &mach,
                                        refl ? refl[i] : 0, w, c);
+        if(mach_io) mach_io[i] = mach;
         for(int k = 0; k < NC; k++) Whalf[k * N + i] = w[k];
         if(status) status[i] = (uint8_t)st;
         all |= st;

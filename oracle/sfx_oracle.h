@@ -75,11 +75,12 @@ unsigned sfxo_solve_for_flux_batch(const sfxo_solver* s, int64_t N,
                                    const uint8_t* refl, double* F,
                                    uint8_t* status, sfxo_counters* c);
 
-/* batched ExactRiemannSolver::solve (half states) */
+/* batched ExactRiemannSolver::solve (half states); mach: [N] in/out (the
+ * reference's running maximum, ExactRiemannSolver.cpp:181-188) or NULL */
 unsigned sfxo_exact_solve_batch(const sfxo_solver* s, int64_t N,
                                 const double* WL, const double* WR,
                                 const double* n, const uint8_t* refl,
-                                double* Whalf, uint8_t* status,
+                                double* Whalf, double* mach, uint8_t* status,
                                 sfxo_counters* c);
 
 /* get_time_derivative; grad is [ndim][ndim+2] (grad[d*(ndim+2)+k]) */

@@ -63,42 +63,3 @@ class FluxPass:
         self.run(stream)
         self.update(stream)
 
-
-def smoke(dev):
-    """A 16x16 periodic Kelvin-Helmholtz mesh: FP64 flux pass vs the oracle,
-    mixed pass vs FP64."""
-    import os
-    import sys
-
-    import numpy as np
-    import torch
-
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, os.path.join(root, "oracle"))
-    import oracle as O
-
-    from . import mesh as M
-
-    g = 5.0 / 3.0
-    m = M.voronoi_periodic(M.jittered_grid(16, 0.25, seed=7))
-    W = M.kh_ics(m)
-    b = M.prepare(m, W, g, kind=1)
-    s64 = RiemannSolver(1, 2, g, 1e-8, -5.0, "fp64")
-    fp = FluxPass(s64, DeviceMesh(b, "fp64", dev))
-    fp.run()
-    torch.cuda.synchronize()
-    dQ = fp.m.dQ_soa()
-    ob = dict(b)
-    ob["grad"] = O.gradients(1, b)
-    dQo, _, _, _ = O.face_flux_pass(1, ob)
-    scale = np.abs(dQo).max()
-    err = np.abs(dQ - dQo).max() / scale
-    print("smoke flux pass fp64 vs oracle: max err %.3e" % err, flush=True)
-    assert err <= 1e-12
-    smx = RiemannSolver(1, 2, g, 1e-8, -5.0, "mixed")
-    fm = FluxPass(smx, DeviceMesh(b, "mixed", dev))
-    fm.run()
-    torch.cuda.synchronize()
-    errm = np.abs(fm.m.dQ_soa() - dQ).max() / scale
-    print("smoke flux pass mixed vs fp64: max err %.3e" % errm, flush=True)
-    assert errm <= 1e-6

@@ -7,6 +7,7 @@ Output: tests/golden/riemann_{2,3}d.npz, each holding, per problem family
 (uniform, mixed, vacuum, shock) and variant (plain: v = 0 no reflective;
 moving: random face velocity and 25% reflective faces):
   inputs  WL, WR, n, v, refl        (float64, float32-representable)
+(and the edge families zero, negp, nan below, same keys)
   outputs F_exact, st_exact, F_hllc, st_hllc, Whalf, mach, st_solve
 plus time-derivative and get_W/get_Q vectors.  gamma = 5/3, tol = 1e-8,
 cutoff = -5 (the configuration the benchmarks use) and a gamma = 1.4 set.
@@ -36,6 +37,42 @@ from shadowfax_amd import workloads as WK  # noqa: E402
 
 N = 768
 FAMILIES = ("uniform", "mixed", "vacuum", "shock")
+# Inputs the solvers handle without iterating or by failing, drawn from the
+# uniform family:
+#   zero  exact-zero density (the only vacuum trigger of the solvers,
+#         ExactRiemannSolver.cpp:130-136, HLLCRiemannSolver.cpp:263-265,
+#         283-288) on the left (40%), the right (40%) or both sides (20%);
+#         the vacuum side's pressure is 0 (what get_W leaves) for three
+#         problems in four, else kept; its velocity is kept (a vacuum cell's
+#         state is boosted into the face frame like any other)
+#   negp  a negative pressure on one side (RiemannSolverException when the
+#         solution goes negative or NaN, ExactRiemannSolver.cpp:276-288)
+#   nan   a NaN in one component (density, a velocity or the pressure) of
+#         one side
+EDGE_FAMILIES = ("zero", "negp", "nan")
+
+
+def edge_batch(n, seed, family, ndim):
+    b = WK.riemann_batch(n, seed=seed, family="uniform", ndim=ndim)
+    WL, WR = b["WL"], b["WR"]
+    NC = ndim + 2
+    rng = np.random.default_rng(seed)
+    u = rng.random(n)
+    side = rng.random(n) < 0.5
+    if family == "zero":
+        for W, sel in ((WL, u < 0.6), (WR, (u >= 0.4))):
+            W[0, sel] = 0.0
+            W[NC - 1, sel & (rng.random(n) < 0.75)] = 0.0
+    elif family == "negp":
+        WL[NC - 1, side] *= -1.0
+        WR[NC - 1, ~side] *= -1.0
+    elif family == "nan":
+        comp = rng.integers(0, NC, n)
+        for i in range(n):
+            (WL if side[i] else WR)[comp[i], i] = np.nan
+    else:
+        raise ValueError(family)
+    return b
 
 
 def riemann_set(ndim, gamma):
@@ -52,6 +89,33 @@ def riemann_set(ndim, gamma):
             else:
                 v = WK.f32(rng.uniform(-1.0, 1.0, (ndim, N)))
                 refl = (rng.random(N) < 0.25).astype(np.uint8)
+            out[key + "WL"] = WL
+            out[key + "WR"] = WR
+            out[key + "n"] = n
+            out[key + "v"] = v
+            out[key + "refl"] = refl
+            for kind, name in ((O.EXACT, "exact"), (O.HLLC, "hllc")):
+                F, st, _ = O.ref_solve_for_flux(kind, ndim, WL, WR, n, v=v, refl=refl,
+                                                gamma=gamma)
+                out[key + "F_" + name] = F
+                out[key + "st_" + name] = st
+            Wh, st, mach = O.ref_exact_solve(ndim, WL, WR, n, refl=refl, gamma=gamma)
+            out[key + "Whalf"] = Wh
+            out[key + "mach"] = mach
+            out[key + "st_solve"] = st
+    # edge families (their own generator, so the arrays above stay as they were)
+    rng_e = np.random.default_rng(4321 + ndim)
+    for fi, fam in enumerate(EDGE_FAMILIES):
+        b = edge_batch(N, 200 + fi, fam, ndim)
+        for variant in ("plain", "moving"):
+            key = "%s_%s_" % (fam, variant)
+            WL, WR, n = b["WL"], b["WR"], b["n"]
+            if variant == "plain":
+                v = np.zeros((ndim, N))
+                refl = np.zeros(N, np.uint8)
+            else:
+                v = WK.f32(rng_e.uniform(-1.0, 1.0, (ndim, N)))
+                refl = (rng_e.random(N) < 0.25).astype(np.uint8)
             out[key + "WL"] = WL
             out[key + "WR"] = WR
             out[key + "n"] = n

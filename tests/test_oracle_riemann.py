@@ -9,7 +9,14 @@ Each check is independent of the oracle's own formulas:
     1e-4 tolerance) and the shapy Sod star state (test/testShapy2d.py.in:34-38);
   * invariants: uniform state gives the Euler flux, mirror symmetry gives zero
     mass flux, lab-frame transport of the face velocity, rotation
-    invariance, f(p*) = 0.
+    invariance, f(p*) = 0;
+  * the vacuum solutions against Toro's closed forms (Riemann Solvers and
+    Numerical Methods for Fluid Dynamics, 3rd ed., section 4.6: vacuum right
+    state, vacuum left state, generation of vacuum), written here in numpy.
+
+The golden families include inputs the solvers do not iterate on or fail
+on: exact-zero densities, negative pressures and NaNs; there the status
+bits (the reference's RiemannSolverException) must agree as well.
 """
 import os
 
@@ -31,8 +38,11 @@ def same(a, b):
     return np.all((a == b) | (np.isnan(a) & np.isnan(b)))
 
 
+FAMILIES = ["uniform", "mixed", "vacuum", "shock", "zero", "negp", "nan"]
+
+
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
+@pytest.mark.parametrize("fam", FAMILIES)
 @pytest.mark.parametrize("variant", ["plain", "moving"])
 def test_solve_for_flux_matches_reference_bitwise(ndim, fam, variant):
     g = load(ndim)
@@ -46,14 +56,116 @@ def test_solve_for_flux_matches_reference_bitwise(ndim, fam, variant):
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
-def test_exact_solve_half_state_bitwise(ndim, fam):
+@pytest.mark.parametrize("fam", FAMILIES)
+@pytest.mark.parametrize("variant", ["plain", "moving"])
+def test_exact_solve_half_state_bitwise(ndim, fam, variant):
     g = load(ndim)
-    key = "%s_moving_" % fam
-    W, st, _ = O.exact_solve(ndim, g[key + "WL"], g[key + "WR"], g[key + "n"],
-                             refl=g[key + "refl"])
+    key = "%s_%s_" % (fam, variant)
+    W, st, mach = O.exact_solve(ndim, g[key + "WL"], g[key + "WR"], g[key + "n"],
+                                refl=g[key + "refl"])
     ok = g[key + "st_solve"] == 0
+    assert np.array_equal(st != 0, ~ok)
     assert same(W[:, ok], g[key + "Whalf"][:, ok])
+    assert same(mach[ok], g[key + "mach"][ok])
+
+
+def test_edge_families_hit_their_branches():
+    """The edge fixtures reach what they are for: vacuum solutions with zero
+    fluxes or fans, failing exact solves (status set), NaN HLLC fluxes."""
+    g = load(3)
+    z = "zero_moving_"
+    assert np.all(g[z + "st_exact"] == 0) and np.all(g[z + "st_solve"] == 0)
+    both = (g[z + "WL"][0] == 0) & (g[z + "WR"][0] == 0)
+    assert both.sum() > 50
+    assert np.all(g[z + "F_hllc"][:, both & (g[z + "refl"] == 0)] == 0)
+    for fam in ("negp", "nan"):
+        assert (g[fam + "_plain_st_exact"] != 0).mean() > 0.5
+        assert (g[fam + "_plain_st_exact"] == 0).sum() > 20
+        assert np.isnan(g[fam + "_plain_F_hllc"]).any(0).mean() > 0.2
+
+
+def toro_vacuum_half_state(WL, WR, n, gamma):
+    """W(x/t = 0) of a Riemann problem with vacuum (Toro, section 4.6):
+    a vacuum right state (4.6.1), a vacuum left state (4.6.2), or the
+    generation of vacuum between two rarefactions (4.6.3), with the normal
+    velocity from the fan and the tangential velocity of the side the
+    sample lies on."""
+    nd = n.shape[0]
+    g = gamma
+    uL = (WL[1:1 + nd] * n).sum(0)
+    uR = (WR[1:1 + nd] * n).sum(0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        aL = np.where(WL[0] > 0, np.sqrt(g * WL[-1] / WL[0]), 0.0)
+        aR = np.where(WR[0] > 0, np.sqrt(g * WR[-1] / WR[0]), 0.0)
+        # fronts of the rarefactions into vacuum (4.78, 4.81, 4.83)
+        sL = uL + 2 * aL / (g - 1)
+        sR = uR - 2 * aR / (g - 1)
+        # left fan at x/t = 0 (4.56 with S = 0)
+        cL = 2 / (g + 1) + (g - 1) / ((g + 1) * aL) * uL
+        fanL = (WL[0] * cL ** (2 / (g - 1)), 2 / (g + 1) * (aL + (g - 1) / 2 * uL),
+                WL[-1] * cL ** (2 * g / (g - 1)))
+        cR = 2 / (g + 1) - (g - 1) / ((g + 1) * aR) * uR
+        fanR = (WR[0] * cR ** (2 / (g - 1)), 2 / (g + 1) * (-aR + (g - 1) / 2 * uR),
+                WR[-1] * cR ** (2 * g / (g - 1)))
+    out = np.zeros_like(WL)
+    vacL, vacR = WL[0] == 0, WR[0] == 0
+    for i in range(WL.shape[1]):
+        # x/t = 0 lies in the left wave system when the right state is
+        # vacuum or the left vacuum front moves right (sL >= 0), in the
+        # right one when the left state is vacuum or sR <= 0; otherwise
+        # (both vacuum, or vacuum generated around x/t = 0) it is vacuum
+        if not vacL[i] and (vacR[i] or sL[i] >= 0):
+            W, u, a, s, fan = WL[:, i], uL[i], aL[i], sL[i], fanL
+            if u - a >= 0:
+                out[:, i] = W
+            elif s > 0:
+                out[0, i], out[-1, i] = fan[0][i], fan[2][i]
+                out[1:1 + nd, i] = W[1:1 + nd] + (fan[1][i] - u) * n[:, i]
+        elif not vacR[i] and (vacL[i] or sR[i] <= 0):
+            W, u, a, s, fan = WR[:, i], uR[i], aR[i], sR[i], fanR
+            if u + a <= 0:
+                out[:, i] = W
+            elif s < 0:
+                out[0, i], out[-1, i] = fan[0][i], fan[2][i]
+                out[1:1 + nd, i] = W[1:1 + nd] + (fan[1][i] - u) * n[:, i]
+    return out
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_vacuum_solutions_match_toro_closed_forms(ndim):
+    """Exact-zero densities (the zero fixtures) and generated vacuum (low
+    pressures, diverging flows): the oracle's half states equal Toro's
+    closed forms to round-off, fans and vacuum regions included."""
+    g = load(ndim)
+    key = "zero_plain_"
+    WL, WR, n = g[key + "WL"], g[key + "WR"], g[key + "n"]
+    rng = np.random.default_rng(11)
+    m = 600
+    GL = np.zeros((ndim + 2, m))
+    GR = np.zeros((ndim + 2, m))
+    GL[0], GR[0] = rng.uniform(0.1, 2.0, m), rng.uniform(0.1, 2.0, m)
+    GL[-1], GR[-1] = rng.uniform(1e-4, 1e-2, m), rng.uniform(1e-4, 1e-2, m)
+    nn = rng.normal(size=(ndim, m))
+    nn /= np.sqrt((nn * nn).sum(0))
+    vt = rng.normal(size=(ndim, m))
+    GL[1:1 + ndim] = vt - rng.uniform(0.0, 2.0, m) * nn
+    GR[1:1 + ndim] = vt + rng.uniform(0.0, 2.0, m) * nn
+    # vacuum is generated where 2 (aL + aR)/(gamma - 1) <= uR - uL (4.82)
+    uL, uR = (GL[1:1 + ndim] * nn).sum(0), (GR[1:1 + ndim] * nn).sum(0)
+    aL, aR = np.sqrt(5 / 3 * GL[-1] / GL[0]), np.sqrt(5 / 3 * GR[-1] / GR[0])
+    gen = 3 * (aL + aR) <= uR - uL
+    sL, sR = uL + 3 * aL, uR - 3 * aR
+    # the generated set covers all three outcomes at x/t = 0
+    assert gen.sum() > 100
+    assert (gen & (sL > 0)).sum() > 10 and (gen & (sR < 0)).sum() > 10
+    assert (gen & (sL < 0) & (sR > 0)).sum() > 10
+    for A, B, nv in ((WL, WR, n), (GL[:, gen], GR[:, gen], nn[:, gen])):
+        W, st, _ = O.exact_solve(ndim, A, B, nv)
+        want = toro_vacuum_half_state(A, B, nv, 5.0 / 3.0)
+        assert np.all(st == 0)
+        scale = np.maximum(np.abs(A).max(0), np.abs(B).max(0))
+        err = np.abs(W - want).max(0) / scale
+        assert err.max() <= 1e-12, err.max()
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
