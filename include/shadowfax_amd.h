@@ -411,6 +411,18 @@ int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ,
                     const double* volume, double* W, uint8_t* cell_status,
                     void* stream);
 
+/* The same with individual time steps (src/Simulation.cpp:476-483,792-796): Q -= dQ
+ * and dQ = 0 for every cell (GasParticle::update_Q, which also raises
+ * SFX_ST_UPDATEQ), but W = get_W(volume, Q) only for the cells whose step
+ * ends now, endtime[i] == current_time (the time after the step; only those
+ * cells have received the fluxes of all their faces); the other cells keep
+ * their W, and their Q is not reset by get_W's pressure floor.
+ * endtime: device uint64 [C] (sfx_hydro_timestep's), or NULL for every
+ * cell (= sfx_update_step). */
+int sfx_update_step_timed(sfx_solver* h, int64_t C, double* Q, double* dQ,
+                          const double* volume, double* W, const uint64_t* endtime,
+                          uint64_t current_time, uint8_t* cell_status, void* stream);
+
 
 /* ---- serving: an ensemble of meshes resident in HBM ------------------ */
 

@@ -113,7 +113,8 @@ def lib():
                                           _u8p, ct.POINTER(Counters)]
         L.sfxo_update.restype = ct.c_uint
         L.sfxo_update.argtypes = [ct.POINTER(SolverParams), ct.c_int64, _dp,
-                                  _dp, _dp, _dp, _u8p]
+                                  _dp, _dp, _dp, ct.POINTER(ct.c_uint64),
+                                  ct.c_uint64, _u8p]
         _u64p = ct.POINTER(ct.c_uint64)
         L.sfxo_hydro_timestep.restype = ct.c_uint
         L.sfxo_hydro_timestep.argtypes = [
@@ -290,16 +291,20 @@ def face_flux_pass(kind, mesh, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0):
     return dQ, dQL, st, c
 
 
-def update(kind, ndim, Q, dQ, volume, gamma=5.0 / 3.0):
+def update(kind, ndim, Q, dQ, volume, gamma=5.0 / 3.0, W=None, endtime=None, tnow=0):
+    """update_Q for every cell; get_W where endtime == tnow (every cell when
+    endtime is None).  W: the states kept by the other cells (zeros if None).
+    Returns (Q, W, status)."""
     s = params(kind, ndim, gamma)
     Q = _c(Q).copy()
     dQ = _c(dQ).copy()
     volume = _c(volume)
     C = Q.shape[1]
-    W = np.zeros_like(Q)
+    W = np.zeros_like(Q) if W is None else _c(W).copy()
     st = np.zeros(C, np.uint8)
+    et = None if endtime is None else _c(endtime, np.uint64)
     lib().sfxo_update(ct.byref(s), C, _p(Q), _p(dQ), _p(volume), _p(W),
-                      _p(st, _u8p))
+                      _p(et, ct.POINTER(ct.c_uint64)), int(tnow), _p(st, _u8p))
     return Q, W, st
 
 

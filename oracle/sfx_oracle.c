@@ -1242,10 +1242,12 @@ unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
     return all;
 }
 
-/* GasParticle::update_Q (src/utilities/GasParticle.cpp:617-624) then get_W
- * (sequencing src/Simulation.cpp:476-483,792-836) */
+/* GasParticle::update_Q (src/utilities/GasParticle.cpp:617-624) for every
+ * cell (the loop at src/Simulation.cpp:478-483), then get_W for the cells
+ * whose step ends at the new current time (src/Simulation.cpp:792-796) */
 unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
-                     const double* volume, double* W, uint8_t* cell_status) {
+                     const double* volume, double* W, const uint64_t* endtime,
+                     uint64_t tnow, uint8_t* cell_status) {
     int nd = s->ndim, NC = nd + 2, P = nd + 1;
     unsigned all = 0;
     for(int64_t i = 0; i < C; i++) {
@@ -1256,10 +1258,13 @@ unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
             CELL(dQ, k, i, C) = 0.;
         }
         if(q[0] < 0. || q[P] < 0.) st |= SFXO_ST_UPDATEQ;
-        sfxo_get_W(s, volume[i], q, w);
-        for(int k = 0; k < NC; k++) {
-            CELL(Q, k, i, C) = q[k];
-            CELL(W, k, i, C) = w[k];
+        for(int k = 0; k < NC; k++) CELL(Q, k, i, C) = q[k];
+        if(!endtime || endtime[i] == tnow) {
+            sfxo_get_W(s, volume[i], q, w);
+            for(int k = 0; k < NC; k++) {
+                CELL(Q, k, i, C) = q[k];
+                CELL(W, k, i, C) = w[k];
+            }
         }
         if(cell_status) cell_status[i] = (uint8_t)st;
         all |= st;

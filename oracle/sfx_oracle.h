@@ -144,10 +144,13 @@ unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
                              double* dQ, double* dQL_face, uint8_t* face_status,
                              sfxo_counters* c);
 
-/* GasParticle::update_Q then get_W for every cell: Q -= dQ, dQ = 0, W = get_W.
- * Returns OR of status bits; cell_status may be NULL. */
+/* GasParticle::update_Q for every cell (Q -= dQ, dQ = 0, status on m < 0 or
+ * e < 0), then W = get_W(V, Q) for the cells whose step ends now,
+ * endtime[i] == tnow (every cell when endtime is NULL).  Returns OR of status
+ * bits; cell_status may be NULL. */
 unsigned sfxo_update(const sfxo_solver* s, int64_t C, double* Q, double* dQ,
-                     const double* volume, double* W, uint8_t* cell_status);
+                     const double* volume, double* W, const uint64_t* endtime,
+                     uint64_t tnow, uint8_t* cell_status);
 
 /* The integer timeline of TimeLine (src/TimeLine.hpp:52-113): the values
  * calculate_timestep reads. */

@@ -117,6 +117,7 @@ def _mesh_sigs():
         "sfx_gather_dQ": [vp, md, vp, vp, vp],
         "sfx_face_flux_pass": [vp, md, vp, vp, vp, vp, vp],
         "sfx_update_step": [vp, i64, vp, vp, vp, vp, vp, vp],
+        "sfx_update_step_timed": [vp, i64, vp, vp, vp, vp, vp, ct.c_uint64, vp, vp],
     }
 
 

@@ -1369,13 +1369,19 @@ __device__ __forceinline__ void get_Q_state(double g, double V, const double* w,
         q[P] = (0.5 * w[0] * v2 + w[P] / (g - 1.)) * V;
 }
 
-// Cell i; returns 1 when m or e went negative.
+// Cell i; returns 1 when m or e went negative.  Q -= dQ and dQ = 0 for every
+// cell (GasParticle::update_Q, src/Simulation.cpp:476-483); W = get_W(V, Q)
+// only where `rebuild` (the cells whose time step ends now, endtime ==
+// currentTime, src/Simulation.cpp:792-796: only they have received the
+// fluxes of all their faces); the others keep their W, and their Q is not
+// touched by get_W's pressure floor.
 template <int ND>
 __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
                                                 double* __restrict__ Q, double* __restrict__ dQ,
                                                 const double* __restrict__ volume,
                                                 double* __restrict__ Wout,
-                                                uint8_t* __restrict__ cstatus) {
+                                                uint8_t* __restrict__ cstatus,
+                                                bool rebuild = true) {
     const int NC = ND + 2, P = ND + 1;
     unsigned fail = 0;
     {
@@ -1386,11 +1392,12 @@ __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
             dQ[i * NC + k] = 0.0;
         }
         uint8_t st = (q[0] < 0. || q[P] < 0.) ? ST_UPDATEQ : 0;
-        get_W_state<ND>(cd.gamma, volume[i], q, w);
+        if(rebuild) get_W_state<ND>(cd.gamma, volume[i], q, w);
 #pragma unroll
-        for(int k = 0; k < NC; k++) {
-            Q[i * NC + k] = q[k];
-            Wout[i * NC + k] = w[k];
+        for(int k = 0; k < NC; k++) Q[i * NC + k] = q[k];
+        if(rebuild) {
+#pragma unroll
+            for(int k = 0; k < NC; k++) Wout[i * NC + k] = w[k];
         }
         if(cstatus) cstatus[i] = st;
         fail = st != 0;
@@ -1398,16 +1405,22 @@ __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
     return fail;
 }
 
+// endtime null: every cell's W is rebuilt
 template <int ND>
 __global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, double* __restrict__ Q,
                                                      double* __restrict__ dQ,
                                                      const double* __restrict__ volume,
                                                      double* __restrict__ Wout,
+                                                     const uint64_t* __restrict__ endtime,
+                                                     uint64_t tnow,
                                                      uint8_t* __restrict__ cstatus,
                                                      unsigned long long* __restrict__ ctr) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned fail = 0;
-    if(i < C) fail = update_cell<ND>(cd, i, Q, dQ, volume, Wout, cstatus);
+    if(i < C) {
+        const bool rebuild = endtime ? endtime[i] == tnow : true;
+        fail = update_cell<ND>(cd, i, Q, dQ, volume, Wout, cstatus, rebuild);
+    }
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
     block_add_counters(ctr, cv);
 }

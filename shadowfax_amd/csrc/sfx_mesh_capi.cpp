@@ -332,8 +332,9 @@ int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, void* dQL, 
     return sfx_gather_dQ(h, m, dQL, dQ, stream);
 }
 
-int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const double* volume,
-                    double* W, uint8_t* cell_status, void* stream) {
+int sfx_update_step_timed(sfx_solver* h, int64_t C, double* Q, double* dQ,
+                          const double* volume, double* W, const uint64_t* endtime,
+                          uint64_t current_time, uint8_t* cell_status, void* stream) {
     if(!h) return set_error(SFX_ERR_ARG, "sfx_update_step: NULL handle");
     if(C < 0) return set_error(SFX_ERR_ARG, "sfx_update_step: C < 0");
     if(C == 0) return SFX_OK;
@@ -341,10 +342,15 @@ int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const doubl
     int rc = bind_device(h);
     if(rc) return rc;
     unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
-    hipError_t e = launch_update(h->ndim, h->c, C, Q, dQ, volume, W, cell_status, ctr,
-                                 (hipStream_t)stream);
+    hipError_t e = launch_update(h->ndim, h->c, C, Q, dQ, volume, W, endtime, current_time,
+                                 cell_status, ctr, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "update kernel launch");
     return SFX_OK;
+}
+
+int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const double* volume,
+                    double* W, uint8_t* cell_status, void* stream) {
+    return sfx_update_step_timed(h, C, Q, dQ, volume, W, nullptr, 0, cell_status, stream);
 }
 
 }  // extern "C"

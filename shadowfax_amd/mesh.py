@@ -251,8 +251,13 @@ def cfl_dt(mesh, W, vel, gamma, cfl=0.4):
     return cfl * float(np.min(h / (dv + cs)))
 
 
-def prepare(mesh, W, gamma=5.0 / 3.0, kind=0, vel=None, dt=None, cfl=0.4):
-    """Everything the flux pass reads, as numpy arrays (FP64)."""
+def prepare(mesh, W, gamma=5.0 / 3.0, kind=0, vel=None, dt=None, cfl=0.4, active=None):
+    """Everything the flux pass reads, as numpy arrays (FP64).
+
+    dt: None (one global CFL step, cfl_dt), a number (the same step for every
+    cell) or an array [C] of per-cell steps (individual time steps, e.g.
+    sfx_hydro_timestep's dt); active: [C] flags of the cells whose step starts
+    now (the face pass skips faces between two inactive cells), None: all."""
     nd, C = mesh["ndim"], mesh["C"]
     b = dict(mesh)
     b["W"] = np.ascontiguousarray(W, np.float64)
@@ -263,9 +268,14 @@ def prepare(mesh, W, gamma=5.0 / 3.0, kind=0, vel=None, dt=None, cfl=0.4):
     b["total_area"] = total_area(mesh)
     if dt is None:
         dt = cfl_dt(mesh, b["W"], b["vel"], gamma, cfl)
-    b["dt_value"] = dt
-    b["dt"] = np.full(C, dt)
-    b["active"] = np.ones(C, np.uint8)
+    if np.ndim(dt) == 0:
+        b["dt_value"] = float(dt)
+        b["dt"] = np.full(C, float(dt))
+    else:
+        b["dt"] = np.ascontiguousarray(dt, np.float64).reshape(C)
+        b["dt_value"] = float(b["dt"].min()) if C else 0.0
+    b["active"] = (np.ones(C, np.uint8) if active is None
+                   else np.ascontiguousarray(active, np.uint8).reshape(C))
     b.update(face_geometry(mesh))
     b["cf_entry"] = cell_entries(mesh)
     b["gather_off"], b["gather_entry"] = gather_index(mesh)

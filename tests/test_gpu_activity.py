@@ -1,0 +1,205 @@
+"""GPU parity of the flux pass and the update with individual time steps.
+
+The per-cell steps and active flags come from the device time step
+(sfx_hydro_timestep, checked bit for bit against the oracle's here), run
+twice as in tests/timesteps.py: the first call puts every cell on a step
+(about half the cells on the smaller one; the 3D blast down its whole
+hierarchy), the second starts the next step of the cells on the smallest
+step, leaving the others inactive.  Then:
+  * the FP64 pass equals the oracle to 1e-12 (gradients of the active cells
+    bit for bit; skipped faces zero);
+  * the mixed pass is within 1e-6 of the FP64 pass, per face and per cell,
+    and conservative;
+  * with steps 40 times the CFL time the mass-flux limiter caps faces on the
+    GPU as in the oracle;
+  * the timed update rebuilds W only where the step ends (bit for bit).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import timesteps as TS
+from shadowfax_amd import mesh as M
+from shadowfax_amd.fluxpass import FluxPass
+from shadowfax_amd.metrics import face_flux_scale, fluxpass_errors
+from shadowfax_amd.riemann import RiemannSolver
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = 5.0 / 3.0
+KEYS = ("pos", "volume", "centroid", "left", "right", "area", "midpoint", "shift", "ghost",
+        "twin", "cf_off", "cf_face", "W", "vel")
+
+
+def golden_mesh(name):
+    nd = 3 if name.endswith("3d") else 2
+    z = np.load(os.path.join(HERE, "golden", "mesh_%dd.npz" % nd))
+    m = {k: z[name + "_" + k] for k in KEYS}
+    m.update(ndim=nd, C=m["volume"].shape[0], F=m["area"].shape[0], box=np.ones(nd))
+    return m, m["W"], m["vel"]
+
+
+def case_mesh(name):
+    if name in ("per12", "mir10", "p3d", "m3d"):
+        return golden_mesh(name)
+    if name == "kh512":
+        m = M.voronoi_periodic(M.hilbert_sorted(M.jittered_grid(512, 0.25, seed=20240603)))
+        W = M.kh_ics(m)
+        return m, W, W[1:3].copy()
+    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(20000, seed=20240604)))
+    W = M.sedov_ics(m)
+    return m, W, W[1:4].copy()
+
+
+def device_hierarchy(b, kind, dev, **kw):
+    """The oracle's two time-step calls (tests/timesteps.py) repeated with
+    sfx_hydro_timestep on the device; both must agree bit for bit."""
+    import torch
+    h = TS.hierarchy(b, kind, **kw)
+    s = RiemannSolver(kind, b["ndim"], G, 1e-8, -5.0, "fp64")
+    t = lambda a, dt: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    W, vel, V = t(b["W"].T, torch.float64), t(b["vel"].T, torch.float64), t(b["volume"], torch.float64)
+    start = torch.zeros(b["C"], dtype=torch.int64, device=dev)
+    end = torch.zeros(b["C"], dtype=torch.int64, device=dev)
+    cfl = 0.4 * kw.get("scale", 1.0)
+    for call, step in (("first", 0), ("second", h["S"])):
+        tl = dict(integer_maxtime=TS.MAXTIME, current_time=0, timestep=step,
+                  max_timestep=h["smax"], min_timestep=h["smin"], tottime=h["tottime"],
+                  mintime=0.0, cfl=cfl, global_timestep=0)
+        dt, act, st, res = s.hydro_timestep(W, vel, V, tl, start, end)
+        torch.cuda.synchronize()
+        so, eo, dto, ao = h[call]
+        assert np.array_equal(start.cpu().numpy().astype(np.uint64), so), call
+        assert np.array_equal(end.cpu().numpy().astype(np.uint64), eo), call
+        assert np.array_equal(dt.cpu().numpy(), dto), call
+        assert np.array_equal(act.cpu().numpy(), ao), call
+        assert int(st.max()) == 0
+    return h
+
+
+def run_gpu(b, kind, precision, dev):
+    import torch
+    s = RiemannSolver(kind, b["ndim"], G, 1e-8, -5.0, precision)
+    fp = FluxPass(s, M.DeviceMesh(b, precision, dev))
+    fp.run()
+    torch.cuda.synchronize()
+    return fp
+
+
+@pytest.mark.parametrize("name", ["per12", "mir10", "p3d", "m3d", "kh512", "sedov20k"])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_individual_steps_fp64_and_mixed(device, name, kind):
+    print("case", name, kind, flush=True)
+    m, W, vel = case_mesh(name)
+    b0 = M.prepare(m, W, G, kind=kind, vel=vel)
+    blast = name == "sedov20k"
+    h = device_hierarchy(b0, kind, device, two_levels=not blast)
+    for call in ("first", "second"):
+        s_, e_, dt, act = h[call]
+        b = M.prepare(m, W, G, kind=kind, vel=vel, dt=dt, active=act)
+        print("  %s call: %d of %d cells active, %d distinct steps"
+              % (call, act.sum(), b["C"], len(np.unique(dt))), flush=True)
+        if call == "first":
+            assert act.all() and len(np.unique(dt)) >= 2
+        elif not blast:
+            assert 0.2 < act.mean() < 0.8
+        else:
+            assert 0 < act.sum() < b["C"]
+        fp = run_gpu(b, kind, "fp64", device)
+        ob = dict(b)
+        ob["grad"] = O.gradients(kind, b)
+        odQ, odQL, ost, _ = O.face_flux_pass(kind, ob)
+        g = fp.m.grad_soa()
+        a = act == 1
+        assert np.array_equal(g[:, :, a], ob["grad"][:, :, a]), "active gradients bit for bit"
+        dQL = fp.m.dQL_soa()
+        assert np.array_equal(fp.m.fstatus.cpu().numpy() != 0, ost != 0)
+        skipped = ~np.any(odQL != 0, axis=0)
+        assert np.all(dQL[:, skipped] == 0)
+        den = np.maximum(np.abs(odQL).max(0), face_flux_scale(b, G))
+        err = (np.abs(dQL - odQL).max(0) / np.where(den > 0, den, 1.0)).max()
+        dQ = fp.m.dQ_soa()
+        errc = np.abs(dQ - odQ).max() / np.abs(odQ).max()
+        print("  fp64 face err %.3e cell err %.3e" % (err, errc), flush=True)
+        assert err <= 1e-12 and errc <= 1e-12
+        fm = run_gpu(b, kind, "mixed", device)
+        errs = fluxpass_errors(b, G, dQL, dQ, fm.m.dQL_soa(), fm.m.dQ_soa(), once_per_pair=True)
+        print("  mixed vs fp64 %s" % errs, flush=True)
+        assert errs["face_max"] <= 1e-6 and errs["cell_gross_max"] <= 1e-6
+        assert errs["conservation"] <= 1e-13
+        del fp, fm
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_mass_flux_limiter_fires_on_gpu(device, kind):
+    """A static KH mesh with steps 40 times the CFL time, half the cells
+    active: the GPU caps the same faces as the oracle, at the area share of
+    the mass times dt / max(dt_L, dt_R)."""
+    m = M.voronoi_periodic(M.jittered_grid(48, 0.25, seed=2))
+    W = M.kh_ics(m)
+    vel = np.zeros((2, m["C"]))
+    b0 = M.prepare(m, W, G, kind=kind, vel=vel)
+    h = device_hierarchy(b0, kind, device, scale=40.0)
+    s2, e2, dt2, a2 = h["second"]
+    b = M.prepare(m, W, G, kind=kind, vel=vel, dt=dt2, active=a2)
+    ob = dict(b)
+    ob["grad"] = O.gradients(kind, b)
+    odQ, odQL, ost, _ = O.face_flux_pass(kind, ob)
+    L, R = b["left"], b["right"]
+    dt = np.minimum(dt2[L], dt2[R])
+    tf = dt / np.maximum(dt2[L], dt2[R])
+    capL = b["area"] / b["total_area"][L] * b["mass"][L] * tf
+    capR = b["area"] / b["total_area"][R] * b["mass"][R] * tf
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+        fp = run_gpu(b, kind, prec, device)
+        dQL = fp.m.dQL_soa()
+        if prec == "mixed":
+            # the twin copy of a periodic pair holds the negated flux
+            tw = b["twin"]
+            sk = (b["ghost"] == 1) & (tw >= 0) & (tw < np.arange(b["F"]))
+            dQL[:, sk] = -dQL[:, tw[sk]]
+        atL = np.abs(dQL[0] - capL) <= 1e-13 * capL
+        atR = np.abs(-dQL[0] - capR) <= 1e-13 * capR
+        print(prec, "faces at the cap:", int(atL.sum() + atR.sum()),
+              "with tf < 1:", int(((atL | atR) & (tf < 1)).sum()), flush=True)
+        assert ((atL | atR) & (tf < 1)).sum() > 10
+        assert np.all(dQL[0] <= capL * (1 + 1e-13)) and np.all(-dQL[0] <= capR * (1 + 1e-13))
+        den = np.maximum(np.abs(odQL).max(0), face_flux_scale(b, G))
+        err = (np.abs(dQL - odQL).max(0) / np.where(den > 0, den, 1.0)).max()
+        print(prec, "face err vs oracle %.3e" % err, flush=True)
+        assert err <= tol
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_timed_update_matches_oracle(device, kind):
+    """sfx_update_step_timed after the all-active first pass: Q -= dQ for
+    every cell, W rebuilt (and the pressure floor's energy reset applied)
+    only where endtime equals the time after the step, bit for bit against
+    the oracle."""
+    import torch
+    m = M.voronoi_periodic(M.jittered_grid(40, 0.25, seed=4))
+    W = M.kh_ics(m)
+    b0 = M.prepare(m, W, G, kind=kind)
+    h = device_hierarchy(b0, kind, device)
+    s1, e1, dt1, a1 = h["first"]
+    b = M.prepare(m, W, G, kind=kind, dt=dt1, active=a1)
+    # a few cells whose energy falls to the pressure floor in the update
+    low = np.arange(b["C"]) % 37 == 0
+    fp = run_gpu(b, kind, "fp64", device)
+    dQ = fp.m.dQ_soa()
+    Qin = b["Q"].copy()
+    Qin[-1, low] = 0.5 * (Qin[1, low] ** 2 + Qin[2, low] ** 2) / Qin[0, low] + dQ[-1, low]
+    fp.m.Q.copy_(torch.tensor(Qin.T, dtype=torch.float64, device=device))
+    tnow = h["S"]
+    ends = e1 == tnow
+    assert (low & ends).sum() > 0 and (low & ~ends).sum() > 0
+    endt = torch.tensor(e1.astype(np.int64), device=device)
+    fp.update(endtime=endt, current_time=tnow)
+    torch.cuda.synchronize()
+    Qo, Wo, cst = O.update(kind, 2, Qin, dQ, b["volume"], W=W, endtime=e1, tnow=tnow)
+    assert np.array_equal(fp.m.Q_soa(), Qo)
+    assert np.array_equal(fp.m.W_soa(), Wo)
+    assert np.abs(fp.m.dQ_soa()).max() == 0.0
+    assert np.array_equal(fp.m.cstatus.cpu().numpy(), cst)

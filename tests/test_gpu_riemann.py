@@ -8,6 +8,12 @@
     normwise relative error <= 1e-6 against the FP64 results.
   * full-size (4,194,304 problems, the benchmark configuration) properties:
     mixed vs FP64 error, uniform state gives the Euler flux, solver counters.
+  * edge fixtures (exact-zero densities, negative pressures, NaNs): the status
+    bits of the reference's exceptions and the positions of NaN fluxes must
+    agree exactly, the finite fluxes to the tolerance of the path.
+  * ExactRiemannSolver::solve (batched and scalar drop-ins): half states,
+    Mach numbers and status against the golden vectors, and the reflective
+    write-back of WR.
 """
 import os
 
@@ -46,8 +52,22 @@ def gpu_flux(kind, ndim, WL, WR, n, v, refl, dev, precision="fp64", f32=False, g
     return F.double().cpu().numpy(), st.cpu().numpy(), s
 
 
+FAMILIES = ["uniform", "mixed", "vacuum", "shock", "zero", "negp", "nan"]
+
+
+def finite_err(F, R, WL, WR, ok):
+    """NaN fluxes must sit where the reference's are; the normwise error of
+    the others (summarized)."""
+    nanF, nanR = np.isnan(F[:, ok]).any(0), np.isnan(R[:, ok]).any(0)
+    assert np.array_equal(nanF, nanR), "NaN fluxes differ from the reference's"
+    fin = np.where(ok)[0][~nanR]
+    s = summarize(normwise_error(F[:, fin], R[:, fin], WL[:, fin], WR[:, fin], G))
+    s["nan"] = int(nanR.sum())
+    return s
+
+
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
+@pytest.mark.parametrize("fam", FAMILIES)
 @pytest.mark.parametrize("variant", ["plain", "moving"])
 @pytest.mark.parametrize("kind", [0, 1])
 def test_fp64_matches_reference_golden(device, ndim, fam, variant, kind):
@@ -60,16 +80,85 @@ def test_fp64_matches_reference_golden(device, ndim, fam, variant, kind):
     ref_st = g[key + "st_" + name]
     assert np.array_equal(st != 0, ref_st != 0)
     ok = ref_st == 0
-    err = normwise_error(F[:, ok], g[key + "F_" + name][:, ok], g[key + "WL"][:, ok],
-                         g[key + "WR"][:, ok], G)
-    s = summarize(err)
-    bitwise = np.mean(np.all(F[:, ok] == g[key + "F_" + name][:, ok], axis=0))
+    R = g[key + "F_" + name]
+    s = finite_err(F, R, g[key + "WL"], g[key + "WR"], ok)
+    bitwise = np.mean(np.all((F[:, ok] == R[:, ok]) | np.isnan(R[:, ok]), axis=0))
     print("fp64 %dd %s %s %s: err %s bitwise %.4f" % (ndim, fam, variant, name, s, bitwise))
     assert s["max"] <= FP64_TOL
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("fam", ["uniform", "mixed", "vacuum", "shock"])
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_exact_solve_batch_matches_reference_golden(device, ndim, fam):
+    """sfx_exact_solve_batch (ExactRiemannSolver::solve over a batch): half
+    states at x/t = 0, the Mach number of the strongest shock (from 0) and
+    the status, against the compiled reference, plain and moving (25%
+    reflective) variants."""
+    import torch
+    g = np.load(os.path.join(HERE, "golden", "riemann_%dd.npz" % ndim))
+    for variant in ("plain", "moving"):
+        key = "%s_%s_" % (fam, variant)
+        print("case", ndim, key, flush=True)
+        WL, WR, n, refl = (g[key + k] for k in ("WL", "WR", "n", "refl"))
+        N = WL.shape[1]
+        s = RiemannSolver(0, ndim, G, 1e-8, -5.0, "fp64")
+        mach = torch.zeros(N, dtype=torch.float64, device=device)
+        st = torch.zeros(N, dtype=torch.uint8, device=device)
+        f64 = torch.float64
+        Wh = s.solve_batch(_t(WL, f64, device), _t(WR, f64, device), _t(n, f64, device),
+                           _t(refl, torch.uint8, device), mach=mach, status=st)
+        torch.cuda.synchronize()
+        Wh, mach, st = Wh.cpu().numpy(), mach.cpu().numpy(), st.cpu().numpy()
+        ref_st = g[key + "st_solve"]
+        assert np.array_equal(st != 0, ref_st != 0)
+        ok = ref_st == 0
+        R = g[key + "Whalf"]
+        scale = np.maximum(np.abs(WL).max(0), np.abs(np.where(refl, WL, WR)).max(0))
+        scale = np.where(np.isfinite(scale) & (scale > 0), scale, 1.0)
+        assert np.array_equal(np.isnan(Wh[:, ok]), np.isnan(R[:, ok]))
+        err = np.nan_to_num(np.abs(Wh - R)).max(0)[ok] / scale[ok]
+        me = np.abs(mach - g[key + "mach"])[ok] / np.maximum(1.0, np.abs(g[key + "mach"][ok]))
+        print("  half state err %.3e  mach err %.3e" % (err.max(), me.max()), flush=True)
+        assert err.max() <= FP64_TOL
+        assert me.max() <= FP64_TOL
+
+
+@pytest.mark.parametrize("fam", FAMILIES)
+def test_scalar_solve_matches_reference_golden(device, fam):
+    """The scalar drop-in sfx_solve (RiemannSolver::solve, host pointers) on
+    48 problems of each family, a third of them reflective: half state,
+    Mach number, the RiemannSolverException (raised as in the reference)
+    and the write-back of WR = WL on reflective faces."""
+    from shadowfax_amd.riemann import RiemannSolverException
+    g = np.load(os.path.join(HERE, "golden", "riemann_3d.npz"))
+    key = "%s_moving_" % fam
+    s = ExactRiemannSolver(G, ndim=3)
+    refl_all = g[key + "refl"]
+    idx = np.concatenate([np.where(refl_all == 1)[0][:16], np.where(refl_all == 0)[0][:32]])
+    for i in idx:
+        WL = g[key + "WL"][:, i].copy()
+        WR = g[key + "WR"][:, i].copy()
+        refl = bool(refl_all[i])
+        ref_st = g[key + "st_solve"][i]
+        try:
+            W, mach = s.solve(WL, WR, g[key + "n"][:, i], 0.0, reflective=refl)
+            raised = False
+        except RiemannSolverException:
+            raised = True
+        assert raised == (ref_st != 0), i
+        if refl:
+            assert np.array_equal(WR, WL, equal_nan=True), i
+        else:
+            assert np.array_equal(WR, g[key + "WR"][:, i], equal_nan=True), i
+        if not raised:
+            R = g[key + "Whalf"][:, i]
+            scale = max(np.abs(WL).max(), np.abs(WR).max(), 1e-300)
+            assert np.abs(W - R).max() <= FP64_TOL * scale, (i, W, R)
+            assert abs(mach - g[key + "mach"][i]) <= FP64_TOL * max(1.0, abs(mach)), i
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("fam", FAMILIES)
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("f32", [True, False])
 def test_mixed_within_1e6_of_fp64(device, ndim, fam, kind, f32):
@@ -80,11 +169,13 @@ def test_mixed_within_1e6_of_fp64(device, ndim, fam, kind, f32):
     F, st, _ = gpu_flux(kind, ndim, g[key + "WL"], g[key + "WR"], g[key + "n"], g[key + "v"],
                         g[key + "refl"], device, precision="mixed", f32=f32)
     ref_st = g[key + "st_" + name]
+    if fam in ("negp", "nan"):
+        # failing problems fail in the mixed path too
+        assert np.array_equal(st != 0, ref_st != 0)
     ok = (ref_st == 0) & (st == 0)
-    assert ok.mean() > 0.99
-    err = normwise_error(F[:, ok], g[key + "F_" + name][:, ok], g[key + "WL"][:, ok],
-                         g[key + "WR"][:, ok], G)
-    s = summarize(err)
+    if fam not in ("negp", "nan"):
+        assert ok.mean() > 0.99
+    s = finite_err(F, g[key + "F_" + name], g[key + "WL"], g[key + "WR"], ok)
     print("mixed %dd %s %s f32=%s: err %s" % (ndim, fam, name, f32, s))
     assert s["max"] <= MIXED_TOL
 
