@@ -1,18 +1,34 @@
-"""Benchmark of the batched Riemann face-flux workload on one MI355X.
+"""Benchmark of the face-flux path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): 4,194,304
-independent 3D problems, gamma = 5/3, rho and p log-uniform in [1e-3, 1e3],
-velocities U(-2, 2), normals uniform on the sphere, plus 5% near-vacuum faces
-(rho 1e-8, p 1e-10 on one side) and 5% strong shocks (pressure ratio 1e6);
-seeded, synthetic (shadowfax_amd.workloads.RIEMANN_4M).
+Headline workload: one full face-flux pass over the 3D benchmark mesh --
+2,097,152 Poisson generators in the 3D periodic unit box (seed 20240604) in
+Hilbert order, Voronoi faces from the host tessellator (16.5M faces), Sedov
+blast (rho 1, p 1e-5, v 0, E = 1 in the 32 cells nearest the centre), exact
+Riemann solver.  One step = gradients (VorCell::estimate_gradient) + face
+fluxes (VorFace::set_v + calculate_flux) + the ordered per-cell gather (the
+scatter of VorTessManager::hydro's face loop), in the mixed-precision path
+(FP32 bulk, FP64 final Newton step for p*, FP64 conserved accumulation),
+with the mesh, states and work buffers resident in HBM.  The plain all-FP64
+pass is timed the same way; their ratio and the mixed-vs-FP64 errors are
+reported with the line.
 
-One step = one solve_for_flux pass of the exact Riemann solver over the whole
-batch in the mixed-precision path (FP32 storage and bulk arithmetic, FP64
-final Newton step for p*), inputs resident in HBM.  The FP64 plain path and
-the HLLC solver are timed the same way and reported alongside, with the
-mixed-vs-FP64 error.
+Also measured (extra objects of the same JSON line): the 2D pass (512 x 512
+jittered periodic Kelvin-Helmholtz mesh, HLLC), the batched Riemann solves
+(4,194,304 problems with 5% near-vacuum and 5% 1e6 shocks; 262,144
+problems), and the 2,000-mesh serving run (p50/p99 request latency).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+                    [--no-riemann] [--no-2d] [--no-serve]
+
+roofline: the pass's dominant kernel (the face-flux kernel), its counted
+FP operations (profiles/flop_counts.json, the reference algorithm on these
+exact inputs) and its algorithmic bytes per launch over its mean launch time
+(HIP events on the stream it runs on), against the FP64/FP32 vector peak
+(78.6 TF; non-packed FP32 issues at the FP64 rate) and 8 TB/s; bound = the
+nearer roof.  The measured HBM traffic comes from profiles/hbm_traffic.json.
+cpu_baseline: the oracle's FP64 restatement of the same pass over the same
+mesh on the host's cores (threads = the job's CPU share), and one thread on
+a contiguous sample.
 """
 import argparse
 import json
@@ -32,6 +48,8 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 FP32_PEAK_TF = 157.3
 FP64_PEAK_TF = 78.6
 METRIC = "face-fluxes/sec at ≤1e-6 rel err vs FP64 plain GPU; p50/p99 served-step latency"
+SEED_3D = 20240604
+SEED_2D = 20240603
 
 
 def log(*a):
@@ -39,6 +57,9 @@ def log(*a):
 
 
 def time_variant(fn, steps, warmup, stream):
+    """warmup untimed calls, then steps calls each bracketed by HIP events on
+    `stream` (the stream the kernels are launched on), synchronised before and
+    after."""
     import torch
     for _ in range(warmup):
         fn()
@@ -57,14 +78,88 @@ def time_variant(fn, steps, warmup, stream):
             "ms_p50": float(np.median(per)), "wall_s": wall}
 
 
-def bench_fluxpass(b, kind, steps, warmup, dev, alg_bytes):
+def host_threads():
+    """The job's CPU share: OMP_NUM_THREADS when the environment sets it (16
+    per GPU on the MI355X boxes), else every core in the affinity mask."""
+    aff = len(os.sched_getaffinity(0))
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(aff, n) if n > 0 else aff), aff
+
+
+def mesh3d():
+    from shadowfax_amd import mesh as M
+    t0 = time.perf_counter()
+    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(2097152, seed=SEED_3D)),
+                           nthreads=host_threads()[0])
+    t_mesh = time.perf_counter() - t0
+    b = M.prepare(m, M.sedov_ics(m), 5.0 / 3.0, kind=0)
+    return b, t_mesh
+
+
+def mesh2d(n=512):
+    from shadowfax_amd import mesh as M
+    t0 = time.perf_counter()
+    # generators in Hilbert order, as the reference keeps its particles
+    m = M.voronoi_periodic(M.hilbert_sorted(M.jittered_grid(n, 0.25, seed=SEED_2D)),
+                           nthreads=host_threads()[0])
+    t_mesh = time.perf_counter() - t0
+    b = M.prepare(m, M.kh_ics(m), 5.0 / 3.0, kind=1)
+    return b, t_mesh
+
+
+def alg_bytes(b, precision, part="pass"):
+    """Algorithmic bytes: every unique input read once, every output written
+    once, at the variant's storage width.  Per face: the two cell indices (2 x
+    4), the area, the midpoint and the periodic shift (1 + 2 ndim values);
+    per cell: W (ndim + 2, FP64 in both variants), the gradients (ndim (ndim
+    + 2)), centroid, generator position and velocity (3 ndim), mass, total
+    area and dt (3, FP64); written: dQ (ndim + 2, FP64) for the pass, dQL per
+    face (ndim + 2 values) for the face kernel alone, which reads no
+    centroid or position beyond the face's own geometry.  FP64 storage:
+    48 F + 200 C (2D) and 64 F + 296 C (3D)."""
+    nd, C, F = b["ndim"], b["C"], b["F"]
+    NC = nd + 2
+    t = 8 if precision == "fp64" else 4
+    face = 8 + t * (1 + 2 * nd)
+    cell_in = 8 * NC + t * nd * NC + t * 3 * nd + 8 * 3
+    if part == "pass":
+        return face * F + (cell_in + 8 * NC) * C
+    # the face kernel: face geometry in, the two cells' states, gradients,
+    # velocities and scalars in (no centroid/position: those enter through
+    # the face geometry), dQL out
+    cell_face = 8 * NC + t * nd * NC + t * nd + 8 * 3
+    return (face + t * NC) * F + cell_face * C
+
+
+def load_profile(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def measured_traffic(kernel, grid=None):
+    """HBM bytes per launch of a kernel from the committed hardware-counter
+    summary (profiles/hbm_traffic.json), if it was measured at this grid."""
+    d = load_profile("hbm_traffic.json")
+    k = d["kernels"].get(kernel) if d else None
+    if not k or (grid is not None and int(k.get("grid", -1)) != grid):
+        return None
+    return float(k["traffic_bytes"])
+
+
+def bench_fluxpass(b, kind, steps, warmup, dev):
     """One full face-flux pass per step (gradients + face fluxes + ordered
-    gather) over a prepared mesh, in the FP64 and the mixed path; per-kernel
-    times, mixed-vs-FP64 errors and conservation."""
+    gather) in the FP64 and the mixed path: pass and per-kernel times,
+    mixed-vs-FP64 errors, conservation, the per-tessellation record build."""
     import torch
+    from shadowfax_amd import mesh as M
     from shadowfax_amd.fluxpass import FluxPass
     from shadowfax_amd.metrics import fluxpass_errors
-    from shadowfax_amd import mesh as M
     from shadowfax_amd.riemann import RiemannSolver
     C, F, nd = b["C"], b["F"], b["ndim"]
     stream = torch.cuda.current_stream()
@@ -73,68 +168,172 @@ def bench_fluxpass(b, kind, steps, warmup, dev, alg_bytes):
     for prec in ("fp64", "mixed"):
         s = RiemannSolver(kind, nd, 5.0 / 3.0, 1e-8, -5.0, prec)
         s.enable_stats(False)
-        fp = FluxPass(s, M.DeviceMesh(b, prec, dev))
-        parts = {"gradients": fp.gradients, "faces": lambda fp=fp: fp.face_fluxes(status=False),
-                 "gather": fp.gather}
+        dm = M.DeviceMesh(b, prec, dev)
+        # the per-tessellation record build (sfx_mesh_entry_geometry), timed once
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        dm.build_entries(s.handle)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms_build = ev[0].elapsed_time(ev[1])
+        fp = FluxPass(s, dm)
         r = time_variant(lambda fp=fp: fp.run(status=False), steps, warmup, stream)
+        r["ms_build_records"] = ms_build
+        parts = {"gradients": fp.gradients,
+                 "faces": lambda fp=fp: fp.face_fluxes(status=False),
+                 "gather": fp.gather}
         for name, fn in parts.items():
             r["ms_" + name] = time_variant(fn, steps, 1, stream)["ms_mean"]
         r["face_fluxes_per_s"] = F / (r["ms_mean"] * 1e-3)
+        r["fresh_mesh_ms"] = r["ms_mean"] + ms_build
         fp.run()
         torch.cuda.synchronize()
-        res[prec] = (fp.m.dQL_soa(), fp.m.dQ_soa(),
-                     int((fp.m.fstatus != 0).sum().item()))
+        res[prec] = (fp.m.dQL_soa(), fp.m.dQ_soa(), int((fp.m.fstatus != 0).sum().item()))
         r["failed_faces"] = res[prec][2]
+        r["alg_bytes_pass"] = alg_bytes(b, prec)
+        r["alg_bytes_faces"] = alg_bytes(b, prec, "faces")
         out[prec] = r
-        del fp
+        del fp, dm
         torch.cuda.empty_cache()
     out["mixed_speedup"] = out["fp64"]["ms_mean"] / out["mixed"]["ms_mean"]
     out["mixed_vs_fp64"] = fluxpass_errors(b, 5.0 / 3.0, res["fp64"][0], res["fp64"][1],
                                            res["mixed"][0], res["mixed"][1], once_per_pair=True)
+    ef = face_errors(b, res["fp64"][0], res["mixed"][0])
+    out["mixed_vs_fp64"]["faces_above_1e-6"] = int((ef > 1e-6).sum())
     out["fp64_conservation"] = float(np.abs(res["fp64"][1].sum(1)).max() /
                                      np.abs(res["fp64"][0]).max())
-    out["alg_bytes_fp64"] = alg_bytes
-    out["fp64"]["achieved_GBs"] = alg_bytes / (out["fp64"]["ms_mean"] * 1e-3) / 1e9
     return out
 
 
-def bench_fluxpass2d(steps, warmup, dev, n=512, kind=1):
-    """The 2D pass: 512^2 jittered periodic Kelvin-Helmholtz mesh, HLLC."""
-    from shadowfax_amd import mesh as M
-    t0 = time.perf_counter()
-    # generators in Hilbert order, as the reference keeps its particles
-    m = M.voronoi_periodic(M.hilbert_sorted(M.jittered_grid(n, 0.25, seed=20240603)),
-                           nthreads=host_threads())
-    b = M.prepare(m, M.kh_ics(m), 5.0 / 3.0, kind=kind)
-    t_mesh = time.perf_counter() - t0
-    # algorithmic bytes of the FP64 pass: faces {iL, iR, A, midpoint, shift}
-    # 48 B, cells {W, grad, centroid, pos, vel, m, A_tot, dt} 168 B read +
-    # dQ 32 B written
-    out = bench_fluxpass(b, kind, steps, warmup, dev, 48 * b["F"] + 200 * b["C"])
-    out["host_mesh_s"] = t_mesh
-    out["workload"] = ("2D periodic unit box, %dx%d jittered generators (seed 20240603) in "
-                       "Hilbert order, Kelvin-Helmholtz, HLLC" % (n, n))
+def face_errors(b, dQL64, dQLmx):
+    from shadowfax_amd.metrics import face_flux_scale
+    d = np.asarray(dQLmx, np.float64).copy()
+    tw = b["twin"]
+    sk = (b["ghost"] == 1) & (tw >= 0) & (tw < np.arange(b["F"]))
+    d[:, sk] = -d[:, tw[sk]]
+    den = np.maximum(np.abs(dQL64).max(0), face_flux_scale(b, 5.0 / 3.0))
+    return np.abs(d - dQL64).max(0) / np.where(den > 0, den, 1.0)
+
+
+def roofline(b, fp, workload_key, face_kernel, pass_kernels):
+    """The face kernel's roofline (counted operations and algorithmic bytes
+    over its mean launch time) and the pass's (max(T_hbm, T_valu) / t)."""
+    counts = load_profile("flop_counts.json")
+    ms_face = fp["mixed"]["ms_faces"]
+    t = ms_face * 1e-3
+    byt = fp["mixed"]["alg_bytes_faces"]
+    gbs = byt / t / 1e9
+    out = {}
+    cnt = counts.get(workload_key) if counts else None
+    if cnt and cnt["faces"] == b["F"]:
+        ops = cnt["faces_kernel"]["ops"]
+        tf = ops / t / 1e12
+        f_hbm, f_valu = gbs / HBM_PEAK_GBS, tf / FP64_PEAK_TF
+        if f_valu >= f_hbm:
+            out = {"bound": "valu", "achieved": tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
+                   "frac": f_valu}
+        else:
+            out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": f_hbm}
+        out["flops_per_launch"] = ops
+        out["flops_per_face"] = cnt["faces_kernel"]["per_face"]
+        out["transcendentals_per_face"] = cnt["faces_kernel"]["transcendentals_per_face"]
+        out["hbm"] = {"achieved_GBs": gbs, "frac": f_hbm}
+        out["valu"] = {"achieved_TFs": tf, "frac": f_valu,
+                       "peak_note": "FP64 vector peak; non-packed FP32 issues at the same rate"}
+        # the whole pass, SURVEY 8(d)'s definition: max(T_hbm, T_valu) / t
+        pass_ops = cnt["pass_ops"]
+        for prec in ("fp64", "mixed"):
+            tp = fp[prec]["ms_mean"] * 1e-3
+            T_hbm = fp[prec]["alg_bytes_pass"] / (HBM_PEAK_GBS * 1e9)
+            T_valu = pass_ops / (FP64_PEAK_TF * 1e12)
+            out["pass_" + prec] = {"T_hbm_ms": T_hbm * 1e3, "T_valu_ms": T_valu * 1e3,
+                                   "t_ms": tp * 1e3, "frac": max(T_hbm, T_valu) / tp,
+                                   "alg_bytes": fp[prec]["alg_bytes_pass"], "ops": pass_ops}
+    else:
+        out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": gbs / HBM_PEAK_GBS, "note": "no FP operation count for this mesh"}
+    traffic = measured_traffic(face_kernel, b["F"])
+    out["traffic"] = traffic
+    out["traffic_source"] = ("profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                             "dispatch, scaled by a copy kernel of known size)") if traffic else None
+    out["kernel"] = face_kernel
+    out["kernel_ms"] = ms_face
+    out["alg_bytes_per_launch"] = byt
+    out["pass_kernels"] = pass_kernels
     return out
 
 
-def bench_fluxpass3d(steps, warmup, dev, n=2097152, kind=0):
-    """The 3D pass: 2,097,152 Poisson generators in the 3D periodic unit box,
-    Sedov blast (E = 1 in the 32 cells nearest the centre), exact solver."""
-    from shadowfax_amd import mesh as M
-    t0 = time.perf_counter()
-    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(n, seed=20240604)),
-                           nthreads=host_threads())
-    t_mesh = time.perf_counter() - t0
-    b = M.prepare(m, M.sedov_ics(m), 5.0 / 3.0, kind=kind)
-    del m
-    # algorithmic bytes of the FP64 pass: faces {iL, iR 8, A 8, midpoint 24,
-    # shift 24} 64 B; cells {W 40, grad 120, centroid/pos/vel 72, m/A_tot/dt
-    # 24} 256 B read + dQ 40 B written
-    out = bench_fluxpass(b, kind, steps, warmup, dev, 64 * b["F"] + 296 * b["C"])
-    out["host_mesh_s"] = t_mesh
-    out["workload"] = ("3D periodic unit box, %d Poisson generators (seed 20240604) in "
-                       "Hilbert order, Sedov blast, exact solver" % n)
-    return out
+def bench_riemann(steps, warmup, dev, N=None):
+    """Batched solve_for_flux over the 4,194,304-problem batch (5% near-vacuum,
+    5% 1e6 shocks): exact and HLLC, FP64 and mixed, errors and counters; the
+    mixed error also for unrounded FP64 inputs (the batch is otherwise drawn
+    FP32-representable, so both paths read identical inputs)."""
+    import torch
+    from shadowfax_amd import workloads as WK
+    from shadowfax_amd.metrics import normwise_error, summarize
+    from shadowfax_amd.riemann import RiemannSolver
+    cfg = dict(WK.RIEMANN_4M)
+    if N:
+        cfg["N"] = N
+    N = cfg["N"]
+    g = cfg["gamma"]
+    b = WK.riemann_batch(N, seed=cfg["seed"], family=cfg["family"], ndim=3)
+    d64 = {k: torch.tensor(b[k], dtype=torch.float64, device=dev) for k in ("WL", "WR", "n")}
+    d32 = {k: torch.tensor(b[k], dtype=torch.float32, device=dev) for k in ("WL", "WR", "n")}
+    stream = torch.cuda.current_stream()
+    variants, outs = {}, {}
+    for kname, kind, prec in (("exact", 0, "fp64"), ("exact", 0, "mixed"),
+                              ("hllc", 1, "fp64"), ("hllc", 1, "mixed")):
+        s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"], prec)
+        s.enable_stats(False)
+        src = d64 if prec == "fp64" else d32
+        F = torch.empty_like(src["WL"])
+        fn = lambda s=s, src=src, F=F: s.solve_for_flux_batch(src["WL"], src["WR"], src["n"],
+                                                              F=F, stream=stream)
+        log("  timing", kname, prec)
+        r = time_variant(fn, steps, warmup, stream)
+        r["face_fluxes_per_s"] = N / (r["ms_mean"] * 1e-3)
+        s.enable_stats(True)
+        s.reset_stats()
+        st = torch.zeros(N, dtype=torch.uint8, device=dev)
+        s.solve_for_flux_batch(src["WL"], src["WR"], src["n"], F=F, status=st, stream=stream)
+        torch.cuda.synchronize()
+        r["stats"] = s.stats()
+        outs[(kname, prec)] = (F.double().cpu().numpy(), st.cpu().numpy())
+        variants["%s_%s" % (kname, prec)] = r
+    errors = {}
+    for kname in ("exact", "hllc"):
+        F64, st64 = outs[(kname, "fp64")]
+        F32, st32 = outs[(kname, "mixed")]
+        ok = (st64 == 0) & (st32 == 0)
+        errors[kname + "_mixed_vs_fp64"] = summarize(
+            normwise_error(F32[:, ok], F64[:, ok], b["WL"][:, ok], b["WR"][:, ok], g))
+        variants[kname + "_mixed"]["speedup_vs_fp64"] = (
+            variants[kname + "_fp64"]["ms_mean"] / variants[kname + "_mixed"]["ms_mean"])
+    # unrounded inputs: the same draws kept in FP64; the FP64 path reads them
+    # as they are, the mixed path reads their FP32 rounding (its storage)
+    ub = WK.riemann_batch(N, seed=cfg["seed"], family=cfg["family"], ndim=3, round32=False)
+    for kname, kind in (("exact", 0), ("hllc", 1)):
+        Fs = {}
+        for prec, dt in (("fp64", torch.float64), ("mixed", torch.float32)):
+            s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"], prec)
+            st = torch.zeros(N, dtype=torch.uint8, device=dev)
+            F = s.solve_for_flux_batch(*(torch.tensor(ub[k], dtype=dt, device=dev)
+                                         for k in ("WL", "WR", "n")), status=st)
+            torch.cuda.synchronize()
+            Fs[prec] = (F.double().cpu().numpy(), st.cpu().numpy())
+        ok = (Fs["fp64"][1] == 0) & (Fs["mixed"][1] == 0)
+        errors[kname + "_mixed_vs_fp64_unrounded_inputs"] = summarize(
+            normwise_error(Fs["mixed"][0][:, ok], Fs["fp64"][0][:, ok], ub["WL"][:, ok],
+                           ub["WR"][:, ok], g))
+        del Fs
+    del d64, d32
+    torch.cuda.empty_cache()
+    return {"workload": "batched Riemann solve_for_flux, %d 3D problems, gamma 5/3, 5%% "
+                        "near-vacuum + 5%% 1e6 shocks (seed %d)" % (N, cfg["seed"]),
+            "N": N, "variants": variants, "errors": errors}
 
 
 def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
@@ -167,7 +366,7 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
         tot_c = sum(ens.cells)
         r["cells"], r["faces"] = int(tot_c), int(faces.sum())
         if mode == S.SERVE_BATCHED:
-            # autotune the persistent launch on a batch of about a million cells
+            # autotune the launch on a batch of about a million cells
             ids, c = [], 0
             for k in range(n_meshes):
                 if c + ens.cells[k] > min(ens.arena_cells, 1_000_000):
@@ -199,35 +398,6 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
     return out
 
 
-def measured_traffic(kernel, grid):
-    """HBM bytes per launch of a kernel from the committed hardware-counter
-    summary (profiles/hbm_traffic.json), if it was measured (at this grid,
-    unless grid is None)."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
-            k = json.load(f)["kernels"].get(kernel)
-    except (OSError, ValueError, KeyError):
-        return None
-    if not k or (grid is not None and int(k.get("grid", -1)) != grid):
-        return None
-    return float(k["traffic_bytes"])
-
-
-def measured_valu(kernel):
-    """VALU wave-instructions per launch of a kernel from the committed
-    hardware-counter summary (profiles/valu_counters.json), if measured."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "valu_counters.json")) as f:
-            k = json.load(f)["kernels"].get(kernel)
-    except (OSError, ValueError, KeyError):
-        return None
-    return float(k["SQ_INSTS_VALU"]) if k else None
-
-
-def host_threads():
-    return max(1, min(16, os.cpu_count() or 1))
-
-
 def copy_probe(mib, dev):
     """A plain elementwise copy of known size (reads and writes mib MiB per
     launch): under a hardware-counter run its FETCH_SIZE / WRITE_SIZE per
@@ -242,43 +412,36 @@ def copy_probe(mib, dev):
     log("copy probe: %d MiB read + %d MiB written per launch" % (mib, mib))
 
 
-def cpu_baseline(b, N):
-    """The reference ExactRiemannSolver compiled from source (oracle/_ref)
-    when present, else the oracle restatement, over the full batch: on one
-    thread (the reference's serial per-rank loop) and on all host threads of
-    this job (contiguous chunks, one solver per thread).  value: the
-    all-threads rate."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_baseline(b, kind):
+    """The oracle's FP64 restatement of the same pass (gradients, face
+    fluxes, ordered gather) over the same mesh: on the job's CPU share of
+    threads over the whole mesh (contiguous cell and face ranges per thread,
+    then the per-cell gather in face order), and on one thread over a
+    contiguous sample (the first 1/16 of the cells and faces: faces are in
+    creation order, so about the same cells).  A 1/64 sample warms each up."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    kind = "port"
-    fn = O.solve_for_flux
-    try:
-        if O.ref_available(3):
-            O.ref(3)
-            fn = O.ref_solve_for_flux
-            kind = "reference"
-    except OSError:
-        pass
-    if kind == "port":
-        O.lib()
-    sl = lambda k, a, z: np.ascontiguousarray(b[k][:, a:z])
-    fn(0, 3, sl("WL", 0, 1000), sl("WR", 0, 1000), sl("n", 0, 1000))  # warm
+    O.lib()
+    T, aff = host_threads()
+    C, F = b["C"], b["F"]
+    cut = lambda n, k: (0, max(1, n // k))
+    O.threaded_pass(kind, b, T, cells=cut(C, 64), faces=cut(F, 64))
+    times = {}
     t0 = time.perf_counter()
-    fn(0, 3, b["WL"], b["WR"], b["n"])
+    O.threaded_pass(kind, b, T, timer=lambda k, s: times.__setitem__(k, s))
+    tT = time.perf_counter() - t0
+    O.threaded_pass(kind, b, 1, cells=cut(C, 256), faces=cut(F, 256))
+    t0 = time.perf_counter()
+    O.threaded_pass(kind, b, 1, cells=cut(C, 16), faces=cut(F, 16))
     t1 = time.perf_counter() - t0
-    T = host_threads()
-    cuts = np.linspace(0, N, T + 1).astype(int)
-    chunks = [(sl("WL", cuts[k], cuts[k + 1]), sl("WR", cuts[k], cuts[k + 1]),
-               sl("n", cuts[k], cuts[k + 1])) for k in range(T)]
-    with ThreadPoolExecutor(T) as ex:
-        t0 = time.perf_counter()
-        list(ex.map(lambda c: fn(0, 3, *c), chunks))
-        tT = time.perf_counter() - t0
-    return {"value": N / tT, "unit": "face-fluxes/s", "cores": T, "kind": kind,
-            "sample": "full batch: %d exact solve_for_flux calls, FP64, %d threads "
-                      "%.2f s (1 thread: %.2f s, %.3g face-fluxes/s)" % (N, T, tT, t1, N / t1),
-            "single_thread": N / t1, "cpu_model": cpu_model()}
+    Fs = cut(F, 16)[1]
+    return {"value": F / tT, "unit": "face-fluxes/s", "cores": T, "kind": "port",
+            "sample": ("the oracle's FP64 pass over the whole mesh (%d cells, %d faces) on %d "
+                       "threads: %.2f s (gradients %.2f, faces %.2f, gather %.2f); one thread "
+                       "over the first %d faces and %d cells: %.2f s, %.3g face-fluxes/s"
+                       % (C, F, T, tT, times["gradients"], times["faces"], times["gather"],
+                          Fs, cut(C, 16)[1], t1, Fs / t1)),
+            "single_thread": Fs / t1, "affinity_cores": aff, "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -298,15 +461,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--N", type=int, default=None)
-    ap.add_argument("--no-fluxpass", action="store_true")
-    ap.add_argument("--no-3d", action="store_true", help="skip the 3D flux pass (2,097,152-cell Sedov mesh)")
+    ap.add_argument("--no-riemann", action="store_true", help="skip the batched Riemann solves")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D flux pass")
     ap.add_argument("--no-serve", action="store_true", help="skip serving (2,000-mesh ensemble)")
+    ap.add_argument("--riemann-N", type=int, default=None)
     ap.add_argument("--serve-meshes", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=4000)
-    ap.add_argument("--only", default=None,
-                    help="comma list of solver variants to time (exact_fp64, exact_mixed, "
-                         "hllc_fp64, hllc_mixed); skips the headline line")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain FP32 copy kernel over this many MiB (5 launches), "
                          "the scale reference for hardware byte counters")
@@ -317,115 +477,61 @@ def main():
         sys.exit(2)
 
     import torch
-    from shadowfax_amd import workloads as WK
-    from shadowfax_amd.metrics import normwise_error, summarize
-    from shadowfax_amd.riemann import RiemannSolver
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    cfg = dict(WK.RIEMANN_4M)
-    if args.N:
-        cfg["N"] = args.N
-    N = cfg["N"]
-    g = cfg["gamma"]
-    log("generating %d problems" % N)
-    b = WK.riemann_batch(N, seed=cfg["seed"], family=cfg["family"], ndim=3)
-    d64 = {k: torch.tensor(b[k], dtype=torch.float64, device=dev) for k in ("WL", "WR", "n")}
-    d32 = {k: torch.tensor(b[k], dtype=torch.float32, device=dev) for k in ("WL", "WR", "n")}
-    stream = torch.cuda.current_stream()
 
-    variants = {}
-    outs = {}
-    only = set(args.only.split(",")) if args.only else None
-    for kname, kind, prec in (("exact", 0, "fp64"), ("exact", 0, "mixed"),
-                              ("exact", 0, "mixed_compacted"), ("hllc", 1, "fp64"),
-                              ("hllc", 1, "mixed")):
-            if only is not None and "%s_%s" % (kname, prec) not in only:
-                continue
-            s = RiemannSolver(kind, 3, g, cfg["tol"], cfg["cutoff"],
-                              "mixed" if prec.startswith("mixed") else prec)
-            if prec == "mixed_compacted":
-                s.set_split(8)  # with the wave-level compaction, for comparison
-            s.enable_stats(False)
-            src = d64 if prec == "fp64" else d32
-            F = torch.empty_like(src["WL"])
-            fn = lambda s=s, src=src, F=F: s.solve_for_flux_batch(src["WL"], src["WR"], src["n"],
-                                                                  F=F, stream=stream)
-            log("timing", kname, prec)
-            r = time_variant(fn, args.steps, args.warmup, stream)
-            r["face_fluxes_per_s"] = N / (r["ms_mean"] * 1e-3)
-            # one counted pass (outside the timed region)
-            s.enable_stats(True)
-            s.reset_stats()
-            st = torch.zeros(N, dtype=torch.uint8, device=dev)
-            s.solve_for_flux_batch(src["WL"], src["WR"], src["n"], F=F, status=st,
-                                   stream=stream)
-            torch.cuda.synchronize()
-            r["stats"] = s.stats()
-            outs[(kname, prec)] = (F.double().cpu().numpy(), st.cpu().numpy())
-            variants["%s_%s" % (kname, prec)] = r
-    errors = {}
-    for kname in ("exact", "hllc"):
-        if (kname, "fp64") not in outs or (kname, "mixed") not in outs:
-            continue
-        F64, st64 = outs[(kname, "fp64")]
-        F32, st32 = outs[(kname, "mixed")]
-        ok = (st64 == 0) & (st32 == 0)
-        errors[kname + "_mixed_vs_fp64"] = summarize(
-            normwise_error(F32[:, ok], F64[:, ok], b["WL"][:, ok], b["WR"][:, ok], g))
-        variants[kname + "_mixed"]["speedup_vs_fp64"] = (
-            variants[kname + "_fp64"]["ms_mean"] / variants[kname + "_mixed"]["ms_mean"])
-
-    if args.copy_probe_mb:
-        copy_probe(args.copy_probe_mb, dev)
-    if "exact_mixed" not in variants:
-        print(json.dumps({"variants": variants}))
-        return
-    head = variants["exact_mixed"]
-    ms = head["ms_mean"]
-    value = N / (ms * 1e-3)
-    bytes_per = (2 * 5 + 3) * 4 + 5 * 4  # WL, WR, n in + F out, FP32
-    achieved = bytes_per * N / (ms * 1e-3) / 1e9
-    kernels = ["sfx::mixed::solve_for_flux_kernel<3, 0, float>"]
-    parts = [measured_traffic(k, None) for k in kernels]
-    traffic = sum(parts) if all(p is not None for p in parts) else None
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "traffic_source": "profiles/hbm_traffic_pmc.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                              "scaled by a copy kernel of known size)" if traffic else None,
-            "kernel": kernels[0] + " (one step: the batched mixed exact solve)",
-            "bytes_per_problem": bytes_per,
-            "note": "exact solver is VALU-bound (Newton on FP32 transcendentals + one FP64 "
-                    "step); HBM fraction shown as the contract asks, VALU issue below"}
-    vi = measured_valu(kernels[0])
-    if vi is not None:
-        va = vi / (ms * 1e-3)
-        roof["valu"] = {"insts_per_launch": vi, "achieved": va, "peak": VALU_PEAK_WAVE_INSTS,
-                        "unit": "wave64 VALU instructions/s", "frac": va / VALU_PEAK_WAVE_INSTS,
-                        "source": "profiles/valu_counters.md (rocprofv3 SQ_INSTS_VALU)"}
-    out = {"metric": METRIC, "value": value, "unit": "face-fluxes/s", "n_gpus": 1,
+    log("3D benchmark mesh (2,097,152 Poisson generators)")
+    b3, t_mesh = mesh3d()
+    log("  %d cells, %d faces, host tessellation %.1f s" % (b3["C"], b3["F"], t_mesh))
+    fp3 = bench_fluxpass(b3, 0, args.steps, args.warmup, dev)
+    fp3["host_mesh_s"] = t_mesh
+    ms = fp3["mixed"]["ms_mean"]
+    F = b3["F"]
+    face_k = "sfx::flux::face_flux_kernel<3, 0, 1, float, double>"
+    roof = roofline(b3, fp3, "3d", face_k,
+                    ["sfx::flux::gradient_kernel<3, float, double, true>", face_k,
+                     "sfx::flux::gather_kernel<3, 1, float>"])
+    out = {"metric": METRIC, "value": F / (ms * 1e-3), "unit": "face-fluxes/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32+f64", "data": "synthetic (seeded, shadowfax_amd.workloads.RIEMANN_4M)",
-           "config": {"workload": "batched Riemann solve_for_flux, 4,194,304 3D problems, "
-                                  "gamma 5/3, 5% near-vacuum + 5% 1e6 shocks; "
-                                  "exact solver, mixed precision",
-                      "N": N, "solver": "exact", "precision": "mixed", "seed": cfg["seed"]},
-           "roofline": roof, "variants": variants, "errors": errors}
-    if not args.no_fluxpass:
+           "dtype": "f32+f64",
+           "data": "synthetic (seeded Poisson generators, host Voronoi tessellation, Sedov "
+                   "blast initial conditions)",
+           "config": {"workload": "one face-flux pass (gradients + face fluxes + ordered "
+                                  "gather), 3D periodic unit box, 2,097,152 Poisson generators "
+                                  "(seed %d), Sedov blast, exact Riemann solver, mixed "
+                                  "precision" % SEED_3D,
+                      "cells": b3["C"], "faces": F, "solver": "exact", "precision": "mixed",
+                      "seed": SEED_3D},
+           "roofline": roof,
+           "fluxpass_3d_sedov": fp3}
+    out["mixed_vs_fp64_speedup"] = fp3["mixed_speedup"]
+    out["errors"] = fp3["mixed_vs_fp64"]
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb, dev)
+    if not args.no_cpu:
+        log("cpu baseline (the oracle's pass over the same mesh)")
+        out["cpu_baseline"] = cpu_baseline(b3, 0)
+    else:
+        out["cpu_baseline"] = None
+    del b3
+    if not args.no_2d:
         log("flux pass 2D (512x512 KH mesh)")
-        out["fluxpass_2d_kh"] = bench_fluxpass2d(args.steps, args.warmup, dev)
-        if not args.no_3d:
-            log("flux pass 3D (2,097,152-cell Sedov mesh)")
-            out["fluxpass_3d_sedov"] = bench_fluxpass3d(args.steps, args.warmup, dev)
+        b2, t2 = mesh2d()
+        fp2 = bench_fluxpass(b2, 1, args.steps, args.warmup, dev)
+        fp2["host_mesh_s"] = t2
+        fp2["workload"] = ("2D periodic unit box, 512x512 jittered generators (seed %d) in "
+                           "Hilbert order, Kelvin-Helmholtz, HLLC" % SEED_2D)
+        fp2["roofline"] = roofline(b2, fp2, "2d", "sfx::flux::face_flux_kernel<2, 1, 1, float, "
+                                                   "double>", [])
+        out["fluxpass_2d_kh"] = fp2
+        del b2
+    if not args.no_riemann:
+        log("batched Riemann solves (4,194,304 problems)")
+        out["riemann_4m"] = bench_riemann(args.steps, args.warmup, dev, args.riemann_N)
     if not args.no_serve:
         log("serving (2,000-mesh ensemble)")
         out["serving"] = bench_serving(dev, args.serve_meshes, args.serve_requests)
-    if not args.no_cpu:
-        log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(b, N)
-    else:
-        out["cpu_baseline"] = None
     print(json.dumps(out))
 
 

@@ -82,7 +82,12 @@ def lib():
         path = os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run make -C oracle")
-        L = ct.CDLL(path)
+        _lib = _bind(ct.CDLL(path))
+    return _lib
+
+
+def _bind(L):
+    if True:
         L.sfxo_solver_init.argtypes = [ct.POINTER(SolverParams), ct.c_int,
                                        ct.c_int, ct.c_double, ct.c_double,
                                        ct.c_double]
@@ -107,6 +112,16 @@ def lib():
         L.sfxo_gradients.argtypes = [ct.POINTER(SolverParams),
                                      ct.POINTER(MeshStruct), _dp]
         L.sfxo_face_velocities.argtypes = [ct.POINTER(MeshStruct), _dp, _dp]
+        L.sfxo_gradients_range.argtypes = [ct.POINTER(SolverParams),
+                                           ct.POINTER(MeshStruct), ct.c_int64,
+                                           ct.c_int64, _dp]
+        L.sfxo_face_flux_range.restype = ct.c_uint
+        L.sfxo_face_flux_range.argtypes = [ct.POINTER(SolverParams),
+                                           ct.POINTER(MeshStruct), ct.c_int64,
+                                           ct.c_int64, _dp, _u8p,
+                                           ct.POINTER(Counters)]
+        L.sfxo_gather.argtypes = [ct.POINTER(MeshStruct), _i64p, _i32p, _dp,
+                                  ct.c_int64, ct.c_int64, _dp]
         L.sfxo_face_flux_pass.restype = ct.c_uint
         L.sfxo_face_flux_pass.argtypes = [ct.POINTER(SolverParams),
                                           ct.POINTER(MeshStruct), _dp, _dp,
@@ -120,8 +135,35 @@ def lib():
         L.sfxo_hydro_timestep.argtypes = [
             ct.POINTER(SolverParams), ct.c_int64, _dp, _dp, _dp,
             ct.POINTER(Timeline), _u64p, _u64p, _dp, _u8p, _u64p, _u64p, _u8p]
-        _lib = L
-    return _lib
+    return L
+
+
+_flops = None
+
+
+def count_flops(fn, *args, **kw):
+    """Run oracle function fn (gradients, face_flux_pass, ...) on the
+    counting build (oracle/libflops.so: the same restatement over a scalar
+    that counts +, -, *, /, sqrt and transcendentals).  Returns (fn's
+    result, operations, transcendentals)."""
+    global _lib, _flops
+    if _flops is None:
+        path = os.path.join(HERE, "libflops.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", HERE, "libflops.so"])
+        L = _bind(ct.CDLL(path))
+        L.sfxf_tally.argtypes = [ct.POINTER(ct.c_ulonglong)]
+        _flops = L
+    saved = lib()
+    _lib = _flops
+    try:
+        _flops.sfxf_reset()
+        out = fn(*args, **kw)
+        t = (ct.c_ulonglong * 2)()
+        _flops.sfxf_tally(t)
+    finally:
+        _lib = saved
+    return out, int(t[0]), int(t[1])
 
 
 def _p(a, t=_dp):
@@ -289,6 +331,56 @@ def face_flux_pass(kind, mesh, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0):
     lib().sfxo_face_flux_pass(ct.byref(s), ct.byref(m), _p(dQ), _p(dQL),
                               _p(st, _u8p), ct.byref(c))
     return dQ, dQL, st, c
+
+
+def threaded_pass(kind, mesh, nthreads, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0,
+                  cells=None, faces=None, timer=None):
+    """The flux pass (gradients, face fluxes, ordered gather) on nthreads
+    threads, each taking a contiguous range of cells or faces (ctypes
+    releases the interpreter lock during the calls).  cells / faces: (lo, hi)
+    ranges to restrict the work to (a sample), default all.  Returns (grad,
+    dQ, dQL, status); timer(name, seconds) is called per phase."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+    ndim = int(mesh["ndim"])
+    s = params(kind, ndim, gamma, tol, cutoff)
+    C, F = int(mesh["C"]), int(mesh["F"])
+    c0, c1 = cells or (0, C)
+    f0, f1 = faces or (0, F)
+    grad = np.zeros((ndim, ndim + 2, C))
+    dQL = np.zeros((ndim + 2, F))
+    st = np.zeros(F, np.uint8)
+    dQ = np.zeros((ndim + 2, C))
+    goff = _c(mesh["gather_off"], np.int64)
+    gent = _c(mesh["gather_entry"], np.int32)
+    L = lib()
+
+    def cuts(lo, hi):
+        e = np.linspace(lo, hi, nthreads + 1).astype(np.int64)
+        return list(zip(e[:-1], e[1:]))
+
+    with ThreadPoolExecutor(nthreads) as ex:
+        m, keep = mesh_struct(mesh)
+        t0 = time.perf_counter()
+        list(ex.map(lambda r: L.sfxo_gradients_range(ct.byref(s), ct.byref(m), int(r[0]),
+                                                     int(r[1]), _p(grad)), cuts(c0, c1)))
+        t1 = time.perf_counter()
+        mm = dict(mesh)
+        mm["grad"] = grad
+        m2, keep2 = mesh_struct(mm)
+        list(ex.map(lambda r: L.sfxo_face_flux_range(ct.byref(s), ct.byref(m2), int(r[0]),
+                                                     int(r[1]), _p(dQL), _p(st, _u8p), None),
+                    cuts(f0, f1)))
+        t2 = time.perf_counter()
+        list(ex.map(lambda r: L.sfxo_gather(ct.byref(m2), _p(goff, _i64p), _p(gent, _i32p),
+                                            _p(dQL), int(r[0]), int(r[1]), _p(dQ)),
+                    cuts(c0, c1)))
+        t3 = time.perf_counter()
+    if timer:
+        timer("gradients", t1 - t0)
+        timer("faces", t2 - t1)
+        timer("gather", t3 - t2)
+    return grad, dQ, dQL, st
 
 
 def update(kind, ndim, Q, dQ, volume, gamma=5.0 / 3.0, W=None, endtime=None, tnow=0):

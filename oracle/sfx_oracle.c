@@ -953,12 +953,13 @@ static void right_gen_pos(const sfxo_mesh* m, int64_t f, double* xr) {
         xr[d] = CELL(m->pos, d, base, m->C) + m->shift[d * m->F + f];
 }
 
-/* VorCell::estimate_gradient, src/VorCell.cpp:464-530 */
-void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m,
-                    double* grad_out) {
+/* VorCell::estimate_gradient, src/VorCell.cpp:464-530, for cells c0..c1-1 */
+void sfxo_gradients_range(const sfxo_solver* s, const sfxo_mesh* m, int64_t c0,
+                          int64_t c1, double* grad_out) {
     int nd = m->ndim, NC = nd + 2;
     int64_t C = m->C, F = m->F;
-    for(int64_t i = 0; i < C; i++) {
+    (void)s;
+    for(int64_t i = c0; i < c1; i++) {
         double delta[3][5];
         for(int d = 0; d < 3; d++)
             for(int k = 0; k < 5; k++) delta[d][k] = 0.;
@@ -1031,6 +1032,11 @@ void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m,
             for(int k = 0; k < NC; k++)
                 grad_out[(int64_t)(l * NC + k) * C + i] = delta[l][k] * alpha[k];
     }
+}
+
+void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m,
+                    double* grad_out) {
+    sfxo_gradients_range(s, m, 0, m->C, grad_out);
 }
 
 /* VorFace::set_v, src/VorFace.cpp:586-660 */
@@ -1122,124 +1128,187 @@ static void predict_side(const sfxo_solver* s, const sfxo_mesh* m,
     for(int k = 0; k < NC; k++) out[k] = Wt[k];
 }
 
+/* VorFace::set_v and VorFace::calculate_flux (src/VorFace.cpp:586-866) for
+ * face f of a single process (every right generator is local, a periodic
+ * copy or a mirror), no gravity: dQLo = the flux exchanged, seen from the
+ * left cell (zero when the face is skipped).  Returns the status bits. */
+static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
+                              int64_t f, double* dQLo, sfxo_counters* c) {
+    int nd = m->ndim, NC = nd + 2;
+    int64_t C = m->C, F = m->F;
+    unsigned st = 0;
+    int32_t L = m->left[f], R = m->right[f];
+    for(int k = 0; k < NC; k++) dQLo[k] = 0.;
+    double xl[3], xr[3], vface[3];
+    for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, L, C);
+    right_gen_pos(m, f, xr);
+    face_velocity(m, f, xl, xr, vface);
+
+    /* gating, :672-683 */
+    if(!(m->area[f] != 0.)) return 0u;
+    int actL = m->active ? m->active[L] : 1;
+    int actR = (R >= 0) ? (m->active ? m->active[R] : 1) : 0;
+    if(R < 0 && !actL) return 0u;
+    if(R >= 0 && !actR && !actL) return 0u;
+
+    /* dt, :695-699 */
+    double dt = m->dt[L];
+    if(R >= 0) dt = smin(dt, m->dt[R]);
+
+    double WL[5], WR[5];
+    for(int k = 0; k < NC; k++) {
+        WL[k] = CELL(m->W, k, L, C);
+        WR[k] = (R >= 0) ? CELL(m->W, k, R, C) : 0.;
+    }
+    for(int d = 0; d < nd; d++) {
+        WL[1 + d] -= vface[d];
+        WR[1 + d] -= vface[d];
+    }
+
+    double r = 0.;
+    if(R >= 0) {
+        double cd[3];
+        for(int d = 0; d < nd; d++)
+            cd[d] = CELL(m->centroid, d, L, C) - CELL(m->centroid, d, R, C);
+        r = (nd == 3) ? sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2])
+                      : sqrt(cd[0] * cd[0] + cd[1] * cd[1]);
+    }
+    double WRrec[5] = {0., 0., 0., 0., 0.}, WLrec[5];
+    double d[3];
+    if(R >= 0) {
+        for(int n = nd; n--;) {
+            double correction = CELL(m->pos, n, R, C) - xr[n];
+            d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, R, C) +
+                   correction;
+        }
+        predict_side(s, m, R, WR, WL, d, r, 1, dt, WRrec);
+    }
+    for(int n = nd; n--;) {
+        double correction = CELL(m->pos, n, L, C) - xl[n];
+        d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, L, C) +
+               correction;
+    }
+    predict_side(s, m, L, WL, WR, d, r, R >= 0, dt, WLrec);
+    if(WL[0] != 0.) memcpy(WL, WLrec, sizeof(double) * NC);
+    if(WR[0] != 0.) memcpy(WR, WRrec, sizeof(double) * NC);
+
+    /* get_normal, :553-571 */
+    double nrm[3], dd[4];
+    for(int k = 0; k < nd; k++) dd[k] = xl[k] - xr[k];
+    dd[3] = (nd == 3) ? sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2])
+                      : sqrt(dd[0] * dd[0] + dd[1] * dd[1]);
+    for(int k = 0; k < nd; k++) nrm[k] = -dd[k] / dd[3];
+
+    double flux[5];
+    st |= sfxo_solve_for_flux(s, WL, WR, nrm, vface, R < 0, flux, c);
+    double dQL[5], dQR[5];
+    for(int k = 0; k < NC; k++) {
+        dQL[k] = dt * m->area[f] * flux[k];
+        dQR[k] = -dt * m->area[f] * flux[k];
+    }
+    /* mass flux limiter, :840-866 */
+    double timefac = 1.;
+    if(R >= 0) {
+        double dtmax = m->dt[L];
+        dtmax = smax(dtmax, m->dt[R]);
+        timefac = dt / dtmax;
+    }
+    double areafac = m->area[f] / m->total_area[L];
+    areafac *= timefac;
+    if(dQL[0] > areafac * m->mass[L]) {
+        double corfac = fabs(areafac * m->mass[L] / dQL[0]);
+        for(int k = 0; k < NC; k++) dQL[k] *= corfac;
+        for(int k = 0; k < NC; k++) dQR[k] = -1. * dQL[k];
+    }
+    if(R >= 0) {
+        double af = m->area[f] / m->total_area[R];
+        af *= timefac;
+        if(dQR[0] > af * m->mass[R]) {
+            double corfac = fabs(af * m->mass[R] / dQR[0]);
+            for(int k = 0; k < NC; k++) dQR[k] *= corfac;
+            for(int k = 0; k < NC; k++) dQL[k] = -1. * dQR[k];
+        }
+    }
+    if(flux[0] != flux[0]) st |= SFXO_ST_NANFLUX;
+    for(int k = 0; k < NC; k++) dQLo[k] = dQL[k];
+    return st;
+}
+
+/* The scatter of one face (src/VorFace.cpp:913-955, single process): the
+ * left cell always takes dQL; the right cell -dQL when it is a local real
+ * cell (not a periodic copy, not a mirror).  dQR is bitwise -dQL
+ * (:834-835, :855, :864). */
+static void scatter_one(const sfxo_mesh* m, int64_t f, const double* dQL,
+                        double* dQ) {
+    int NC = m->ndim + 2;
+    int64_t C = m->C;
+    int32_t L = m->left[f], R = m->right[f];
+    for(int k = 0; k < NC; k++) CELL(dQ, k, L, C) += dQL[k];
+    if(R >= 0 && !(m->ghost && m->ghost[f]))
+        for(int k = 0; k < NC; k++) CELL(dQ, k, R, C) += -dQL[k];
+}
+
 /* VorTessManager::hydro face loop (src/VorTessManager.cpp:205-210) calling
- * VorFace::set_v and VorFace::calculate_flux (src/VorFace.cpp:670-1024) for a
- * single process (every right generator is local, a periodic copy or a
- * mirror); no gravity. */
+ * VorFace::set_v and VorFace::calculate_flux for every face in order, each
+ * face's flux scattered as soon as it is computed. */
 unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
                              double* dQ, double* dQL_face, uint8_t* face_status,
                              sfxo_counters* c) {
-    int nd = m->ndim, NC = nd + 2;
-    int64_t C = m->C, F = m->F;
+    int NC = m->ndim + 2;
+    int64_t F = m->F;
     unsigned all = 0;
     for(int64_t f = 0; f < F; f++) {
-        unsigned st = 0;
-        int32_t L = m->left[f], R = m->right[f];
-        if(dQL_face)
-            for(int k = 0; k < NC; k++) dQL_face[(int64_t)k * F + f] = 0.;
-        if(face_status) face_status[f] = 0;
-        double xl[3], xr[3], vface[3];
-        for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, L, C);
-        right_gen_pos(m, f, xr);
-        face_velocity(m, f, xl, xr, vface);
-
-        /* gating, :672-683 */
-        if(!(m->area[f] != 0.)) continue;
-        int actL = m->active ? m->active[L] : 1;
-        int actR = (R >= 0) ? (m->active ? m->active[R] : 1) : 0;
-        if(R < 0 && !actL) continue;
-        if(R >= 0 && !actR && !actL) continue;
-
-        /* dt, :695-699 */
-        double dt = m->dt[L];
-        if(R >= 0) dt = smin(dt, m->dt[R]);
-
-        double WL[5], WR[5];
-        for(int k = 0; k < NC; k++) {
-            WL[k] = CELL(m->W, k, L, C);
-            WR[k] = (R >= 0) ? CELL(m->W, k, R, C) : 0.;
-        }
-        for(int d = 0; d < nd; d++) {
-            WL[1 + d] -= vface[d];
-            WR[1 + d] -= vface[d];
-        }
-
-        double r = 0.;
-        if(R >= 0) {
-            double cd[3];
-            for(int d = 0; d < nd; d++)
-                cd[d] = CELL(m->centroid, d, L, C) - CELL(m->centroid, d, R, C);
-            r = (nd == 3) ? sqrt(cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2])
-                          : sqrt(cd[0] * cd[0] + cd[1] * cd[1]);
-        }
-        double WRrec[5] = {0., 0., 0., 0., 0.}, WLrec[5];
-        double d[3];
-        if(R >= 0) {
-            for(int n = nd; n--;) {
-                double correction = CELL(m->pos, n, R, C) - xr[n];
-                d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, R, C) +
-                       correction;
-            }
-            predict_side(s, m, R, WR, WL, d, r, 1, dt, WRrec);
-        }
-        for(int n = nd; n--;) {
-            double correction = CELL(m->pos, n, L, C) - xl[n];
-            d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, L, C) +
-                   correction;
-        }
-        predict_side(s, m, L, WL, WR, d, r, R >= 0, dt, WLrec);
-        if(WL[0] != 0.) memcpy(WL, WLrec, sizeof(double) * NC);
-        if(WR[0] != 0.) memcpy(WR, WRrec, sizeof(double) * NC);
-
-        /* get_normal, :553-571 */
-        double nrm[3], dd[4];
-        for(int k = 0; k < nd; k++) dd[k] = xl[k] - xr[k];
-        dd[3] = (nd == 3) ? sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2])
-                          : sqrt(dd[0] * dd[0] + dd[1] * dd[1]);
-        for(int k = 0; k < nd; k++) nrm[k] = -dd[k] / dd[3];
-
-        double flux[5];
-        st |= sfxo_solve_for_flux(s, WL, WR, nrm, vface, R < 0, flux, c);
-        double dQL[5], dQR[5];
-        for(int k = 0; k < NC; k++) {
-            dQL[k] = dt * m->area[f] * flux[k];
-            dQR[k] = -dt * m->area[f] * flux[k];
-        }
-        /* mass flux limiter, :840-866 */
-        double timefac = 1.;
-        if(R >= 0) {
-            double dtmax = m->dt[L];
-            dtmax = smax(dtmax, m->dt[R]);
-            timefac = dt / dtmax;
-        }
-        double areafac = m->area[f] / m->total_area[L];
-        areafac *= timefac;
-        if(dQL[0] > areafac * m->mass[L]) {
-            double corfac = fabs(areafac * m->mass[L] / dQL[0]);
-            for(int k = 0; k < NC; k++) dQL[k] *= corfac;
-            for(int k = 0; k < NC; k++) dQR[k] = -1. * dQL[k];
-        }
-        if(R >= 0) {
-            double af = m->area[f] / m->total_area[R];
-            af *= timefac;
-            if(dQR[0] > af * m->mass[R]) {
-                double corfac = fabs(af * m->mass[R] / dQR[0]);
-                for(int k = 0; k < NC; k++) dQR[k] *= corfac;
-                for(int k = 0; k < NC; k++) dQL[k] = -1. * dQR[k];
-            }
-        }
-        /* scatter, :913-955 (single process: the left cell is always
-         * updated; the right cell only when it is a local real cell) */
-        for(int k = 0; k < NC; k++) CELL(dQ, k, L, C) += dQL[k];
-        if(R >= 0 && !(m->ghost && m->ghost[f]))
-            for(int k = 0; k < NC; k++) CELL(dQ, k, R, C) += dQR[k];
-        if(flux[0] != flux[0]) st |= SFXO_ST_NANFLUX;
+        double dQL[5];
+        unsigned st = face_flux_one(s, m, f, dQL, c);
+        scatter_one(m, f, dQL, dQ);
         if(dQL_face)
             for(int k = 0; k < NC; k++) dQL_face[(int64_t)k * F + f] = dQL[k];
         if(face_status) face_status[f] = (uint8_t)st;
         all |= st;
     }
     return all;
+}
+
+/* Faces f0..f1-1 without the scatter: dQL_face [(ndim+2)][F] and
+ * face_status [F] (may be NULL) of those faces.  For running the pass on
+ * several threads; sfxo_gather then applies the fluxes in face order. */
+unsigned sfxo_face_flux_range(const sfxo_solver* s, const sfxo_mesh* m,
+                              int64_t f0, int64_t f1, double* dQL_face,
+                              uint8_t* face_status, sfxo_counters* c) {
+    int NC = m->ndim + 2;
+    int64_t F = m->F;
+    unsigned all = 0;
+    for(int64_t f = f0; f < f1; f++) {
+        double dQL[5];
+        unsigned st = face_flux_one(s, m, f, dQL, c);
+        for(int k = 0; k < NC; k++) dQL_face[(int64_t)k * F + f] = dQL[k];
+        if(face_status) face_status[f] = (uint8_t)st;
+        all |= st;
+    }
+    return all;
+}
+
+/* dQ of cells c0..c1-1 from per-face fluxes: the cell's gather entries
+ * (2 face + side, ascending face order) added in order, -dQL for side 1 --
+ * the same additions, in the same order, as the scatter of the face loop
+ * (dQ zeroed first). */
+void sfxo_gather(const sfxo_mesh* m, const int64_t* gather_off,
+                 const int32_t* gather_entry, const double* dQL_face,
+                 int64_t c0, int64_t c1, double* dQ) {
+    int NC = m->ndim + 2;
+    int64_t C = m->C, F = m->F;
+    for(int64_t i = c0; i < c1; i++) {
+        double acc[5] = {0., 0., 0., 0., 0.};
+        for(int64_t j = gather_off[i]; j < gather_off[i + 1]; j++) {
+            int32_t e = gather_entry[j];
+            int64_t f = e >> 1;
+            for(int k = 0; k < NC; k++) {
+                double x = dQL_face[(int64_t)k * F + f];
+                acc[k] += (e & 1) ? -x : x;
+            }
+        }
+        for(int k = 0; k < NC; k++) CELL(dQ, k, i, C) = acc[k];
+    }
 }
 
 /* GasParticle::update_Q (src/utilities/GasParticle.cpp:617-624) for every

@@ -132,6 +132,9 @@ typedef struct {
  * grad layout.  Inactive cells are left untouched (src/Simulation.cpp:
  * 414-420 only re-estimates active cells). */
 void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m, double* grad_out);
+/* the same for cells c0..c1-1 */
+void sfxo_gradients_range(const sfxo_solver* s, const sfxo_mesh* m, int64_t c0,
+                          int64_t c1, double* grad_out);
 
 /* VorFace::set_v and get_normal per face: vface, normal [ndim][F] */
 void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal);
@@ -143,6 +146,18 @@ void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal);
 unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
                              double* dQ, double* dQL_face, uint8_t* face_status,
                              sfxo_counters* c);
+
+/* Faces f0..f1-1 without the scatter (dQL_face required, face_status may be
+ * NULL), and the ordered per-cell sums of such fluxes for cells c0..c1-1
+ * through the gather index (ascending face order; entries 2 face + side):
+ * together the same result as sfxo_face_flux_pass, bit for bit, computable
+ * in pieces on several threads. */
+unsigned sfxo_face_flux_range(const sfxo_solver* s, const sfxo_mesh* m,
+                              int64_t f0, int64_t f1, double* dQL_face,
+                              uint8_t* face_status, sfxo_counters* c);
+void sfxo_gather(const sfxo_mesh* m, const int64_t* gather_off,
+                 const int32_t* gather_entry, const double* dQL_face,
+                 int64_t c0, int64_t c1, double* dQ);
 
 /* GasParticle::update_Q for every cell (Q -= dQ, dQ = 0, status on m < 0 or
  * e < 0), then W = get_W(V, Q) for the cells whose step ends now,

@@ -2,9 +2,11 @@
 
 Counter-based SplitMix64: value(stream, i) depends only on (seed, stream, i),
 so any slice of a batch can be regenerated independently and the CPU and GPU
-legs see the same numbers.  Every generated value is rounded to float32 and
-stored as float64, so the FP64 and the mixed (FP32-storage) paths read
-identical inputs and differences measure arithmetic only.
+legs see the same numbers.  By default every generated value is rounded to
+float32 and stored as float64, so the FP64 and the mixed (FP32-storage)
+paths read identical inputs and differences measure arithmetic only;
+round32=False keeps the FP64 draws (then the mixed path's FP32 storage
+rounds them, and the difference includes that).
 
 Problem families (ndim = 3 unless stated):
   uniform   rho, p = 10**U(-3, 3) per side; velocity components U(-2, 2);
@@ -47,10 +49,10 @@ def f32(x):
     return np.asarray(x, np.float64).astype(np.float32).astype(np.float64)
 
 
-def riemann_batch(N, seed=1, family="uniform", ndim=3, start=0):
-    """Returns dict with WL, WR (ndim+2, N), n (ndim, N), all float64 holding
-    float32-representable values, plus 'category' (0 uniform, 1 vacuum,
-    2 shock)."""
+def riemann_batch(N, seed=1, family="uniform", ndim=3, start=0, round32=True):
+    """Returns dict with WL, WR (ndim+2, N), n (ndim, N), float64 holding
+    float32-representable values (round32=False: the draws unrounded), plus
+    'category' (0 uniform, 1 vacuum, 2 shock)."""
     idx = np.arange(start, start + N, dtype=np.uint64)
     NC = ndim + 2
     WL = np.empty((NC, N))
@@ -95,6 +97,8 @@ def riemann_batch(N, seed=1, family="uniform", ndim=3, start=0):
     up = uniform(seed, S_SHOCK, idx) < 0.5
     WR[NC - 1, m & up] = WL[NC - 1, m & up] * 1e6
     WR[NC - 1, m & ~up] = WL[NC - 1, m & ~up] * 1e-6
+    if not round32:
+        return {"WL": WL, "WR": WR, "n": n, "category": cat}
     return {"WL": f32(WL), "WR": f32(WR), "n": f32(n), "category": cat}
 
 

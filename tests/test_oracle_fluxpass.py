@@ -236,3 +236,51 @@ def test_linear_field_gradient_is_exact_3d():
         assert exact.mean() > 0.95
         a = gi[~exact] / val
         assert np.all((a >= 0.0) & (a <= 1.0 + 1e-9))
+
+
+@pytest.mark.parametrize("kind", [O.EXACT, O.HLLC])
+def test_threaded_pass_equals_sequential(kind):
+    """The CPU baseline's threaded pass (contiguous cell and face ranges, then
+    the ordered gather) gives the sequential face loop's results bit for
+    bit, with and without an activity mask."""
+    m = M.voronoi_periodic(M.jittered_grid(32, 0.25, seed=6))
+    b = M.prepare(m, M.kh_ics(m), G, kind=kind)
+    for act in (None, (np.arange(m["C"]) % 3 != 0).astype(np.uint8)):
+        bb = dict(b)
+        if act is not None:
+            bb["active"] = act
+        g = O.gradients(kind, bb)
+        bb["grad"] = g
+        dQ, dQL, st, _ = O.face_flux_pass(kind, bb)
+        g2, dQ2, dQL2, st2 = O.threaded_pass(kind, bb, 5)
+        assert np.array_equal(g2[:, :, act != 0] if act is not None else g2,
+                              g[:, :, act != 0] if act is not None else g)
+        assert np.array_equal(dQL2, dQL) and np.array_equal(dQ2, dQ)
+        assert np.array_equal(st2, st)
+
+
+def test_flop_counter_counts_operations():
+    """The counting build (oracle/libflops.so) computes what the oracle
+    computes, bit for bit, and counts the operations of a hand-counted
+    function: get_Q takes 1 + ndim products for mass and momenta, the squared
+    speed (3 operations in 2D; in 3D the reference forms the 2D sum and then
+    the 3D one, 3 + 5), and 6 for the energy (plus the solver's constants,
+    counted once per call)."""
+    rng = np.random.default_rng(2)
+    for nd, per in ((2, 12), (3, 18)):
+        for kind in (O.EXACT, O.HLLC):
+            W = rng.uniform(0.5, 2.0, (nd + 2, 50))
+            V = rng.uniform(0.1, 1.0, 50)
+            _, init, _ = O.count_flops(O.params, kind, nd)
+            Q, n, t = O.count_flops(O.get_Q, kind, nd, V, W)
+            assert np.array_equal(Q, O.get_Q(kind, nd, V, W))
+            assert init < 30
+            assert n == per * 50 + init and t == 0, (nd, kind, n, init)
+    m, _ = golden_mesh("p3d")
+    b = bundle(m, 0)
+    b["grad"] = O.gradients(0, b)
+    r0 = O.face_flux_pass(0, b)
+    r1, n, t = O.count_flops(O.face_flux_pass, 0, b)
+    assert np.array_equal(r0[1], r1[1]) and np.array_equal(r0[0], r1[0])
+    # the exact solver takes 2 pow per pressure function evaluation
+    assert n > 500 * b["F"] and t > 0
