@@ -272,6 +272,23 @@ int sfx_face_geometry(int ndim, int64_t C, int64_t F, const double* pos,
                       const double* shift, double* dL, double* dR, double* rRL,
                       double* mc, double* rcent);
 
+/* Closure residual of every cell (host, FP64): closure[d*C + i] = the sum,
+ * over cell i's faces with nonzero area in reverse walk order, of
+ * area * r_ij[d] / |r_ij|, r_ij = position of i - position of its neighbour
+ * (= -rRL for the face's left cell, rRL for its right cell).  The gradient
+ * sum of VorCell::estimate_gradient (src/VorCell.cpp:471-500),
+ *   sum_j A_ij [(W_j - W_i) c_ij / |r_ij| - (W_i + W_j)/2 r_ij / |r_ij|],
+ * equals the difference form sum_j (W_j - W_i) A_ij (c_ij - r_ij/2)/|r_ij|
+ * minus W_i * closure_i; the mixed pass sums the difference form (no
+ * cancellation of the W_i terms in FP32 weights) and subtracts W_i *
+ * closure_i when sfx_mesh.closure is given.  The faces of a Voronoi cell
+ * close (sum of A n = 0), so closure is round-off there; a face list that
+ * does not close (faces moved or turned into mirrors by hand) needs it.
+ * cf_entry: sfx_mesh_cell_entries; rRL: sfx_face_geometry [ndim][F]. */
+int sfx_cell_closure(int ndim, int64_t C, int64_t F, const int64_t* cf_off,
+                     const int32_t* cf_entry, const double* area, const double* rRL,
+                     double* closure);
+
 /* Per-cell face entries in walk order: cf_entry[j] = 2*face + side, side 1
  * when the cell is the face's right (used by the gradient kernel). */
 int sfx_mesh_cell_entries(int64_t C, int64_t F, const int32_t* left,
@@ -305,7 +322,8 @@ typedef struct {
     /* cells */
     const double* W;           /* [C][ndim+2] primitives (FP64 in both passes);
                                   16-byte aligned, as egeo */
-    const void* vel;           /* T [C][ndim] generator velocities */
+    const double* vel;         /* [C][ndim] generator velocities (FP64 in
+                                  both passes) */
     const double* mass;        /* [C] Q.m */
     const double* total_area;  /* [C] sum of the cell's face areas */
     const double* dt;          /* [C] real time of the cell's time step */
@@ -340,6 +358,11 @@ typedef struct {
     int64_t egeo_rows;         /* that rows value, or 0 if not given (with it,
                                   and records under 4 GB, a lane past its
                                   cell's entries reads no padding) */
+    const void* closure;       /* T [ndim][C] or NULL (mixed pass only):
+                                  every cell's closure residual
+                                  (sfx_cell_closure); NULL when the cells'
+                                  faces close, as those of a Voronoi
+                                  tessellation do */
 } sfx_mesh;
 
 /* Per-entry face records for the gradient pass, built once per
@@ -405,11 +428,14 @@ int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad,
 
 /* GasParticle::update_Q then get_W (src/utilities/GasParticle.cpp:617-624,
  * src/riemann/ExactRiemannSolver.cpp:716-766): Q -= dQ, dQ = 0,
- * W = get_W(volume, Q) (Q, dQ, W [C][ndim+2]);
- * cell_status bit SFX_ST_UPDATEQ where m or e went negative. */
+ * W = get_W(volume, Q) (Q, dQ, W [C][ndim+2]); mass [C] (may be NULL) is
+ * set to the new Q.m, the sfx_mesh.mass the next pass's mass-flux limiter
+ * reads (VorFace::calculate_flux takes the cell's current mass,
+ * src/VorFace.cpp:850-851); cell_status bit SFX_ST_UPDATEQ where m or e went
+ * negative. */
 int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ,
-                    const double* volume, double* W, uint8_t* cell_status,
-                    void* stream);
+                    const double* volume, double* W, double* mass,
+                    uint8_t* cell_status, void* stream);
 
 /* The same with individual time steps (src/Simulation.cpp:476-483,792-796): Q -= dQ
  * and dQ = 0 for every cell (GasParticle::update_Q, which also raises
@@ -420,8 +446,9 @@ int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ,
  * endtime: device uint64 [C] (sfx_hydro_timestep's), or NULL for every
  * cell (= sfx_update_step). */
 int sfx_update_step_timed(sfx_solver* h, int64_t C, double* Q, double* dQ,
-                          const double* volume, double* W, const uint64_t* endtime,
-                          uint64_t current_time, uint8_t* cell_status, void* stream);
+                          const double* volume, double* W, double* mass,
+                          const uint64_t* endtime, uint64_t current_time,
+                          uint8_t* cell_status, void* stream);
 
 
 /* ---- serving: an ensemble of meshes resident in HBM ------------------ */
@@ -453,8 +480,7 @@ typedef struct {
 
 /* Capacities are totals over all meshes (entries: cf_off[C] summed, the
  * same bound is used for the gather index).  The handle's precision selects
- * the storage (SFX_PREC_MIXED: FP32 velocities, gradients and face
- * geometry). */
+ * the storage (SFX_PREC_MIXED: FP32 gradients and face geometry). */
 int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells,
                         int64_t max_faces, int64_t max_entries,
                         int64_t arena_cells, int64_t arena_faces,
@@ -462,7 +488,8 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells,
 int sfx_ensemble_destroy(sfx_ensemble* e);
 
 /* Copies one mesh (device arrays with local indices, the sfx_mesh layout of
- * the handle's precision; egeo, if set, is ignored) and its conserved
+ * the handle's precision; egeo, if set, is ignored; closure, if set, is
+ * kept by a mixed ensemble) and its conserved
  * variables Q ([C][ndim+2]) into the ensemble.  E = cf_off[C],
  * G = gather_off[C].  Synchronous.  SFX_ERR_NOMEM when the ensemble is
  * full. */

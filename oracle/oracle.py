@@ -54,7 +54,8 @@ class MeshStruct(ct.Structure):
                 ("volume", _dp), ("active", _u8p), ("left", _i32p),
                 ("right", _i32p), ("area", _dp), ("midpoint", _dp),
                 ("shift", _dp), ("ghost", _u8p), ("cf_off", _i64p),
-                ("cf_face", _i32p)]
+                ("cf_face", _i32p), ("geo_dL", _dp), ("geo_dR", _dp),
+                ("geo_rRL", _dp), ("geo_mc", _dp), ("geo_rcent", _dp)]
 
 
 class Timeline(ct.Structure):
@@ -288,6 +289,10 @@ def mesh_struct(mesh):
     m.F = int(mesh["F"])
     for name in ("W", "grad", "centroid", "pos", "vel", "mass", "total_area",
                  "dt", "volume", "area", "midpoint", "shift"):
+        a = arr(name)
+        setattr(m, name, _p(a))
+    # the stored per-face geometry, when the bundle carries it under geo_*
+    for name in ("geo_dL", "geo_dR", "geo_rRL", "geo_mc", "geo_rcent"):
         a = arr(name)
         setattr(m, name, _p(a))
     for name, dt, t in (("active", np.uint8, _u8p), ("ghost", np.uint8, _u8p),

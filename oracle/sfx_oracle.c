@@ -1039,11 +1039,13 @@ void sfxo_gradients(const sfxo_solver* s, const sfxo_mesh* m,
     sfxo_gradients_range(s, m, 0, m->C, grad_out);
 }
 
-/* VorFace::set_v, src/VorFace.cpp:586-660 */
+/* VorFace::set_v, src/VorFace.cpp:586-660; rRL = xr - xl and mc = midpoint -
+ * (xl + xr)/2 of the face (face_vectors) */
 static void face_velocity(const sfxo_mesh* m, int64_t f, const double* xl,
-                          const double* xr, double* vface) {
+                          const double* xr, const double* rRL,
+                          const double* mc, double* vface) {
     int nd = m->ndim;
-    int64_t C = m->C, F = m->F;
+    int64_t C = m->C;
     int32_t L = m->left[f], R = m->right[f];
     double vl[3], vr[3];
     for(int d = 0; d < nd; d++) vl[d] = CELL(m->vel, d, L, C);
@@ -1059,29 +1061,47 @@ static void face_velocity(const sfxo_mesh* m, int64_t f, const double* xl,
         vr[0] = Wd[1];
         vr[1] = Wd[2];
     }
-    double rRL[3], rRL2;
-    for(int d = 0; d < nd; d++) rRL[d] = xr[d] - xl[d];
-    rRL2 = rRL[0] * rRL[0] + rRL[1] * rRL[1];
+    double rRL2 = rRL[0] * rRL[0] + rRL[1] * rRL[1];
     if(nd == 3) rRL2 = rRL[0] * rRL[0] + rRL[1] * rRL[1] + rRL[2] * rRL[2];
-    double fac = (vl[0] - vr[0]) *
-                 (m->midpoint[0 * F + f] - 0.5 * (xl[0] + xr[0]));
-    fac += (vl[1] - vr[1]) * (m->midpoint[1 * F + f] - 0.5 * (xl[1] + xr[1]));
-    if(nd == 3)
-        fac += (vl[2] - vr[2]) *
-               (m->midpoint[2 * F + f] - 0.5 * (xl[2] + xr[2]));
+    double fac = (vl[0] - vr[0]) * mc[0];
+    fac += (vl[1] - vr[1]) * mc[1];
+    if(nd == 3) fac += (vl[2] - vr[2]) * mc[2];
     fac /= rRL2;
     for(int d = 0; d < nd; d++) vface[d] = 0.5 * (vl[d] + vr[d]) + fac * rRL[d];
+}
+
+/* The generator positions of face f's two sides, rRL = xr - xl and mc =
+ * midpoint - (xl + xr)/2 (src/VorFace.cpp:586-660); with the stored
+ * geometry, xl = 0 and xr = rRL stand for the positions (only their
+ * difference is used). */
+static void face_vectors(const sfxo_mesh* m, int64_t f, double* xl, double* xr,
+                         double* rRL, double* mc) {
+    int nd = m->ndim;
+    int64_t C = m->C, F = m->F;
+    if(m->geo_rRL) {
+        for(int d = 0; d < nd; d++) {
+            xl[d] = 0.;
+            xr[d] = rRL[d] = m->geo_rRL[d * F + f];
+            mc[d] = m->geo_mc[d * F + f];
+        }
+        return;
+    }
+    for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, m->left[f], C);
+    right_gen_pos(m, f, xr);
+    for(int d = 0; d < nd; d++) {
+        rRL[d] = xr[d] - xl[d];
+        mc[d] = m->midpoint[d * F + f] - 0.5 * (xl[d] + xr[d]);
+    }
 }
 
 /* VorFace::set_v and VorFace::get_normal for every face (testing aid) */
 void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal) {
     int nd = m->ndim;
-    int64_t C = m->C, F = m->F;
+    int64_t F = m->F;
     for(int64_t f = 0; f < F; f++) {
-        double xl[3], xr[3], v[3], dd[4];
-        for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, m->left[f], C);
-        right_gen_pos(m, f, xr);
-        face_velocity(m, f, xl, xr, v);
+        double xl[3], xr[3], rRL[3], mc[3], v[3], dd[4];
+        face_vectors(m, f, xl, xr, rRL, mc);
+        face_velocity(m, f, xl, xr, rRL, mc, v);
         for(int k = 0; k < nd; k++) dd[k] = xl[k] - xr[k];
         dd[3] = (nd == 3) ? sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2])
                           : sqrt(dd[0] * dd[0] + dd[1] * dd[1]);
@@ -1139,10 +1159,9 @@ static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
     unsigned st = 0;
     int32_t L = m->left[f], R = m->right[f];
     for(int k = 0; k < NC; k++) dQLo[k] = 0.;
-    double xl[3], xr[3], vface[3];
-    for(int d = 0; d < nd; d++) xl[d] = CELL(m->pos, d, L, C);
-    right_gen_pos(m, f, xr);
-    face_velocity(m, f, xl, xr, vface);
+    double xl[3], xr[3], rRL[3], mc[3], vface[3];
+    face_vectors(m, f, xl, xr, rRL, mc);
+    face_velocity(m, f, xl, xr, rRL, mc, vface);
 
     /* gating, :672-683 */
     if(!(m->area[f] != 0.)) return 0u;
@@ -1166,7 +1185,9 @@ static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
     }
 
     double r = 0.;
-    if(R >= 0) {
+    if(R >= 0 && m->geo_rRL) {
+        r = m->geo_rcent[f];
+    } else if(R >= 0) {
         double cd[3];
         for(int d = 0; d < nd; d++)
             cd[d] = CELL(m->centroid, d, L, C) - CELL(m->centroid, d, R, C);
@@ -1177,6 +1198,10 @@ static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
     double d[3];
     if(R >= 0) {
         for(int n = nd; n--;) {
+            if(m->geo_rRL) {
+                d[n] = m->geo_dR[n * F + f];
+                continue;
+            }
             double correction = CELL(m->pos, n, R, C) - xr[n];
             d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, R, C) +
                    correction;
@@ -1184,6 +1209,10 @@ static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
         predict_side(s, m, R, WR, WL, d, r, 1, dt, WRrec);
     }
     for(int n = nd; n--;) {
+        if(m->geo_rRL) {
+            d[n] = m->geo_dL[n * F + f];
+            continue;
+        }
         double correction = CELL(m->pos, n, L, C) - xl[n];
         d[n] = m->midpoint[n * F + f] - CELL(m->centroid, n, L, C) +
                correction;

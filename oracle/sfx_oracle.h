@@ -126,6 +126,14 @@ typedef struct {
     const uint8_t* ghost;
     const int64_t* cf_off;
     const int32_t* cf_face;
+    /* optional per-face geometry, [ndim][F] (geo_rcent [F]): when geo_rRL is
+     * set the face pass takes rRL = xRgen - xL, mc = midpoint - (xL +
+     * xRgen)/2, the two sides' d vectors and the centroid distance from these
+     * arrays instead of forming them from pos, shift, midpoint and centroid
+     * (the same expressions: given the host's FP64 values the results are
+     * bit-identical; given values rounded as a pass stores them, the oracle
+     * computes exactly on that pass's stored inputs) */
+    const double *geo_dL, *geo_dR, *geo_rRL, *geo_mc, *geo_rcent;
 } sfxo_mesh;
 
 /* VorCell::estimate_gradient for every active cell; grad_out has the mesh

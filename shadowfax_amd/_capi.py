@@ -37,7 +37,8 @@ class MeshDesc(ct.Structure):
                 ("left", vp), ("right", vp), ("ghost", vp), ("twin", vp),
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
                 ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
-                ("gather_entry", vp), ("egeo_group", vp), ("egeo_rows", i64)]
+                ("gather_entry", vp), ("egeo_group", vp), ("egeo_rows", i64),
+                ("closure", vp)]
 
 
 _lib = None
@@ -116,8 +117,8 @@ def _mesh_sigs():
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],
         "sfx_face_flux_pass": [vp, md, vp, vp, vp, vp, vp],
-        "sfx_update_step": [vp, i64, vp, vp, vp, vp, vp, vp],
-        "sfx_update_step_timed": [vp, i64, vp, vp, vp, vp, vp, ct.c_uint64, vp, vp],
+        "sfx_update_step": [vp, i64, vp, vp, vp, vp, vp, vp, vp],
+        "sfx_update_step_timed": [vp, i64, vp, vp, vp, vp, vp, vp, ct.c_uint64, vp, vp],
     }
 
 

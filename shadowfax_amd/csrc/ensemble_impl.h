@@ -165,11 +165,12 @@ __global__ __launch_bounds__(256) void drain_update(Consts cd, double* __restric
                                                     const double* __restrict__ volume,
                                                     double* __restrict__ W,
                                                     uint8_t* __restrict__ cstatus,
+                                                    double* __restrict__ mass,
                                                     unsigned long long* __restrict__ ctr,
                                                     Plan* __restrict__ p, int grid) {
     unsigned fail = 0;
     drain<false>(p, 3, grid, [&](const Slot&, int64_t i) {
-        fail += flux::update_cell<ND>(cd, i, Q, dQ, volume, W, cstatus);
+        fail += flux::update_cell<ND>(cd, i, Q, dQ, volume, W, cstatus, mass);
     });
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
     block_add_counters(ctr, cv);
@@ -220,11 +221,12 @@ __global__ __launch_bounds__(256) void range_update(Consts cd, double* __restric
                                                     const double* __restrict__ volume,
                                                     double* __restrict__ W,
                                                     uint8_t* __restrict__ cstatus,
+                                                    double* __restrict__ mass,
                                                     unsigned long long* __restrict__ ctr,
                                                     int64_t cell0, int64_t ncell) {
     const int64_t k = xcd_block() * blockDim.x + threadIdx.x;
     unsigned fail = 0;
-    if(k < ncell) fail = flux::update_cell<ND>(cd, cell0 + k, Q, dQ, volume, W, cstatus);
+    if(k < ncell) fail = flux::update_cell<ND>(cd, cell0 + k, Q, dQ, volume, W, cstatus, mass);
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
     block_add_counters(ctr, cv);
 }
@@ -241,6 +243,7 @@ struct EnsArgs {
     double* dQ;    // [C][NC]
     double* Q;     // [C][NC]
     double* W;     // [C][NC] (the same array as m.W)
+    double* mass;  // [C] (the same array as m.mass): the update writes Q.m
     uint8_t* fstatus;
     uint8_t* cstatus;
     Consts c;

@@ -32,7 +32,7 @@ hipError_t ens_step_impl(int ndim, int kind, const EnsArgs<T>& a, hipStream_t s)
         hipLaunchKernelGGL((ens::drain_gather<ND, MIXED, T>), gc, b, 0, s, a.m.gather_off,         \
                            a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, a.plan, grid);   \
         hipLaunchKernelGGL((ens::drain_update<ND>), gc, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,    \
-                           a.W, a.cstatus, a.ctr, a.plan, grid);                                \
+                           a.W, a.cstatus, a.mass, a.ctr, a.plan, grid);                        \
     } while(0)
     if(ndim == 3)
         SFX_ENS(3);
@@ -68,7 +68,7 @@ hipError_t ens_range_impl(int ndim, int kind, const EnsArgs<T>& a, int64_t cell0
                            a.m.gather_entry, a.m.ghost, a.m.twin, a.dQL, a.dQ, cell0, ncell,    \
                            qshift);                                                             \
         hipLaunchKernelGGL((ens::range_update<ND>), gc, b, 0, s, a.c, a.Q, a.dQ, a.m.volume,    \
-                           a.W, a.cstatus, a.ctr, cell0, ncell);                                \
+                           a.W, a.cstatus, a.mass, a.ctr, cell0, ncell);                        \
     } while(0)
     if(ndim == 3)
         SFX_RNG(3);

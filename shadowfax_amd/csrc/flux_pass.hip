@@ -79,15 +79,15 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
 
 hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
                          const double* volume, double* W, const uint64_t* endtime,
-                         uint64_t tnow, uint8_t* cstatus, unsigned long long* ctr,
-                         hipStream_t s) {
+                         uint64_t tnow, uint8_t* cstatus, double* mass,
+                         unsigned long long* ctr, hipStream_t s) {
     if(C <= 0) return hipSuccess;
     if(ndim == 3)
         hipLaunchKernelGGL((flux::update_kernel<3>), grid_for(C, 256), dim3(256), 0, s, c, C, Q,
-                           dQ, volume, W, endtime, tnow, cstatus, ctr);
+                           dQ, volume, W, endtime, tnow, cstatus, mass, ctr);
     else
         hipLaunchKernelGGL((flux::update_kernel<2>), grid_for(C, 256), dim3(256), 0, s, c, C, Q,
-                           dQ, volume, W, endtime, tnow, cstatus, ctr);
+                           dQ, volume, W, endtime, tnow, cstatus, mass, ctr);
     return hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ template <typename T>
 struct MeshView {
     int64_t C, F;
     const double* W;      // [C][NC], FP64 in both passes
-    const T* vel;         // [C][ND]
+    const double* vel;    // [C][ND], FP64 in both passes
     const double* mass;   // [C]
     const double* total_area;
     const double* dt;
@@ -101,6 +101,9 @@ struct MeshView {
     // bytes of the grouped records, 0 if not known: when known and under
     // 4 GB they are read through a buffer descriptor with that bound
     uint64_t egeo_bytes = 0;
+    // [ND][C] every cell's closure residual sum A r_ij/|r_ij| (mixed
+    // gradients), or null where the cells' faces close
+    const T* closure = nullptr;
 };
 
 // Per-entry record of a cell's face, in the cell's walk order.  FP64
@@ -710,6 +713,18 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         } else if(pb > KB) {
             load_chunk(pb - KB, lo, hi);
         }
+    }
+    // the reference sum (src/VorCell.cpp:493-494) is the difference form
+    // minus W_i times the cell's closure residual, which vanishes when the
+    // cell's faces close; a face list that does not close brings it
+    if(m.closure) {
+        double S[ND];
+#pragma unroll
+        for(int l = 0; l < ND; l++) S[l] = (double)m.closure[l * m.C + i];
+#pragma unroll
+        for(int l = 0; l < ND; l++)
+#pragma unroll
+            for(int k = 0; k < NC; k++) delta[l][k] -= W[k] * S[l];
     }
 #pragma unroll
     for(int l = 0; l < ND; l++)
@@ -1374,13 +1389,16 @@ __device__ __forceinline__ void get_Q_state(double g, double V, const double* w,
 // only where `rebuild` (the cells whose time step ends now, endtime ==
 // currentTime, src/Simulation.cpp:792-796: only they have received the
 // fluxes of all their faces); the others keep their W, and their Q is not
-// touched by get_W's pressure floor.
+// touched by get_W's pressure floor.  mass (may be null) takes the new Q.m:
+// the next pass's mass-flux limiter reads the cell's current mass
+// (src/VorFace.cpp:850-851).
 template <int ND>
 __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
                                                 double* __restrict__ Q, double* __restrict__ dQ,
                                                 const double* __restrict__ volume,
                                                 double* __restrict__ Wout,
                                                 uint8_t* __restrict__ cstatus,
+                                                double* __restrict__ mass,
                                                 bool rebuild = true) {
     const int NC = ND + 2, P = ND + 1;
     unsigned fail = 0;
@@ -1395,6 +1413,7 @@ __device__ __forceinline__ unsigned update_cell(const Consts& cd, int64_t i,
         if(rebuild) get_W_state<ND>(cd.gamma, volume[i], q, w);
 #pragma unroll
         for(int k = 0; k < NC; k++) Q[i * NC + k] = q[k];
+        if(mass) mass[i] = q[0];
         if(rebuild) {
 #pragma unroll
             for(int k = 0; k < NC; k++) Wout[i * NC + k] = w[k];
@@ -1414,12 +1433,13 @@ __global__ __launch_bounds__(256) void update_kernel(Consts cd, int64_t C, doubl
                                                      const uint64_t* __restrict__ endtime,
                                                      uint64_t tnow,
                                                      uint8_t* __restrict__ cstatus,
+                                                     double* __restrict__ mass,
                                                      unsigned long long* __restrict__ ctr) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned fail = 0;
     if(i < C) {
         const bool rebuild = endtime ? endtime[i] == tnow : true;
-        fail = update_cell<ND>(cd, i, Q, dQ, volume, Wout, cstatus, rebuild);
+        fail = update_cell<ND>(cd, i, Q, dQ, volume, Wout, cstatus, mass, rebuild);
     }
     const unsigned cv[CNT_N] = {0u, 0u, 0u, 0u, fail};
     block_add_counters(ctr, cv);

@@ -40,6 +40,9 @@ struct sfx_ensemble {
     uint64_t rec_bytes = 0;  // the grouped records built in it
     void *area = nullptr, *dL = nullptr, *dR = nullptr, *rRL = nullptr, *mc = nullptr,
          *rcent = nullptr, *egeo = nullptr;
+    // mixed: the cells' closure residuals [nd][cap_alloc], allocated (zeroed)
+    // when the first mesh that brings one is added
+    void* closure = nullptr;
     // mesh table
     std::vector<int64_t> c0, nc, f0, nf;
     int64_t *d_c0 = nullptr, *d_nc = nullptr, *d_f0 = nullptr, *d_nf = nullptr;
@@ -90,7 +93,7 @@ flux::MeshView<T> ens_view(const sfx_ensemble* e) {
     v.C = e->cap_alloc;
     v.F = e->cap_faces;
     v.W = e->W;
-    v.vel = (const T*)e->vel;
+    v.vel = (const double*)e->vel;
     v.mass = e->mass;
     v.total_area = e->total_area;
     v.dt = e->dt;
@@ -113,6 +116,7 @@ flux::MeshView<T> ens_view(const sfx_ensemble* e) {
     v.egeo_bytes = e->rec_bytes;
     v.gather_off = e->gather_off;
     v.gather_entry = e->gather_entry;
+    v.closure = (const T*)e->closure;
     return v;
 }
 
@@ -125,6 +129,7 @@ EnsArgs<T> ens_args(const sfx_ensemble* e) {
     a.dQ = e->dQ;
     a.Q = e->Q;
     a.W = e->W;
+    a.mass = e->mass;
     a.fstatus = e->fstatus;
     a.cstatus = e->cstatus;
     a.c = e->h->c;
@@ -352,7 +357,7 @@ int sfx_ensemble_create(sfx_solver* h, int64_t max_meshes, int64_t max_cells, in
     A(&e->total_area, 8 * C);
     A(&e->dt, 8 * C);
     A(&e->volume, 8 * C);
-    A(&e->vel, t * C * nd);
+    A(&e->vel, 8 * C * nd);
     A(&e->active, C);
     A(&e->cstatus, C);
     A(&e->cf_off, 8 * (C + 1));
@@ -448,7 +453,7 @@ int sfx_ensemble_add_mesh(sfx_ensemble* e, const sfx_mesh* m, const double* Q, i
     };
     cp(e->W + c0 * NC, m->W, 8 * C * NC);
     cp(e->Q + c0 * NC, Q, 8 * C * NC);
-    cp((char*)e->vel + c0 * nd * t, m->vel, t * C * nd);
+    cp((char*)e->vel + c0 * nd * 8, m->vel, 8 * C * nd);
     cp(e->mass + c0, m->mass, 8 * C);
     cp(e->total_area + c0, m->total_area, 8 * C);
     cp(e->dt + c0, m->dt, 8 * C);
@@ -457,6 +462,16 @@ int sfx_ensemble_add_mesh(sfx_ensemble* e, const sfx_mesh* m, const double* Q, i
         cp(e->active + c0, m->active, C);
     else if(err == hipSuccess)
         err = hipMemsetAsync(e->active + c0, 1, C, s);
+    if(m->closure && e->prec == SFX_PREC_MIXED) {
+        if(!e->closure) {
+            if(err == hipSuccess) err = dalloc(e, &e->closure, t * e->cap_alloc * nd);
+            if(err == hipSuccess) err = hipMemsetAsync(e->closure, 0, t * e->cap_alloc * nd, s);
+            drop_graph(e);  // the captured drains were built without it
+        }
+        for(int l = 0; l < nd; l++)
+            cp((char*)e->closure + ((size_t)l * e->cap_alloc + c0) * t,
+               (const char*)m->closure + (size_t)l * C * t, t * C);
+    }
     cp(e->ghost + f0, m->ghost, F);
     cp((char*)e->area + f0 * t, m->area, t * F);
     cp((char*)e->rcent + f0 * t, m->rcent, t * F);

@@ -69,11 +69,12 @@ template <typename T>
 hipError_t launch_face_fluxes(int ndim, int kind, const flux::MeshView<T>& m, const T* grad,
                               const Consts& c, T* dQL, uint8_t* fstatus,
                               unsigned long long* ctr, hipStream_t s);
-// endtime null: every cell's W is rebuilt; else those with endtime == tnow
+// endtime null: every cell's W is rebuilt; else those with endtime == tnow;
+// mass (may be null) takes the new Q.m
 hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double* dQ,
                          const double* volume, double* W, const uint64_t* endtime,
-                         uint64_t tnow, uint8_t* cstatus, unsigned long long* ctr,
-                         hipStream_t s);
+                         uint64_t tnow, uint8_t* cstatus, double* mass,
+                         unsigned long long* ctr, hipStream_t s);
 hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* goff,
                               const int32_t* gent, const double* dQL, double* dQ,
                               hipStream_t s);

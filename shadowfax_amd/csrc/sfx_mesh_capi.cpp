@@ -37,7 +37,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.C = m->C;
     v.F = m->F;
     v.W = m->W;
-    v.vel = (const T*)m->vel;
+    v.vel = (const double*)m->vel;
     v.mass = m->mass;
     v.total_area = m->total_area;
     v.dt = m->dt;
@@ -62,6 +62,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
                        sizeof(T);
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
+    v.closure = (const T*)m->closure;
     return v;
 }
 
@@ -185,6 +186,35 @@ int sfx_face_geometry(int ndim, int64_t C, int64_t F, const double* pos, const d
         } else {
             rcent[f] = 0.0;
         }
+    }
+    return SFX_OK;
+}
+
+int sfx_cell_closure(int ndim, int64_t C, int64_t F, const int64_t* cf_off,
+                     const int32_t* cf_entry, const double* area, const double* rRL,
+                     double* closure) {
+    if(ndim != 2 && ndim != 3) return set_error(SFX_ERR_ARG, "sfx_cell_closure: ndim");
+    if(C < 0 || F < 0) return set_error(SFX_ERR_ARG, "sfx_cell_closure: negative size");
+    if(!cf_off || !cf_entry || !area || !rRL || !closure)
+        return set_error(SFX_ERR_ARG, "sfx_cell_closure: NULL array");
+    for(int64_t i = 0; i < C; i++) {
+        double S[3] = {0.0, 0.0, 0.0};
+        if(cf_off[i] > cf_off[i + 1])
+            return set_error(SFX_ERR_ARG, "sfx_cell_closure: cf_off not ascending");
+        // the faces in reverse walk order, as the gradient loop visits them
+        for(int64_t j = cf_off[i + 1]; j-- > cf_off[i];) {
+            const int32_t e = cf_entry[j];
+            const int64_t f = e >> 1;
+            if(e < 0 || f >= F) return set_error(SFX_ERR_ARG, "sfx_cell_closure: entry");
+            const double A = area[f];
+            if(!(A != 0.0)) continue;
+            double r[3];
+            for(int d = 0; d < ndim; d++) r[d] = (e & 1) ? rRL[d * F + f] : -rRL[d * F + f];
+            const double rn = (ndim == 3) ? std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])
+                                          : std::sqrt(r[0] * r[0] + r[1] * r[1]);
+            for(int d = 0; d < ndim; d++) S[d] += A * r[d] / rn;
+        }
+        for(int d = 0; d < ndim; d++) closure[d * C + i] = S[d];
     }
     return SFX_OK;
 }
@@ -333,8 +363,9 @@ int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad, void* dQL, 
 }
 
 int sfx_update_step_timed(sfx_solver* h, int64_t C, double* Q, double* dQ,
-                          const double* volume, double* W, const uint64_t* endtime,
-                          uint64_t current_time, uint8_t* cell_status, void* stream) {
+                          const double* volume, double* W, double* mass,
+                          const uint64_t* endtime, uint64_t current_time,
+                          uint8_t* cell_status, void* stream) {
     if(!h) return set_error(SFX_ERR_ARG, "sfx_update_step: NULL handle");
     if(C < 0) return set_error(SFX_ERR_ARG, "sfx_update_step: C < 0");
     if(C == 0) return SFX_OK;
@@ -343,14 +374,14 @@ int sfx_update_step_timed(sfx_solver* h, int64_t C, double* Q, double* dQ,
     if(rc) return rc;
     unsigned long long* ctr = h->stats_on ? h->d_ctr : nullptr;
     hipError_t e = launch_update(h->ndim, h->c, C, Q, dQ, volume, W, endtime, current_time,
-                                 cell_status, ctr, (hipStream_t)stream);
+                                 cell_status, mass, ctr, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "update kernel launch");
     return SFX_OK;
 }
 
 int sfx_update_step(sfx_solver* h, int64_t C, double* Q, double* dQ, const double* volume,
-                    double* W, uint8_t* cell_status, void* stream) {
-    return sfx_update_step_timed(h, C, Q, dQ, volume, W, nullptr, 0, cell_status, stream);
+                    double* W, double* mass, uint8_t* cell_status, void* stream) {
+    return sfx_update_step_timed(h, C, Q, dQ, volume, W, mass, nullptr, 0, cell_status, stream);
 }
 
 }  // extern "C"

@@ -91,6 +91,7 @@ struct sfx_h5 {
         t.size = (int)u(off + 4, 4);
         if(cls == 0) {
             if(bits & 1) throw Error{"big-endian integers are not supported"};
+            if(t.size < 1 || t.size > 8) throw Error{"integers of 1 to 8 bytes only"};
             t.cls = 0;
             t.is_signed = (bits >> 3) & 1;
             return 8 + 4;
@@ -243,6 +244,7 @@ struct sfx_h5 {
 
     std::string heap_name(uint64_t heap, uint64_t off) const {
         const uint64_t h = abs_(heap);
+        need(h, 8 + 3 * (uint64_t)L);  // signature, sizes and the segment address
         if(std::memcmp(&buf[h], "HEAP", 4) != 0) throw Error{"bad local heap"};
         const uint64_t seg = abs_(addr(h + 8 + 2 * (uint64_t)L));
         uint64_t p = seg + off;

@@ -52,16 +52,18 @@ class FluxPass:
                                            _stream(stream)), "sfx_face_flux_pass")
 
     def update(self, stream=None, endtime=None, current_time=0):
-        """Q -= dQ, dQ = 0, W = get_W(V, Q) (writes the mesh's W in place).
-        With endtime (a device uint64 tensor [C]) W is rebuilt only where
-        endtime == current_time, the time after the step (individual time
-        steps, src/Simulation.cpp:792-796)."""
+        """Q -= dQ, dQ = 0, W = get_W(V, Q) (writes the mesh's W in place),
+        and the mesh's masses take the new Q.m (read by the next pass's
+        mass-flux limiter).  With endtime (a device uint64 tensor [C]) W is
+        rebuilt only where endtime == current_time, the time after the step
+        (individual time steps, src/Simulation.cpp:792-796)."""
         a = self.m.arr
         et = None if endtime is None else self._v(endtime)
         C.check(C.lib().sfx_update_step_timed(self.s.handle, self.m.C, self._v(self.m.Q),
                                               self._v(self.m.dQ), self._v(a["volume"]),
-                                              self._v(a["W"]), et, int(current_time),
-                                              self._v(self.m.cstatus), _stream(stream)),
+                                              self._v(a["W"]), self._v(a["mass"]), et,
+                                              int(current_time), self._v(self.m.cstatus),
+                                              _stream(stream)),
                 "sfx_update_step_timed")
 
     def step(self, stream=None):

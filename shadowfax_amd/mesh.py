@@ -158,6 +158,27 @@ def face_geometry(mesh):
     return g
 
 
+def cell_closure(b):
+    """sfx_cell_closure: every cell's closure residual sum A r_ij/|r_ij|
+    [ndim][C] (FP64, host), from the face areas, rRL and the cell entries of
+    a prepared bundle."""
+    L = _lib()
+    nd, C, F = b["ndim"], b["C"], b["F"]
+    out = np.zeros((nd, C))
+    off = np.ascontiguousarray(b["cf_off"], np.int64)
+    ent = np.ascontiguousarray(b["cf_entry"], np.int32)
+    area, rRL = np.ascontiguousarray(b["area"], np.float64), np.ascontiguousarray(b["rRL"])
+    Cp.check(L.sfx_cell_closure(nd, C, F, _p(off, _i64p), _p(ent, _i32p), _p(area), _p(rRL),
+                                _p(out)), "sfx_cell_closure")
+    return out
+
+
+# a closure residual below this fraction of the cell's total face area is the
+# round-off of a tessellation whose cells close; the mixed pass then leaves
+# the correction out
+CLOSURE_TOL = 1e-12
+
+
 def cell_entries(mesh):
     L = _lib()
     off = np.ascontiguousarray(mesh["cf_off"], np.int64)
@@ -279,12 +300,15 @@ def prepare(mesh, W, gamma=5.0 / 3.0, kind=0, vel=None, dt=None, cfl=0.4, active
     b.update(face_geometry(mesh))
     b["cf_entry"] = cell_entries(mesh)
     b["gather_off"], b["gather_entry"] = gather_index(mesh)
+    S = cell_closure(b)
+    ta = np.where(b["total_area"] > 0, b["total_area"], 1.0)
+    b["closure"] = S if C and (np.abs(S).max(0) / ta).max() > CLOSURE_TOL else None
     return b
 
 
 # ------------------------------------------------------------- device --
 
-T_ARRAYS = ("vel", "area", "dL", "dR", "rRL", "mc", "rcent")
+T_ARRAYS = ("area", "dL", "dR", "rRL", "mc", "rcent")
 F64_ARRAYS = ("mass", "total_area", "dt", "volume")
 
 
@@ -312,9 +336,13 @@ class DeviceMesh:
         t = lambda a, dt: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=self.dev)
         self.arr = {}
         for k in T_ARRAYS:
-            a = b[k].T if k == "vel" else b[k]
-            self.arr[k] = t(a, tdt)
-        self.arr["W"] = t(b["W"].T, torch.float64)  # FP64 in both passes
+            self.arr[k] = t(b[k], tdt)
+        # FP64 in both passes: the states, and the generator velocities (the
+        # face frame's velocity is subtracted from the states, and cold gas
+        # moving at many times its sound speed loses its relative velocity
+        # in FP32: 5e-6 face errors on the golden 3D meshes)
+        self.arr["W"] = t(b["W"].T, torch.float64)
+        self.arr["vel"] = t(b["vel"].T, torch.float64)
         for k in F64_ARRAYS:
             self.arr[k] = t(b[k], torch.float64)
         self.arr["active"] = t(b["active"], torch.uint8)
@@ -350,6 +378,10 @@ class DeviceMesh:
         self.fstatus = torch.zeros(self.F, dtype=torch.uint8, device=self.dev)
         self.cstatus = torch.zeros(self.C, dtype=torch.uint8, device=self.dev)
         self.use_twin = storage == "mixed"
+        # the mixed gradients' closure correction, for face lists whose cells
+        # do not close (None: they do)
+        if storage == "mixed" and b.get("closure") is not None:
+            self.arr["closure"] = t(b["closure"], tdt)
         self.desc = self._desc()
 
     def _desc(self):
@@ -368,6 +400,7 @@ class DeviceMesh:
         d.egeo_group = p("egeo_group") if self.grouped else None
         d.egeo_rows = self.egeo_rows if self.grouped else 0
         d.twin = p("twin") if self.use_twin else None
+        d.closure = p("closure") if "closure" in a else None
         return d
 
     def build_entries(self, solver_handle, stream=None):

@@ -253,14 +253,13 @@ def build_ensemble(solver, dev, w, classes=None, arena_cells=4_000_000, log=None
     ac = max(arena_cells, cmax)
     af = max(int(3.2 * ac), max(x[1] for x in sizes))
     ens = Ensemble(solver, w["n"], C, F, E, ac, af)
-    tdt = torch.float64 if solver.precision == "fp64" else torch.float32
     faces = np.zeros(w["n"], np.int64)
     for i in range(w["n"]):
         b, dm = tmpl[w["cls"][i]]
         pos = np.mod(b["pos"] + w["shift"][:, i:i + 1], 1.0)
         W, vel, Q, dt = mesh_state(b, pos, int(w["ic"][i]), float(w["icu"][i]), g, kind)
         dm.arr["W"].copy_(torch.from_numpy(np.ascontiguousarray(W.T)))
-        dm.arr["vel"].copy_(torch.from_numpy(np.ascontiguousarray(vel.T)).to(tdt))
+        dm.arr["vel"].copy_(torch.from_numpy(np.ascontiguousarray(vel.T)))
         dm.arr["mass"].copy_(torch.from_numpy(np.ascontiguousarray(Q[0])))
         dm.arr["dt"].fill_(dt)
         dm.Q.copy_(torch.from_numpy(np.ascontiguousarray(Q.T)))

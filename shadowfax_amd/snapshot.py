@@ -272,7 +272,8 @@ def write_gadget_snapshot(path, box, time, positions, density, velocities, press
     (cgs factors; SI when not given) and /PartType0 with Coordinates
     (double, [n][3]), Density (double), Velocities (float, [n][3]),
     InternalEnergy = 1.5 p / rho (0 where rho = 0), ParticleIDs (uint64),
-    the passive scalars given and, with masses, Masses."""
+    every passive scalar (zeros where not given) and, with masses, Masses.
+    BoxSize holds three values (z = 0 in 2D), as the reference writes it."""
     pos = np.asarray(positions, np.float64)
     n, nd = pos.shape
     p3 = np.zeros((n, 3))
@@ -290,7 +291,10 @@ def write_gadget_snapshot(path, box, time, positions, density, velocities, press
     num[0] = n
     with H5Writer() as w:
         w.group("/Header")
-        w.attr("/Header", "BoxSize", np.asarray(box, np.float64)[:nd], np.float64)
+        # three values in 2D too, z = 0 (src/GadgetSnapshotWriter.cpp:186-195)
+        box3 = np.zeros(3)
+        box3[:nd] = np.asarray(box, np.float64)[:nd]
+        w.attr("/Header", "BoxSize", box3, np.float64)
         w.attr("/Header", "Flag_Entropy_ICs", np.zeros(6, np.uint32), np.uint32)
         w.attr("/Header", "MassTable", np.zeros(6), np.float64)
         w.attr("/Header", "NumFilesPerSnapshot", np.array([1], np.int32), np.int32)
@@ -313,8 +317,17 @@ def write_gadget_snapshot(path, box, time, positions, density, velocities, press
         w.dataset("/PartType0/Velocities", v3, np.float32)
         w.dataset("/PartType0/InternalEnergy", u, np.float64)
         w.dataset("/PartType0/ParticleIDs", ids, np.uint64)
-        for name, vals in (paqs or {}).items():
-            w.dataset("/PartType0/" + name, np.asarray(vals, np.float64), np.float64)
+        # every passive scalar, zeros where none is given
+        # (src/GadgetSnapshotWriter.cpp:376-383)
+        paqs = dict(paqs or {})
+        unknown = set(paqs) - set(PAQ_NAMES)
+        if unknown:
+            raise ValueError("unknown passive scalars %s (known: %s)" % (sorted(unknown),
+                                                                         PAQ_NAMES))
+        for name in PAQ_NAMES:
+            vals = paqs.get(name)
+            w.dataset("/PartType0/" + name,
+                      np.zeros(n) if vals is None else np.asarray(vals, np.float64), np.float64)
         if masses is not None:
             w.dataset("/PartType0/Masses", np.asarray(masses, np.float64), np.float64)
         w.save(path)

@@ -1,0 +1,108 @@
+"""The oracle's face pass computed exactly on what the mixed pass stores.
+
+The mixed pass keeps the states, generator velocities and per-cell scalars
+in FP64 and stores the gradients and the per-face geometry (area, the two d
+vectors, rRL, mc, the centroid distance) in FP32.  Feeding the oracle those
+stored values (its FP64 arithmetic, the reference's algorithm) separates the
+two parts of the mixed pass's deviation from the FP64 pass:
+
+  * storage: stored-input oracle vs the FP64 oracle -- what FP32 storage of
+    those inputs costs, whatever the arithmetic;
+  * arithmetic: the mixed pass vs the stored-input oracle -- what its kernels
+    add (FP32 solver iterations, FP32 face fluxes, reordered arithmetic).
+
+Storage rounding moves a face's flux by about (the input's relative rounding,
+6e-8) x (the face-frame speed / the sound speed): on cold gas moving at many
+times its sound speed relative to the face, as in the golden 3D meshes
+(random velocities of 0.3 against sound speeds of 4e-3), single faces move by
+a few 1e-6; on the benchmark states (Kelvin-Helmholtz, Sedov at rest) they
+stay below 1e-6.
+"""
+import numpy as np
+
+import oracle as O
+
+GEO = ("dL", "dR", "rRL", "mc", "rcent")
+
+
+def f32(a):
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+def stored_bundle(b, grad):
+    """The oracle bundle of what the mixed pass reads: grad (ndim, ndim+2, C)
+    as stored (FP32 values), the face geometry and areas rounded to FP32."""
+    ob = dict(b)
+    ob["grad"] = f32(grad)
+    ob["area"] = f32(b["area"])
+    for k in GEO:
+        ob["geo_" + k] = f32(b[k])
+    return ob
+
+
+def stored_pass(kind, b, grad):
+    """(dQ, dQL, status) of the oracle's face pass on the stored inputs."""
+    dQ, dQL, st, _ = O.face_flux_pass(kind, stored_bundle(b, grad))
+    return dQ, dQL, st
+
+
+def face_err(b, dQL, dQL_ref, scale):
+    """Per-face normwise error against dQL_ref (floor: scale per face)."""
+    den = np.maximum(np.abs(dQL_ref).max(0), scale)
+    return np.abs(np.asarray(dQL, np.float64) - dQL_ref).max(0) / np.where(den > 0, den, 1.0)
+
+
+def twin_mapped(b, dQL):
+    """The mixed pass evaluates each periodic pair once and writes the second
+    copy's row as the first's negated: the same mapping applied to a pass
+    that evaluated both copies."""
+    d = np.asarray(dQL, np.float64).copy()
+    tw = b["twin"]
+    sk = (b["ghost"] == 1) & (tw >= 0) & (tw < np.arange(b["F"]))
+    d[:, sk] = -d[:, tw[sk]]
+    return d
+
+
+def split_errors(kind, b, grad_mx, dQL64, dQLmx, gamma=5.0 / 3.0):
+    """Per-face errors of the mixed pass's fluxes dQLmx against the FP64 pass's
+    dQL64, split into storage (stored-input oracle vs dQL64) and arithmetic
+    (dQLmx vs the stored-input oracle).  Returns (total, storage, arithmetic)
+    arrays [F]."""
+    from shadowfax_amd.metrics import face_flux_scale
+    sc = face_flux_scale(b, gamma)
+    _, sdQL, _ = stored_pass(kind, b, grad_mx)
+    sdQL = twin_mapped(b, sdQL)
+    mx = twin_mapped(b, dQLmx)
+    return face_err(b, mx, dQL64, sc), face_err(b, sdQL, dQL64, sc), face_err(b, mx, sdQL, sc)
+
+
+FACE_KEYS = ("left", "right", "area", "midpoint", "shift", "ghost", "twin") + GEO
+
+
+def face_subset(b, faces):
+    """The bundle with only the given faces (every cell kept): the face pass
+    over a sample of a large mesh."""
+    faces = np.asarray(faces, np.int64)
+    s = dict(b)
+    for k in FACE_KEYS:
+        a = b[k]
+        s[k] = a[..., faces] if a.ndim > 1 else a[faces]
+    s["F"] = len(faces)
+    return s
+
+
+def split_errors_on(kind, b, faces, grad64, grad_mx, dQL64, dQLmx, gamma=5.0 / 3.0):
+    """split_errors for a subset of faces of a large mesh (no twin mapping:
+    the mixed pass writes both copies of a periodic pair).  grad64: the FP64
+    pass's gradients (the oracle's, bit for bit)."""
+    from shadowfax_amd.metrics import face_flux_scale
+    s = face_subset(b, faces)
+    ob = dict(s)
+    ob["grad"] = grad64
+    _, odQL, _, _ = O.face_flux_pass(kind, ob)
+    _, sdQL, _ = stored_pass(kind, s, grad_mx)
+    sc = face_flux_scale(s, gamma)
+    ref = np.asarray(dQL64, np.float64)[:, faces]
+    mx = np.asarray(dQLmx, np.float64)[:, faces]
+    return (face_err(s, mx, ref, sc), face_err(s, sdQL, ref, sc), face_err(s, mx, sdQL, sc),
+            face_err(s, odQL, ref, sc))

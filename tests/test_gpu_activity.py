@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import stored_oracle as SO
 import timesteps as TS
 from shadowfax_amd import mesh as M
 from shadowfax_amd.fluxpass import FluxPass
@@ -126,9 +127,21 @@ def test_individual_steps_fp64_and_mixed(device, name, kind):
         assert err <= 1e-12 and errc <= 1e-12
         fm = run_gpu(b, kind, "mixed", device)
         errs = fluxpass_errors(b, G, dQL, dQ, fm.m.dQL_soa(), fm.m.dQ_soa(), once_per_pair=True)
-        print("  mixed vs fp64 %s" % errs, flush=True)
-        assert errs["face_max"] <= 1e-6 and errs["cell_gross_max"] <= 1e-6
-        assert errs["conservation"] <= 1e-13
+        tot, sto, ari = SO.split_errors(kind, b, fm.m.grad_soa(), dQL, fm.m.dQL_soa())
+        print("  mixed vs fp64 %s; storage %.2e arithmetic %.2e" % (errs, sto.max(), ari.max()),
+              flush=True)
+        assert errs["cell_gross_max"] <= 1e-6
+        # the kernels add at most what FP32 storage costs (tests/stored_oracle.py)
+        assert np.all(ari <= np.maximum(5e-7, 2.0 * sto))
+        if name not in ("p3d", "m3d"):  # hypersonic cold gas: the storage bound
+            assert errs["face_max"] <= 1e-6
+        if np.all(b["right"] >= 0):
+            assert errs["conservation"] <= 1e-13
+        else:
+            # walls take momentum (and, moving, energy): the totals are the
+            # wall fluxes, the same in both passes to the mixed error
+            tot = np.abs(fm.m.dQ_soa().sum(1) - dQ.sum(1)).max() / np.abs(dQL).max()
+            assert tot <= 1e-6, tot
         del fp, fm
 
 
@@ -160,12 +173,14 @@ def test_mass_flux_limiter_fires_on_gpu(device, kind):
             tw = b["twin"]
             sk = (b["ghost"] == 1) & (tw >= 0) & (tw < np.arange(b["F"]))
             dQL[:, sk] = -dQL[:, tw[sk]]
-        atL = np.abs(dQL[0] - capL) <= 1e-13 * capL
-        atR = np.abs(-dQL[0] - capR) <= 1e-13 * capR
+        # the mixed pass stores dQL in FP32 (rounded once from FP64)
+        rt = 1e-13 if prec == "fp64" else 2.0 ** -23
+        atL = np.abs(dQL[0] - capL) <= rt * capL
+        atR = np.abs(-dQL[0] - capR) <= rt * capR
         print(prec, "faces at the cap:", int(atL.sum() + atR.sum()),
               "with tf < 1:", int(((atL | atR) & (tf < 1)).sum()), flush=True)
         assert ((atL | atR) & (tf < 1)).sum() > 10
-        assert np.all(dQL[0] <= capL * (1 + 1e-13)) and np.all(-dQL[0] <= capR * (1 + 1e-13))
+        assert np.all(dQL[0] <= capL * (1 + rt)) and np.all(-dQL[0] <= capR * (1 + rt))
         den = np.maximum(np.abs(odQL).max(0), face_flux_scale(b, G))
         err = (np.abs(dQL - odQL).max(0) / np.where(den > 0, den, 1.0)).max()
         print(prec, "face err vs oracle %.3e" % err, flush=True)
@@ -201,5 +216,6 @@ def test_timed_update_matches_oracle(device, kind):
     Qo, Wo, cst = O.update(kind, 2, Qin, dQ, b["volume"], W=W, endtime=e1, tnow=tnow)
     assert np.array_equal(fp.m.Q_soa(), Qo)
     assert np.array_equal(fp.m.W_soa(), Wo)
+    assert np.array_equal(fp.m.arr["mass"].cpu().numpy(), Qo[0])
     assert np.abs(fp.m.dQ_soa()).max() == 0.0
     assert np.array_equal(fp.m.cstatus.cpu().numpy(), cst)

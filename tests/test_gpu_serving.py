@@ -111,3 +111,73 @@ def test_autotune_keeps_states(device):
     ens.step([0, 1, 2, 3])
     ens.sync()
     check_meshes(ens, bs, kind, prec, device, [1, 1, 1, 1])
+
+
+def workload_mesh(w, classes, i):
+    """Mesh i of the serving workload as its own prepared bundle: its size
+    class's tessellation with its own states, as build_ensemble copies it."""
+    b = dict(classes[int(w["cls"][i])])
+    pos = np.mod(b["pos"] + w["shift"][:, i:i + 1], 1.0)
+    W, vel, Q, dt = S.mesh_state(b, pos, int(w["ic"][i]), float(w["icu"][i]), w["gamma"],
+                                 w["kind"])
+    b.update(W=W, vel=vel, Q=Q, mass=Q[0].copy(), dt=np.full(b["C"], dt), dt_value=dt)
+    return b
+
+
+def test_serving_workload_2000_meshes(device):
+    """The serving workload at full size (the bench's): 2,000 periodic 2D
+    meshes, cell counts log-uniform in [1e3, 2.5e5] (88M cells), Sod /
+    Kelvin-Helmholtz / Sedov states, HLLC; the 4,000 requests of its Poisson
+    trace served at once, batched mixed and plain FP64.  Each path applies
+    every request exactly once: meshes without a request keep their states;
+    the smallest, the largest and 12 random meshes equal that many standalone
+    FluxPass steps bit for bit; on those meshes the mixed pass is within
+    1e-6 of the FP64 pass per face and per cell and conserves to FP64
+    round-off."""
+    import torch
+    from shadowfax_amd.metrics import fluxpass_errors
+    w = S.ensemble_workload(2000)
+    classes = S.ensemble_classes(w)
+    nreq = 4000
+    _, mesh = S.poisson_trace(nreq, 1.0, 2000, w["seed"] + 1)
+    counts = np.bincount(mesh, minlength=2000)
+    sizes = np.array([classes[int(c)]["C"] for c in w["cls"]])
+    rng = np.random.default_rng(11)
+    sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))} |
+                    set(rng.choice(2000, 12, replace=False).tolist()))
+    untouched = np.where(counts == 0)[0][:20]
+    assert len(untouched) > 0 and counts[sample].max() >= 2
+    got = {}
+    for prec, mode in (("mixed", S.SERVE_BATCHED), ("fp64", S.SERVE_PLAIN)):
+        print("serving %s: building the ensemble" % prec, flush=True)
+        solver = RiemannSolver(1, 2, G, 1e-8, -5.0, prec)
+        ens, faces = S.build_ensemble(solver, device, w, classes)
+        assert sum(ens.cells) > 8e7
+        Q0 = {int(i): ens.read(int(i), S.ENS_Q) for i in untouched}
+        lat, bat, st = ens.serve(np.zeros(nreq), mesh, mode)
+        print("  served %d requests in %d steps, %.3f s" % (st["nreq"], st["nbatch"], st["wall_s"]),
+              flush=True)
+        assert st["nreq"] == nreq and np.all(np.isfinite(lat)) and np.all(lat > 0)
+        for i in untouched:
+            assert np.array_equal(ens.read(int(i), S.ENS_Q), Q0[int(i)]), i
+        got[prec] = {i: (ens.read(i, S.ENS_W), ens.read(i, S.ENS_Q)) for i in sample}
+        ens.close()
+        del ens
+        torch.cuda.empty_cache()
+    for i in sample:
+        b = workload_mesh(w, classes, i)
+        print("mesh %d: %d cells, %d requests" % (i, b["C"], counts[i]), flush=True)
+        for prec in ("mixed", "fp64"):
+            W, Q = standalone(b, 1, prec, device, int(counts[i]))
+            assert np.array_equal(got[prec][i][0], W), (i, prec)
+            assert np.array_equal(got[prec][i][1], Q), (i, prec)
+        runs = {}
+        for prec in ("fp64", "mixed"):
+            fp = FluxPass(RiemannSolver(1, 2, G, 1e-8, -5.0, prec), M.DeviceMesh(b, prec, device))
+            fp.run()
+            torch.cuda.synchronize()
+            runs[prec] = (fp.m.dQL_soa(), fp.m.dQ_soa())
+        e = fluxpass_errors(b, G, runs["fp64"][0], runs["fp64"][1], runs["mixed"][0],
+                            runs["mixed"][1], once_per_pair=True)
+        assert e["face_max"] <= 1e-6 and e["cell_gross_max"] <= 1e-6, (i, e)
+        assert e["conservation"] <= 1e-13, (i, e)

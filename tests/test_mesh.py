@@ -77,6 +77,75 @@ def test_volumes_closure_and_face_contract(nd):
     assert np.abs(d).max() < 0.1
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+def test_cell_closure_vanishes_on_voronoi_cells(nd):
+    """sfx_cell_closure on the host tessellation: every cell closes to
+    round-off, so prepare() leaves the mixed pass's correction out."""
+    pos = M.jittered_grid(40, 0.25, seed=5) if nd == 2 else M.poisson_points(3000, seed=5)
+    m = M.voronoi_periodic(pos)
+    b = M.prepare(m, M.sod_ics(m), 5.0 / 3.0)
+    S = M.cell_closure(b)
+    assert S.shape == (nd, b["C"])
+    assert (np.abs(S).max(0) / b["total_area"]).max() <= 1e-13
+    assert b["closure"] is None
+
+
+def _golden_mesh(nd, name):
+    z = np.load(os.path.join(HERE, "golden", "mesh_%dd.npz" % nd))
+    keys = ("pos", "volume", "centroid", "left", "right", "area", "midpoint", "shift", "ghost",
+            "twin", "cf_off", "cf_face", "W", "vel")
+    m = {k: z[name + "_" + k] for k in keys}
+    m.update(ndim=nd, C=m["volume"].shape[0], F=m["area"].shape[0], box=np.ones(nd))
+    return m
+
+
+@pytest.mark.parametrize("nd,name", [(2, "mir10"), (3, "m3d")])
+def test_cell_closure_of_open_cells_and_the_gradient_identity(nd, name):
+    """The golden mirror meshes turn periodic faces into mirror faces without
+    moving them, so their wall cells do not close.  sfx_cell_closure equals a
+    per-cell sum over each cell's faces of A (x_i - x_j)/|x_i - x_j| (x_j the
+    neighbour's generator: the periodic image, or the mirror image of x_i),
+    and with it the reference's gradient sum (src/VorCell.cpp:493-494)
+    equals the difference form minus W_i * closure_i for arbitrary states
+    (a sign or factor slip in either fails)."""
+    m = _golden_mesh(nd, name)
+    b = M.prepare(m, m["W"], 5.0 / 3.0, vel=m["vel"])
+    S = M.cell_closure(b)
+    C = b["C"]
+    pos, L, R, sh = m["pos"], m["left"], m["right"], m["shift"]
+    rng = np.random.default_rng(7)
+    worst_rel, worst_id = 0.0, 0.0
+    for i in range(C):
+        Si = np.zeros(nd)
+        Wi = rng.normal(size=nd + 2)
+        ref = np.zeros((nd, nd + 2))
+        diff = np.zeros((nd, nd + 2))
+        for f in m["cf_face"][m["cf_off"][i]:m["cf_off"][i + 1]]:
+            A = m["area"][f]
+            if A == 0:
+                continue
+            if L[f] == i:
+                xj = pos[:, R[f] if R[f] >= 0 else i] + sh[:, f]
+            else:
+                xj = pos[:, L[f]]
+            rij = pos[:, i] - xj
+            rn = np.linalg.norm(rij)
+            c = m["midpoint"][:, f] - 0.5 * (pos[:, i] + xj)
+            Si += A * rij / rn
+            Wj = rng.normal(size=nd + 2)
+            for l in range(nd):
+                ref[l] += A * ((Wj - Wi) * c[l] / rn - 0.5 * (Wi + Wj) * rij[l] / rn)
+                diff[l] += (Wj - Wi) * (A * (c[l] - 0.5 * rij[l]) / rn)
+        worst_rel = max(worst_rel, np.abs(S[:, i] - Si).max() / b["total_area"][i])
+        corr = diff - np.outer(S[:, i], Wi)
+        worst_id = max(worst_id, np.abs(ref - corr).max() / max(np.abs(ref).max(), 1e-300))
+    assert worst_rel <= 1e-14
+    assert worst_id <= 1e-12
+    # the wall cells do not close, so prepare() hands the correction on
+    rel = np.abs(S).max(0) / b["total_area"]
+    assert rel.max() > 1e-3 and b["closure"] is not None
+
+
 @pytest.mark.parametrize("nd,C", [(2, 300), (3, 400)])
 def test_tessellation_matches_qhull(nd, C):
     spatial = pytest.importorskip("scipy.spatial")

@@ -115,11 +115,19 @@ def test_written_bytes_follow_the_format(tmp_path):
     assert u64(24) == 0 and u64(32) == 2 ** 64 - 1 and u64(40) == len(b)
     assert int.from_bytes(b[72:76], "little") == 1
     assert b[u64(80):u64(80) + 4] == b"TREE" and b[u64(88):u64(88) + 4] == b"HEAP"
-    # 2D positions are padded to 3 coordinates; BoxSize keeps 2 values and
-    # the reader pads it as the reference does
+    # 2D positions are padded to 3 coordinates; BoxSize holds 3 values with
+    # z = 0, as the reference writes it in 2D
+    # (src/GadgetSnapshotWriter.cpp:186-195); every passive scalar is written,
+    # zeros where none is given (:376-383)
     s = S.read_gadget_snapshot(path, ndim=2)
     assert s["gas"]["positions"].shape == (10, 3) and np.all(s["gas"]["positions"][:, 2] == 0)
-    assert np.array_equal(s["box"], [1.0, 1.0, 1.0])
+    assert np.array_equal(s["box"], [1.0, 1.0, 0.0])
+    assert sorted(s["gas"]["paqs"]) == sorted(S.PAQ_NAMES)
+    assert all(np.all(v == 0) for v in s["gas"]["paqs"].values())
+    with pytest.raises(ValueError):
+        S.write_gadget_snapshot(path, [1.0, 1.0], 0.0, paqs={"Zn": np.zeros(10)},
+                                **{k: v[:, :2] if v.ndim == 2 else v
+                                   for k, v in gas_arrays(10).items()})
 
 
 def test_group_of_many_members(tmp_path):
