@@ -338,6 +338,19 @@ def face_flux_pass(kind, mesh, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0):
     return dQ, dQL, st, c
 
 
+def face_states(kind, mesh, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0):
+    """(WL, WR) [(ndim+2), F]: the face-frame states every face hands its
+    Riemann solver, after the predictor (zeros for skipped faces)."""
+    ndim = int(mesh["ndim"])
+    s = params(kind, ndim, gamma, tol, cutoff)
+    m, keep = mesh_struct(mesh)
+    F = int(mesh["F"])
+    WL = np.zeros((ndim + 2, F))
+    WR = np.zeros((ndim + 2, F))
+    lib().sfxo_face_states(ct.byref(s), ct.byref(m), _p(WL), _p(WR))
+    return WL, WR
+
+
 def threaded_pass(kind, mesh, nthreads, gamma=5.0 / 3.0, tol=1e-8, cutoff=-5.0,
                   cells=None, faces=None, timer=None):
     """The flux pass (gradients, face fluxes, ordered gather) on nthreads

@@ -1153,7 +1153,8 @@ static void predict_side(const sfxo_solver* s, const sfxo_mesh* m,
  * copy or a mirror), no gravity: dQLo = the flux exchanged, seen from the
  * left cell (zero when the face is skipped).  Returns the status bits. */
 static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
-                              int64_t f, double* dQLo, sfxo_counters* c) {
+                              int64_t f, double* dQLo, sfxo_counters* c,
+                              double* wstate) {
     int nd = m->ndim, NC = nd + 2;
     int64_t C = m->C, F = m->F;
     unsigned st = 0;
@@ -1220,6 +1221,10 @@ static unsigned face_flux_one(const sfxo_solver* s, const sfxo_mesh* m,
     predict_side(s, m, L, WL, WR, d, r, R >= 0, dt, WLrec);
     if(WL[0] != 0.) memcpy(WL, WLrec, sizeof(double) * NC);
     if(WR[0] != 0.) memcpy(WR, WRrec, sizeof(double) * NC);
+    if(wstate) {
+        memcpy(wstate, WL, sizeof(double) * NC);
+        memcpy(wstate + NC, WR, sizeof(double) * NC);
+    }
 
     /* get_normal, :553-571 */
     double nrm[3], dd[4];
@@ -1288,7 +1293,7 @@ unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
     unsigned all = 0;
     for(int64_t f = 0; f < F; f++) {
         double dQL[5];
-        unsigned st = face_flux_one(s, m, f, dQL, c);
+        unsigned st = face_flux_one(s, m, f, dQL, c, NULL);
         scatter_one(m, f, dQL, dQ);
         if(dQL_face)
             for(int k = 0; k < NC; k++) dQL_face[(int64_t)k * F + f] = dQL[k];
@@ -1309,12 +1314,33 @@ unsigned sfxo_face_flux_range(const sfxo_solver* s, const sfxo_mesh* m,
     unsigned all = 0;
     for(int64_t f = f0; f < f1; f++) {
         double dQL[5];
-        unsigned st = face_flux_one(s, m, f, dQL, c);
+        unsigned st = face_flux_one(s, m, f, dQL, c, NULL);
         for(int k = 0; k < NC; k++) dQL_face[(int64_t)k * F + f] = dQL[k];
         if(face_status) face_status[f] = (uint8_t)st;
         all |= st;
     }
     return all;
+}
+
+/* The states every face hands its Riemann solver: left and right, in the
+ * face frame, after the predictor, limiter and vacuum keep (zeros for a
+ * skipped face; a mirror face's right state is zero, the solver mirrors the
+ * left one).  WLo, WRo: [(ndim+2)][F].  Testing aid. */
+void sfxo_face_states(const sfxo_solver* s, const sfxo_mesh* m, double* WLo,
+                      double* WRo) {
+    int NC = m->ndim + 2;
+    int64_t F = m->F;
+    sfxo_counters c;
+    memset(&c, 0, sizeof c);
+    for(int64_t f = 0; f < F; f++) {
+        double dQL[5], w[10];
+        memset(w, 0, sizeof w);
+        face_flux_one(s, m, f, dQL, &c, w);
+        for(int k = 0; k < NC; k++) {
+            WLo[(int64_t)k * F + f] = w[k];
+            WRo[(int64_t)k * F + f] = w[NC + k];
+        }
+    }
 }
 
 /* dQ of cells c0..c1-1 from per-face fluxes: the cell's gather entries

@@ -154,6 +154,11 @@ void sfxo_face_velocities(const sfxo_mesh* m, double* vface, double* normal);
 unsigned sfxo_face_flux_pass(const sfxo_solver* s, const sfxo_mesh* m,
                              double* dQ, double* dQL_face, uint8_t* face_status,
                              sfxo_counters* c);
+/* The face-frame left and right states every face hands its Riemann solver
+ * (after the predictor; zeros for a skipped face), [(ndim+2)][F] each.
+ * Testing aid. */
+void sfxo_face_states(const sfxo_solver* s, const sfxo_mesh* m, double* WLo,
+                      double* WRo);
 
 /* Faces f0..f1-1 without the scatter (dQL_face required, face_status may be
  * NULL), and the ordered per-cell sums of such fluxes for cells c0..c1-1

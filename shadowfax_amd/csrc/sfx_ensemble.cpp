@@ -607,29 +607,35 @@ int sfx_ensemble_autotune(sfx_ensemble* e, const int32_t* ids, int32_t n, int re
     if(n == 0) return set_error(SFX_ERR_ARG, "sfx_ensemble_autotune: empty batch");
     if((rc = bind_device(e->h))) return rc;
     if((rc = build_entries(e))) return rc;
-    // the trial steps advance the batch's meshes: keep W and Q to restore
+    // the trial steps advance the batch's meshes: keep W, Q and the masses
+    // (which the update refreshes) to restore
     int64_t cells = 0;
     for(int32_t k = 0; k < n; k++) cells += e->nc[ids[k]];
     hipError_t err = hipStreamSynchronize(e->stream);
     double* save = nullptr;
-    if(err == hipSuccess) err = hipMalloc(&save, 16 * (size_t)cells * e->NC);
+    if(err == hipSuccess) err = hipMalloc(&save, 8 * (size_t)cells * (2 * e->NC + 1));
     if(err != hipSuccess) return hip_error(err, "sfx_ensemble_autotune");
     auto copy_state = [&](bool back) {
-        int64_t o = 0;
+        int64_t o = 0, om = 0;
         for(int32_t k = 0; k < n; k++) {
             const size_t bytes = 8 * (size_t)e->nc[ids[k]] * e->NC;
             double* w = e->W + e->c0[ids[k]] * e->NC;
             double* q = e->Q + e->c0[ids[k]] * e->NC;
+            double* m = e->mass + e->c0[ids[k]];
             double* sw = save + o;
             double* sq = save + (size_t)cells * e->NC + o;
+            double* sm = save + 2 * (size_t)cells * e->NC + om;
             if(back) {
                 hipMemcpyAsync(w, sw, bytes, hipMemcpyDeviceToDevice, e->stream);
                 hipMemcpyAsync(q, sq, bytes, hipMemcpyDeviceToDevice, e->stream);
+                hipMemcpyAsync(m, sm, 8 * e->nc[ids[k]], hipMemcpyDeviceToDevice, e->stream);
             } else {
                 hipMemcpyAsync(sw, w, bytes, hipMemcpyDeviceToDevice, e->stream);
                 hipMemcpyAsync(sq, q, bytes, hipMemcpyDeviceToDevice, e->stream);
+                hipMemcpyAsync(sm, m, 8 * e->nc[ids[k]], hipMemcpyDeviceToDevice, e->stream);
             }
             o += (int64_t)e->nc[ids[k]] * e->NC;
+            om += e->nc[ids[k]];
         }
     };
     copy_state(false);

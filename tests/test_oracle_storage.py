@@ -77,9 +77,19 @@ def test_storage_cost_on_benchmark_states_below_1e6(case):
 
 @pytest.mark.parametrize("name", ["p3d", "m3d"])
 def test_storage_cost_on_hypersonic_golden_meshes(name):
+    """Single faces move by more than 1e-6; the cells stay within 1e-6; the
+    sensitivity estimate covers the storage cost and is itself bounded."""
     b = golden_bundle(name, O.EXACT)
     e = storage_cost(b, O.EXACT)
     print(name, e)
-    # single faces move by more than 1e-6, bounded; the cells stay within it
     assert 1e-6 < e["face_max"] <= 1e-5
     assert e["cell_gross_max"] <= 1e-6
+    g = O.gradients(O.EXACT, b)
+    ob = dict(b)
+    ob["grad"] = g
+    _, dQL, _, _ = O.face_flux_pass(O.EXACT, ob)
+    _, sdQL, _ = SO.stored_pass(O.EXACT, b, g)
+    sto = SO.face_err(b, sdQL, dQL, face_flux_scale(b, G))
+    sens = SO.sensitivity(O.EXACT, b, g)
+    assert np.all(sto <= sens) and sens.max() <= 2e-5
+    assert np.any(sto > 1e-6)
