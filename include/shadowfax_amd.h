@@ -426,6 +426,19 @@ int sfx_face_flux_pass(sfx_solver* h, const sfx_mesh* m, void* grad,
                        void* dQL, double* dQ, uint8_t* face_status,
                        void* stream);
 
+/* No counterpart in the reference.  sfx_face_flux_pass for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the face fluxes dQL, the pass's only
+ * scratch, live in a static device array of SFX_STATIC_SCRATCH_BYTES per
+ * device instead of a caller workspace; grad stays the caller's (inactive
+ * cells keep their gradients from step to step).  Allocates nothing;
+ * SFX_ERR_ARG when F * (ndim+2) * sizeof(T) exceeds the scratch.  Every call
+ * on one device shares that device's scratch: calls that overlap on one
+ * device (other streams or threads) race. */
+#define SFX_STATIC_SCRATCH_BYTES (64u << 20)
+int sfx_face_flux_pass_static(sfx_solver* h, const sfx_mesh* m, void* grad,
+                              double* dQ, uint8_t* face_status, void* stream);
+
 /* GasParticle::update_Q then get_W (src/utilities/GasParticle.cpp:617-624,
  * src/riemann/ExactRiemannSolver.cpp:716-766): Q -= dQ, dQ = 0,
  * W = get_W(volume, Q) (Q, dQ, W [C][ndim+2]); mass [C] (may be NULL) is

@@ -14,6 +14,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "shadowfax_amd.h")
 SFX_EXACT, SFX_HLLC = 0, 1
 SFX_PREC_FP64, SFX_PREC_MIXED = 0, 1
 SFX_OK = 0
+SFX_ERR_ARG = -1
+SFX_STATIC_SCRATCH_BYTES = 64 << 20
 ST_SOLVER, ST_NANFLUX, ST_ITERCAP, ST_UPDATEQ = 1, 2, 4, 8
 
 vp = ct.c_void_p
@@ -117,6 +119,7 @@ def _mesh_sigs():
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],
         "sfx_face_flux_pass": [vp, md, vp, vp, vp, vp, vp],
+        "sfx_face_flux_pass_static": [vp, md, vp, vp, vp, vp],
         "sfx_update_step": [vp, i64, vp, vp, vp, vp, vp, vp, vp],
         "sfx_update_step_timed": [vp, i64, vp, vp, vp, vp, vp, vp, ct.c_uint64, vp, vp],
     }

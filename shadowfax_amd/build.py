@@ -38,6 +38,7 @@ SOURCES = [
                              "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sfx_capi.cpp", []),
     ("sfx_mesh_capi.cpp", ["-ffp-contract=off"]),
+    ("sfx_static.hip", []),
     ("mesh_voronoi.cpp", ["-ffp-contract=off"]),
     ("mesh_voronoi3d.cpp", ["-ffp-contract=off"]),
     ("ensemble_fp64.hip", ["-ffp-contract=off"]),

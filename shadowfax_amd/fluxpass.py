@@ -51,6 +51,16 @@ class FluxPass:
                                            self._v(self.m.dQL), self._v(self.m.dQ), st,
                                            _stream(stream)), "sfx_face_flux_pass")
 
+    def run_static(self, stream=None, status=True):
+        """run() without a face-flux workspace: sfx_face_flux_pass_static
+        keeps the face fluxes in the library's static device scratch (for
+        callers that cannot allocate; calls that overlap on one device race
+        on it).  dmesh.dQL is not written."""
+        st = self._v(self.m.fstatus) if status else None
+        C.check(C.lib().sfx_face_flux_pass_static(self.s.handle, self._d, self._v(self.m.grad),
+                                                  self._v(self.m.dQ), st, _stream(stream)),
+                "sfx_face_flux_pass_static")
+
     def update(self, stream=None, endtime=None, current_time=0):
         """Q -= dQ, dQ = 0, W = get_W(V, Q) (writes the mesh's W in place),
         and the mesh's masses take the new Q.m (read by the next pass's
