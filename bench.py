@@ -392,6 +392,25 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
         out[name] = r
         log("  %s: capacity %.0f req/s, p50 %.3f ms p99 %.3f ms" %
             (name, cap_req, r["open_loop"]["p50_ms"], r["open_loop"]["p99_ms"]))
+        if mode == S.SERVE_BATCHED:
+            # the same ensemble served from the request queue on the device
+            # (one persistent kernel: admission, planning, the four phases)
+            q = {}
+            lat, bat, st = ens.serve(np.zeros(nreq), mesh, S.SERVE_QUEUE)
+            cap_q = nreq / st["wall_s"]
+            q["capacity"] = {"requests_per_s": cap_q,
+                             "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"],
+                             "mean_batch": st["mean_batch"], "steps": st["nbatch"]}
+            rate = load * cap_q
+            lat, bat, st = ens.serve(arr / rate, mesh, S.SERVE_QUEUE)
+            q["open_loop"] = {"rate_per_s": rate, "p50_ms": st["p50_s"] * 1e3,
+                              "p99_ms": st["p99_s"] * 1e3, "mean_ms": st["mean_s"] * 1e3,
+                              "max_ms": st["max_s"] * 1e3, "mean_batch": st["mean_batch"],
+                              "steps": st["nbatch"],
+                              "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"]}
+            out["queue_mixed"] = q
+            log("  queue_mixed: capacity %.0f req/s, p50 %.3f ms p99 %.3f ms" %
+                (cap_q, q["open_loop"]["p50_ms"], q["open_loop"]["p99_ms"]))
         ens.close()
         del ens
         torch.cuda.empty_cache()

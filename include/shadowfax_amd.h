@@ -490,6 +490,8 @@ typedef struct {
 #define SFX_ENS_DQ 2
 #define SFX_SERVE_BATCHED 0 /* batches through the captured graph */
 #define SFX_SERVE_PLAIN 1   /* one mesh per step, launches sized to the mesh */
+#define SFX_SERVE_QUEUE 2   /* requests posted to a queue on the device, drained,
+                               admitted and stepped by one persistent kernel */
 
 /* Capacities are totals over all meshes (entries: cf_off[C] summed, the
  * same bound is used for the gather index).  The handle's precision selects
@@ -551,7 +553,12 @@ int sfx_serve_admit(int64_t npending, const int32_t* pending,
  * mesh_id[r] arrives arrival_s[r] seconds after the call (ascending); while
  * the device is busy arrivals queue; when it is free the admitted prefix of
  * the queue is stepped (SFX_SERVE_BATCHED) or the head alone
- * (SFX_SERVE_PLAIN).  latency_s[r] = completion - arrival; batch_of[r] (or
+ * (SFX_SERVE_PLAIN).  SFX_SERVE_QUEUE: the host only posts each request,
+ * at its arrival time, to a mailbox in pinned memory; one persistent kernel
+ * (launched cooperatively, every block resident) moves posted requests into
+ * a queue in HBM, admits them by the same rule, plans and runs each batch's
+ * four phases with grid barriers between them, and marks each request done
+ * in the mailbox.  latency_s[r] = completion - arrival; batch_of[r] (or
  * NULL) the step that served it. */
 int sfx_ensemble_serve(sfx_ensemble* e, int mode, int64_t nreq,
                        const double* arrival_s, const int32_t* mesh_id,

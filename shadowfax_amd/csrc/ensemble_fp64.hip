@@ -99,6 +99,12 @@ hipError_t ens_launch_range<double>(int ndim, int kind, const EnsArgs<double>& a
 }
 
 template <>
+hipError_t ens_launch_queue<double>(int ndim, int kind, const EnsArgs<double>& a, const QueueArgs& qa,
+                                int ncu, hipStream_t s, int* blocks) {
+    return ens_queue_impl<double, 0>(ndim, kind, a, qa, ncu, s, blocks);
+}
+
+template <>
 hipError_t ens_launch_entries<double>(int ndim, const flux::MeshView<double>& m, int64_t E,
                                       double* egeo, hipStream_t s) {
     return launch_entry_geometry<double>(ndim, m, E, egeo, s);

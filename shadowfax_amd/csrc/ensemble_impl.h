@@ -264,6 +264,26 @@ hipError_t ens_launch_range(int ndim, int kind, const EnsArgs<T>& a, int64_t cel
 template <typename T>
 hipError_t ens_launch_entries(int ndim, const flux::MeshView<T>& m, int64_t E, T* egeo,
                               hipStream_t s);
+// The request queue of the persistent serving kernel (ensemble_queue.h).
+namespace ens {
+struct DevQueue;
+struct HostBox;
+}  // namespace ens
+struct QueueArgs {
+    ens::DevQueue* q;          // device
+    ens::HostBox* box;         // the pinned mailbox, as the device sees it
+    const int32_t* box_mesh;   // [nreq] mesh of each request (mailbox)
+    int32_t* box_done;         // [nreq] batch number once done (mailbox)
+    const int64_t *mc0, *mnc, *mf0, *mnf;  // mesh table (device)
+    int64_t arena_cells, arena_faces;
+    int nmesh_words;           // 32-bit words of the leader's mesh bitset
+};
+// launches the persistent kernel cooperatively, every block resident (as
+// many per CU as occupancy allows); *blocks: the grid
+template <typename T>
+hipError_t ens_launch_queue(int ndim, int kind, const EnsArgs<T>& a, const QueueArgs& qa,
+                            int ncu, hipStream_t s, int* blocks);
+
 // ensemble_fp64.hip
 hipError_t ens_launch_plan(const int32_t* batch, const int64_t* mcell0, const int64_t* mncell,
                            const int64_t* mface0, const int64_t* mnface, int32_t tile,

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "ensemble_impl.h"
+#include "ensemble_queue.h"
 
 namespace sfx {
 
@@ -76,6 +77,32 @@ hipError_t ens_range_impl(int ndim, int kind, const EnsArgs<T>& a, int64_t cell0
         SFX_RNG(2);
 #undef SFX_RNG
     return hipGetLastError();
+}
+
+template <typename T, int MIXED>
+hipError_t ens_queue_impl(int ndim, int kind, const EnsArgs<T>& a, const QueueArgs& qa, int ncu,
+                          hipStream_t s, int* blocks) {
+    mixed::CF cf = mixed::make_cf(a.c);
+    const void* fn = nullptr;
+    if(ndim == 3)
+        fn = kind == 0 ? (const void*)ens::queue_kernel<3, 0, MIXED, T, double>
+                       : (const void*)ens::queue_kernel<3, 1, MIXED, T, double>;
+    else
+        fn = kind == 0 ? (const void*)ens::queue_kernel<2, 0, MIXED, T, double>
+                       : (const void*)ens::queue_kernel<2, 1, MIXED, T, double>;
+    const size_t shmem = 4 * (size_t)qa.nmesh_words;
+    int per_cu = 0;
+    hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, a.block, shmem);
+    if(err != hipSuccess) return err;
+    if(per_cu < 1) return hipErrorInvalidConfiguration;
+    *blocks = per_cu * ncu;
+    EnsArgs<T> aa = a;
+    QueueArgs q = qa;
+    void* args[] = {&aa,          &cf,        &q.q,           &q.box,         &q.box_mesh,
+                    &q.box_done,  &q.mc0,     &q.mnc,         &q.mf0,         &q.mnf,
+                    &q.arena_cells, &q.arena_faces, &q.nmesh_words};
+    return hipLaunchCooperativeKernel(fn, dim3((unsigned)*blocks), dim3((unsigned)a.block), args,
+                                      (unsigned)shmem, s);
 }
 
 }  // namespace sfx

@@ -18,6 +18,12 @@ hipError_t ens_launch_range<float>(int ndim, int kind, const EnsArgs<float>& a, 
 }
 
 template <>
+hipError_t ens_launch_queue<float>(int ndim, int kind, const EnsArgs<float>& a, const QueueArgs& qa,
+                                int ncu, hipStream_t s, int* blocks) {
+    return ens_queue_impl<float, 1>(ndim, kind, a, qa, ncu, s, blocks);
+}
+
+template <>
 hipError_t ens_launch_entries<float>(int ndim, const flux::MeshView<float>& m, int64_t E,
                                      float* egeo, hipStream_t s) {
     return launch_entry_geometry<float>(ndim, m, E, egeo, s);

@@ -42,10 +42,10 @@
 #define SFX_GRAD_KB 2
 #endif
 #ifndef SFX_GRAD_KB_2D  // FP32-record gradients of 2D meshes (about 6 entries a cell)
-#define SFX_GRAD_KB_2D 8
+#define SFX_GRAD_KB_2D 4
 #endif
 #ifndef SFX_GRAD_KN_2D  // FP32-record gradients of 2D meshes: neighbour rows in flight
-#define SFX_GRAD_KN_2D 4
+#define SFX_GRAD_KN_2D 2
 #endif
 #ifndef SFX_REC_BUFFER  // grouped records through a bounded buffer descriptor
 #define SFX_REC_BUFFER 1
@@ -644,16 +644,17 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             H[u] = half(p >= 0 ? p : 0, 1);
         }
     };
-    // 2D: a cell's entries fit one chunk of 8 (the records' second halves
-    // are read again for the limiter); 3D: chunk after chunk, the next
-    // chunk's records in flight while this chunk's neighbour states load
+    // 2D: chunks of 4 records (a cell has about 6), each chunk's neighbour
+    // rows two at a time -- 124 VGPRs, so four waves per SIMD and the 2D
+    // benchmark mesh's 4,096 waves in one round (chunks of 8, rows four at a
+    // time: 152 VGPRs, three waves, 20.3 against 18.1 us); 3D: chunk after
+    // chunk, the next chunk's records in flight while this chunk's
+    // neighbour states load
     constexpr bool PIPE = ND == 3 && SFX_GRAD_PIPE;
     float4 lo[KB], hi[KB];
     if(n > 0) load_chunk(n, lo, hi);
     for(int64_t pb = n; pb > 0; pb -= KB) {
-        // the neighbour rows in batches of KN (all KB at once in 3D; two
-        // batches of 4 in 2D: fewer registers in flight, measured as fast as
-        // one batch of 8)
+        // the neighbour rows in batches of KN (all KB at once in 3D)
         constexpr int KN = (ND == 2) ? SFX_GRAD_KN_2D : KB;
         float4 nlo[PIPE ? KB : 1], nhi[PIPE ? KB : 1];
 #pragma unroll

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "ensemble_impl.h"
+#include "ensemble_queue.h"
 #include "sfx_capi_internal.h"
 #include "shadowfax_amd.h"
 
@@ -276,6 +277,106 @@ int run_plain(sfx_ensemble* e, int32_t id) {
                                                        cell0, ncell, face0, nface, gshift,
                                                        qshift, e->stream);
     if(err != hipSuccess) return hip_error(err, "plain step launch");
+    return SFX_OK;
+}
+
+// SFX_SERVE_QUEUE: post the trace's requests to the pinned mailbox at their
+// arrival times and watch their done flags while the persistent kernel
+// serves them.
+int serve_queue(sfx_ensemble* e, int64_t nreq, const double* arrival_s, const int32_t* mesh_id,
+                double* latency_s, int32_t* batch_of, int64_t* nbatch_out, double* wall_out) {
+    // mailbox: HostBox, mesh[nreq], done[nreq] (pinned, coherent)
+    const size_t off_mesh = 64, off_done = off_mesh + 4 * (size_t)nreq;
+    const size_t bytes = off_done + 4 * (size_t)nreq + 64;
+    char* hb = nullptr;
+    hipError_t err = hipHostMalloc((void**)&hb, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if(err != hipSuccess) return hip_error(err, "serve queue: mailbox");
+    std::memset(hb, 0, bytes);
+    auto* box = reinterpret_cast<ens::HostBox*>(hb);
+    auto* bmesh = reinterpret_cast<int32_t*>(hb + off_mesh);
+    auto* bdone = reinterpret_cast<int32_t*>(hb + off_done);
+    char* dbox = nullptr;
+    ens::DevQueue* dq = nullptr;
+    err = hipHostGetDevicePointer((void**)&dbox, hb, 0);
+    if(err == hipSuccess) err = hipMalloc(&dq, sizeof(ens::DevQueue));
+    if(err == hipSuccess) err = hipMemsetAsync(dq, 0, sizeof(ens::DevQueue), e->stream);
+    int blocks = 0;
+    if(err == hipSuccess) {
+        QueueArgs qa;
+        qa.q = dq;
+        qa.box = reinterpret_cast<ens::HostBox*>(dbox);
+        qa.box_mesh = reinterpret_cast<const int32_t*>(dbox + off_mesh);
+        qa.box_done = reinterpret_cast<int32_t*>(dbox + off_done);
+        qa.mc0 = e->d_c0;
+        qa.mnc = e->d_nc;
+        qa.mf0 = e->d_f0;
+        qa.mnf = e->d_nf;
+        qa.arena_cells = e->arena_cells;
+        qa.arena_faces = e->arena_faces;
+        qa.nmesh_words = (int)((e->nmesh + 31) / 32);
+        err = e->prec == SFX_PREC_FP64
+                      ? ens_launch_queue<double>(e->nd, e->h->kind, ens_args<double>(e), qa, e->ncu,
+                                                 e->stream, &blocks)
+                      : ens_launch_queue<float>(e->nd, e->h->kind, ens_args<float>(e), qa, e->ncu,
+                                                e->stream, &blocks);
+    }
+    if(err != hipSuccess) {
+        hipStreamSynchronize(e->stream);
+        if(dq) hipFree(dq);
+        hipHostFree(hb);
+        return hip_error(err, "serve queue: launch");
+    }
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    int64_t next = 0, first = 0, completed = 0, maxb = 0;
+    double last_progress = 0.0;
+    int rc = SFX_OK;
+    while(completed < nreq) {
+        const double tnow = now();
+        bool progress = false;
+        while(next < nreq && arrival_s[next] <= tnow) {
+            bmesh[next] = mesh_id[next];
+            __atomic_store_n(&box->tail, (uint64_t)(next + 1), __ATOMIC_RELEASE);
+            next++;
+            progress = true;
+        }
+        if(next == nreq) __atomic_store_n(&box->stop, 1, __ATOMIC_RELEASE);
+        // done flags of the posted, unfinished requests (they finish in
+        // batches, mostly in order)
+        for(int64_t r = first; r < next; r++) {
+            if(latency_s[r] >= 0.0) continue;
+            const int32_t d = __atomic_load_n(&bdone[r], __ATOMIC_ACQUIRE);
+            if(d == 0) continue;
+            latency_s[r] = now() - arrival_s[r];
+            if(batch_of) batch_of[r] = d - 1;
+            maxb = std::max<int64_t>(maxb, d);
+            completed++;
+            progress = true;
+        }
+        while(first < next && latency_s[first] >= 0.0) first++;
+        if(progress) {
+            last_progress = now();
+        } else if(now() - last_progress > 30.0) {
+            rc = set_error(SFX_ERR_HIP, "serve queue: no request completed for 30 s");
+            break;
+        }
+    }
+    const double wall = now();
+    __atomic_store_n(&box->stop, 1, __ATOMIC_RELEASE);
+    err = hipStreamSynchronize(e->stream);
+    int32_t qerr = 0;
+    uint64_t nb = 0;
+    if(err == hipSuccess) err = hipMemcpy(&qerr, &dq->error, sizeof qerr, hipMemcpyDeviceToHost);
+    if(err == hipSuccess) err = hipMemcpy(&nb, &dq->nbatch, sizeof nb, hipMemcpyDeviceToHost);
+    hipFree(dq);
+    hipHostFree(hb);
+    if(rc) return rc;
+    if(err != hipSuccess) return hip_error(err, "serve queue");
+    if(qerr) return set_error(SFX_ERR_HIP, "serve queue: a device wait ran out (%d)", qerr);
+    *nbatch_out = (int64_t)nb;
+    *wall_out = wall;
+    (void)blocks;
     return SFX_OK;
 }
 
@@ -686,7 +787,7 @@ int sfx_ensemble_serve(sfx_ensemble* e, int mode, int64_t nreq, const double* ar
                        sfx_serve_stats* st) {
     if(!e || nreq < 0 || (nreq > 0 && (!arrival_s || !mesh_id || !latency_s)))
         return set_error(SFX_ERR_ARG, "sfx_ensemble_serve: bad arguments");
-    if(mode != SFX_SERVE_BATCHED && mode != SFX_SERVE_PLAIN)
+    if(mode != SFX_SERVE_BATCHED && mode != SFX_SERVE_PLAIN && mode != SFX_SERVE_QUEUE)
         return set_error(SFX_ERR_ARG, "sfx_ensemble_serve: mode");
     for(int64_t r = 0; r < nreq; r++) {
         if(mesh_id[r] < 0 || mesh_id[r] >= e->nmesh)
@@ -701,6 +802,35 @@ int sfx_ensemble_serve(sfx_ensemble* e, int mode, int64_t nreq, const double* ar
     if(mode == SFX_SERVE_BATCHED && (rc = capture(e))) return rc;
     hipError_t err = hipStreamSynchronize(e->stream);
     if(err != hipSuccess) return hip_error(err, "sfx_ensemble_serve");
+    if(mode == SFX_SERVE_QUEUE) {
+        std::vector<int32_t> bat(nreq > 0 ? nreq : 1, -1);
+        for(int64_t r = 0; r < nreq; r++) latency_s[r] = -1.0;
+        int64_t nbatch = 0;
+        double wall = 0.0;
+        if(nreq > 0 && (rc = serve_queue(e, nreq, arrival_s, mesh_id, latency_s, bat.data(),
+                                         &nbatch, &wall)))
+            return rc;
+        if(batch_of) std::copy(bat.begin(), bat.begin() + nreq, batch_of);
+        if(st) {
+            std::vector<int64_t> per(nbatch > 0 ? nbatch : 1, 0);
+            for(int64_t r = 0; r < nreq; r++)
+                if(bat[r] >= 0 && bat[r] < nbatch) per[bat[r]]++;
+            std::vector<double> v(latency_s, latency_s + nreq);
+            double sum = 0.0;
+            for(double x : v) sum += x;
+            st->nreq = nreq;
+            st->nbatch = nbatch;
+            st->wall_s = wall;
+            st->busy_s = wall;
+            st->p50_s = pct(v, 0.50);
+            st->p99_s = pct(v, 0.99);
+            st->max_s = v.empty() ? 0.0 : *std::max_element(v.begin(), v.end());
+            st->mean_s = v.empty() ? 0.0 : sum / (double)v.size();
+            st->mean_batch = nbatch ? (double)nreq / (double)nbatch : 0.0;
+            st->max_batch = *std::max_element(per.begin(), per.end());
+        }
+        return SFX_OK;
+    }
     hipEvent_t done;
     if((err = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess)
         return hip_error(err, "sfx_ensemble_serve");

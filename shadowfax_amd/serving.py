@@ -22,7 +22,7 @@ _i32p = ct.POINTER(ct.c_int32)
 _i64p = ct.POINTER(ct.c_int64)
 
 ENS_W, ENS_Q, ENS_DQ = 0, 1, 2
-SERVE_BATCHED, SERVE_PLAIN = 0, 1
+SERVE_BATCHED, SERVE_PLAIN, SERVE_QUEUE = 0, 1, 2
 
 
 class ServeStats(ct.Structure):

@@ -113,6 +113,47 @@ def test_autotune_keeps_states(device):
     check_meshes(ens, bs, kind, prec, device, [1, 1, 1, 1])
 
 
+@pytest.mark.parametrize("block", [64, 256])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_device_queue_applies_every_request_once(device, prec, block):
+    """SFX_SERVE_QUEUE: the persistent kernel drains the posted requests,
+    admits them (a small arena, so batches split; a second request for a
+    mesh waits for the next batch), and steps them; every request once, each
+    mesh's steps in arrival order, states bit for bit the standalone steps;
+    then a trace arriving over 2 ms, and the queue mode again."""
+    kind = 1
+    bs = bundles(kind)
+    ens, ids = make_ensemble(kind, prec, device, bs, arena_cells=500)
+    ens.set_launch(block, 0)
+    mesh = np.array([0, 1, 0, 2, 3, 3, 1, 0, 2, 2, 1, 3, 0, 1, 2, 3, 0, 0, 1, 2, 3, 1, 1, 0],
+                    np.int32)
+    print("case %s block %d: burst" % (prec, block), flush=True)
+    lat, bat, st = ens.serve(np.zeros(len(mesh)), mesh, S.SERVE_QUEUE)
+    print("stats", st, flush=True)
+    assert st["nreq"] == len(mesh) and np.all(lat > 0)
+    assert st["max_batch"] >= 2 and st["nbatch"] < len(mesh)
+    for mid in range(4):
+        b = bat[mesh == mid]
+        assert np.all(np.diff(b) > 0)  # one step per batch, in arrival order
+    print("case %s block %d: spread arrivals" % (prec, block), flush=True)
+    lat2, bat2, st2 = ens.serve(np.linspace(0.0, 2e-3, len(mesh)), mesh, S.SERVE_QUEUE)
+    assert st2["nreq"] == len(mesh) and np.all(lat2 > 0)
+    counts = [2 * int((mesh == k).sum()) for k in range(4)]
+    if prec == "fp64":
+        check_meshes(ens, bs, kind, prec, device, counts)
+        return
+    # mixed: the persistent kernel inlines the same per-item code into one
+    # function, where the compiler may fuse a multiply-add the standalone
+    # kernels leave separate (the mixed code is compiled with contraction;
+    # the FP64 code without): equal to rounding, not bit for bit
+    for mid, k in enumerate(counts):
+        W, Q = standalone(bs[mid], kind, prec, device, k)
+        dW = np.abs(ens.read(mid, S.ENS_W) - W).max() / np.abs(W).max()
+        dQ = np.abs(ens.read(mid, S.ENS_Q) - Q).max() / np.abs(Q).max()
+        print("mesh %d after %d steps: W %.2e Q %.2e" % (mid, k, dW, dQ), flush=True)
+        assert dW <= 1e-12 and dQ <= 1e-12
+
+
 def workload_mesh(w, classes, i):
     """Mesh i of the serving workload as its own prepared bundle: its size
     class's tessellation with its own states, as build_ensemble copies it."""
