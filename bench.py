@@ -142,12 +142,14 @@ def load_profile(name):
         return None
 
 
-def measured_traffic(kernel, grid=None):
+def measured_traffic(kernel, items=None):
     """HBM bytes per launch of a kernel from the committed hardware-counter
-    summary (profiles/hbm_traffic.json), if it was measured at this grid."""
+    summary (profiles/hbm_traffic.json), if it was measured on this many
+    items (the counters record the launch grid: items rounded up to whole
+    blocks of 256)."""
     d = load_profile("hbm_traffic.json")
     k = d["kernels"].get(kernel) if d else None
-    if not k or (grid is not None and int(k.get("grid", -1)) != grid):
+    if not k or (items is not None and (int(k.get("grid", -1)) + 255) // 256 != (items + 255) // 256):
         return None
     return float(k["traffic_bytes"])
 
