@@ -510,7 +510,7 @@ def main():
     F = b3["F"]
     face_k = "sfx::flux::face_flux_kernel<3, 0, 1, float, double>"
     roof = roofline(b3, fp3, "3d", face_k,
-                    ["sfx::flux::gradient_kernel<3, float, double, true>", face_k,
+                    ["sfx::flux::gradient_lds_kernel<3>", face_k,
                      "sfx::flux::gather_kernel<3, 1, float>"])
     out = {"metric": METRIC, "value": F / (ms * 1e-3), "unit": "face-fluxes/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,

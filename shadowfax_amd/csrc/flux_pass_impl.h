@@ -56,11 +56,20 @@
 #ifndef SFX_GRAD_PIPE  // FP32-record gradients: next chunk's records in flight
 #define SFX_GRAD_PIPE 1
 #endif
+#ifndef SFX_GRAD_LDS_K  // 3D FP32-record gradients: limiter d vectors kept in LDS per lane
+#define SFX_GRAD_LDS_K 16
+#endif
+#ifndef SFX_GRAD_LDS_K_2D  // the same for 2D meshes; 0: off (8 measured 17.7 against 15.8 us on
+#define SFX_GRAD_LDS_K_2D 0  // 512x512: the first walk would load second halves it otherwise skips)
+#endif
 #ifndef SFX_GATHER_KB
 #define SFX_GATHER_KB 8
 #endif
 #ifndef SFX_GATHER_KB_F64  // FP64 rows: the same registers as 8 FP32 rows
 #define SFX_GATHER_KB_F64 4
+#endif
+#ifndef SFX_FACE_WAVES_2D  // mixed 2D face kernel: waves per SIMD it is compiled for (0: free)
+#define SFX_FACE_WAVES_2D 0
 #endif
 
 namespace sfx {
@@ -586,9 +595,16 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
 // a = Wmax - W, b = dp for dp > 0 and a = W - Wmin, b = -dp for dp < 0
 // (dp = 0 gives 1, which never lowers alpha <= 1); the smallest a/b is kept
 // as a pair, compared by cross products, and divided out once per component.
-template <int ND, bool GROUPED, bool ACT, bool BUF>
+// LK > 0: the block's LDS array dcache ([LK][ND][256] floats) keeps the d
+// vectors of the cell's positions 0 .. LK - 1 from the first walk, so the
+// limiter reads those from LDS instead of loading the records' second
+// halves again (zeros where the area is 0, which the limiter skips either
+// way: d = 0 gives dp = 0, or NaN against a non-finite gradient, and neither
+// changes alpha).
+template <int ND, bool GROUPED, bool ACT, bool BUF, int LK = 0>
 __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t i,
-                                              float* __restrict__ grad, int64_t gshift) {
+                                              float* __restrict__ grad, int64_t gshift,
+                                              float* __restrict__ dcache = nullptr) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, float>();
     // The loads that depend on nothing go out together, before any wait:
     // the activity flag (ACT: m.active is set), the record range, the cell's
@@ -657,6 +673,22 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         // the neighbour rows in batches of KN (all KB at once in 3D)
         constexpr int KN = (ND == 2) ? SFX_GRAD_KN_2D : KB;
         float4 nlo[PIPE ? KB : 1], nhi[PIPE ? KB : 1];
+        if constexpr(LK > 0) {
+            // the chunk's d vectors go to LDS as soon as its records are in,
+            // so their registers are free again before the neighbour rows
+#pragma unroll
+            for(int u = 0; u < KB; u++) {
+                const int64_t p = pb - 1 - u;
+                if(p >= 0 && p < LK) {
+                    // the limiter's area test: the second half's w (3D), z (2D)
+                    const bool z = !(((ND == 3) ? hi[u].w : hi[u].z) != 0.f);
+                    float* c = dcache + p * (ND * 256) + threadIdx.x;
+                    c[0] = z ? 0.f : hi[u].x;
+                    c[256] = z ? 0.f : hi[u].y;
+                    if(ND == 3) c[512] = z ? 0.f : hi[u].z;
+                }
+            }
+        }
 #pragma unroll
         for(int u0 = 0; u0 < KB; u0 += KN) {
             // every neighbour row is loaded unconditionally (a mirror
@@ -756,7 +788,10 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         // next chunk loading while this one is used, saves little here: the
         // work per chunk is short next to a load's latency)
         constexpr int KL = (ND == 2) ? 8 : SFX_GRAD_KL_3D;
-        for(int64_t pb = n; pb > 0; pb -= KL) {
+        // positions nc .. n - 1 from the records, 0 .. nc - 1 (LK > 0) from
+        // LDS, in the same descending order
+        const int64_t nc = (LK > 0) ? (n < LK ? n : (int64_t)LK) : 0;
+        for(int64_t pb = n; pb > nc; pb -= KL) {
             float4 h[KL];
 #pragma unroll
             for(int u = 0; u < KL; u++) {
@@ -764,7 +799,20 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
                 h[u] = half(p >= 0 ? p : 0, 1);
             }
 #pragma unroll
-            for(int u = 0; u < KL; u++) limit(h[u], u < pb);
+            for(int u = 0; u < KL; u++) limit(h[u], pb - 1 - u >= nc);
+        }
+        if constexpr(LK > 0) {
+            for(int64_t pb = nc; pb > 0; pb -= 4) {
+                float4 h[4];
+#pragma unroll
+                for(int u = 0; u < 4; u++) {
+                    const int64_t p = pb - 1 - u;
+                    const float* c = dcache + (p >= 0 ? p : 0) * (ND * 256) + threadIdx.x;
+                    h[u] = make_float4(c[0], c[256], (ND == 3) ? c[512] : 1.f, 1.f);
+                }
+#pragma unroll
+                for(int u = 0; u < 4; u++) limit(h[u], u < pb);
+            }
         }
     }
     if(!act) return;
@@ -967,6 +1015,35 @@ template <int ND, typename T, typename R, bool DIFF>
 __global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restrict__ grad) {
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
     if(i < m.C) grad_cell<ND, T, R, DIFF>(m, i, grad, 0);
+}
+
+// LDS positions per lane and waves per SIMD of gradient_lds_kernel: 3D
+// keeps 16 positions (48 KB a block) at 3 waves per SIMD (at most 168
+// VGPRs), as gradient_kernel runs; 2D (when SFX_GRAD_LDS_K_2D is set) at 4
+// waves per SIMD (at most 128 VGPRs).
+template <int ND>
+constexpr int grad_lds_k() {
+    return ND == 3 ? SFX_GRAD_LDS_K : SFX_GRAD_LDS_K_2D;
+}
+template <int ND>
+constexpr int grad_lds_waves() {
+    return ND == 3 ? 3 : 4;
+}
+
+// FP32 records, grouped and read through the buffer descriptor (the host
+// launches this only then, in blocks of 256): the limiter's d vectors of
+// each cell's first grad_lds_k positions stay in LDS.
+template <int ND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_lds_waves<ND>()))) void
+gradient_lds_kernel(MeshView<float> m, float* __restrict__ grad) {
+    constexpr int LK = grad_lds_k<ND>() > 0 ? grad_lds_k<ND>() : 1;
+    __shared__ float dcache[LK * ND * 256];
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if(i >= m.C) return;
+    if(m.active)
+        grad_cell_f32<ND, true, true, true, LK>(m, i, grad, 0, dcache);
+    else
+        grad_cell_f32<ND, true, false, true, LK>(m, i, grad, 0, dcache);
 }
 
 // ------------------------------------------------------- K3 face fluxes --
@@ -1259,8 +1336,14 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
     cnt[CNT_NFAIL] += fail;
 }
 
+template <int ND, int MIXED>
+constexpr int face_waves() {
+    return (ND == 2 && MIXED && SFX_FACE_WAVES_2D > 0) ? SFX_FACE_WAVES_2D : 1;
+}
+
 template <int ND, int KIND, int MIXED, typename T, typename R>
-__global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(face_waves<ND, MIXED>()))) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
                                                         Consts cd, mixed::CF cf,
                                                         T* __restrict__ dQL,
                                                         uint8_t* __restrict__ fstatus,

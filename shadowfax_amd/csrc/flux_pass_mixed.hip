@@ -48,7 +48,16 @@ template <>
 hipError_t launch_gradients<float>(int ndim, const flux::MeshView<float>& m, float* grad,
                                    hipStream_t s) {
     if(m.C <= 0) return hipSuccess;
-    if(ndim == 3)
+    // the conditions under which gradient_kernel reads grouped records
+    // through the buffer descriptor (rec_buffer)
+    const bool buf = SFX_REC_BUFFER && m.egrp && m.egeo_bytes > 0 && m.egeo_bytes <= 0x7FFFFFF0ull;
+    if(ndim == 3 && buf && SFX_GRAD_LDS_K > 0)
+        hipLaunchKernelGGL((flux::gradient_lds_kernel<3>), grid_for(m.C, 256), dim3(256), 0, s, m,
+                           grad);
+    else if(ndim == 2 && buf && SFX_GRAD_LDS_K_2D > 0)
+        hipLaunchKernelGGL((flux::gradient_lds_kernel<2>), grid_for(m.C, 256), dim3(256), 0, s, m,
+                           grad);
+    else if(ndim == 3)
         hipLaunchKernelGGL((flux::gradient_kernel<3, float, double, true>), grid_for(m.C, 256),
                            dim3(256), 0, s, m, grad);
     else
