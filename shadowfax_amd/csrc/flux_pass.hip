@@ -15,11 +15,12 @@ hipError_t launch_entry_geometry<double>(int ndim, const flux::MeshView<double>&
         // grouped layout: a block per group of 64 cells (the last one padded)
         if(m.C <= 0) return hipSuccess;
         const dim3 g((unsigned)((m.C + 63) / 64));  // one block per group
+        const unsigned egeo_lds = flux::occupancy_lds_bytes(SFX_EGEO_WAVES, m.C, SFX_GATHER_BIG, 1024);
         if(ndim == 3)
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, double>), g, dim3(256), 0,
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, double>), g, dim3(256), egeo_lds,
                                s, m, egeo);
         else
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, double>), g, dim3(256), 0,
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, double>), g, dim3(256), egeo_lds,
                                s, m, egeo);
         return hipGetLastError();
     }
@@ -68,12 +69,13 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
                               const int32_t* gent, const double* dQL, double* dQ,
                               hipStream_t s) {
     if(C <= 0) return hipSuccess;
+    const unsigned lds = flux::gather_lds_bytes<double>(C);
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 0, double>), grid_for(C, 256), dim3(256), 0, s, C, F,
-                           goff, gent, nullptr, nullptr, dQL, dQ);
+        hipLaunchKernelGGL((flux::gather_kernel<3, 0, double>), grid_for(C, 256), dim3(256), lds, s,
+                           C, F, goff, gent, nullptr, nullptr, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 0, double>), grid_for(C, 256), dim3(256), 0, s, C, F,
-                           goff, gent, nullptr, nullptr, dQL, dQ);
+        hipLaunchKernelGGL((flux::gather_kernel<2, 0, double>), grid_for(C, 256), dim3(256), lds, s,
+                           C, F, goff, gent, nullptr, nullptr, dQL, dQ);
     return hipGetLastError();
 }
 

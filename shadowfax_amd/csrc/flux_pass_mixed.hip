@@ -26,12 +26,13 @@ hipError_t launch_entry_geometry<float>(int ndim, const flux::MeshView<float>& m
         // grouped layout: a block per group of 64 cells (the last one padded)
         if(m.C <= 0) return hipSuccess;
         const dim3 g((unsigned)((m.C + 63) / 64));  // one block per group
+        const unsigned egeo_lds = flux::occupancy_lds_bytes(SFX_EGEO_WAVES, m.C, SFX_GATHER_BIG, 1024);
         if(ndim == 3)
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, float>), g, dim3(256), 0, s, m,
-                               egeo);
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<3, float>), g, dim3(256), egeo_lds,
+                               s, m, egeo);
         else
-            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, float>), g, dim3(256), 0, s, m,
-                               egeo);
+            hipLaunchKernelGGL((flux::entry_geometry_grouped_kernel<2, float>), g, dim3(256), egeo_lds,
+                               s, m, egeo);
         return hipGetLastError();
     }
     if(E <= 0) return hipSuccess;
@@ -87,12 +88,13 @@ hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* go
                                const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
                                const float* dQL, double* dQ, hipStream_t s) {
     if(C <= 0) return hipSuccess;
+    const unsigned lds = flux::gather_lds_bytes<float>(C);
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 1, float>), grid_for(C, 256), dim3(256), 0, s, C, F,
-                           goff, gent, ghost, twin, dQL, dQ);
+        hipLaunchKernelGGL((flux::gather_kernel<3, 1, float>), grid_for(C, 256), dim3(256), lds, s,
+                           C, F, goff, gent, ghost, twin, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 1, float>), grid_for(C, 256), dim3(256), 0, s, C, F,
-                           goff, gent, ghost, twin, dQL, dQ);
+        hipLaunchKernelGGL((flux::gather_kernel<2, 1, float>), grid_for(C, 256), dim3(256), lds, s,
+                           C, F, goff, gent, ghost, twin, dQL, dQ);
     return hipGetLastError();
 }
 
