@@ -363,6 +363,12 @@ typedef struct {
                                   (sfx_cell_closure); NULL when the cells'
                                   faces close, as those of a Voronoi
                                   tessellation do */
+    const void* frec;          /* float4 [sfx_mesh_face_record_planes(...)][F]
+                                  or NULL (mixed pass only): the faces' own
+                                  values packed by sfx_mesh_face_records,
+                                  16-byte aligned; the face kernel then reads
+                                  them instead of left, right, area, dL, dR,
+                                  rRL, mc and rcent (which stay required) */
 } sfx_mesh;
 
 /* Per-entry face records for the gradient pass, built once per
@@ -384,6 +390,23 @@ int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* e
 /* Elements of type T per entry record: 8 / 12 (FP64, 2D / 3D), 8 (mixed);
  * -1 for other arguments. */
 int sfx_mesh_entry_record(int ndim, int precision);
+
+/* Per-face records of the mixed face pass (no counterpart in the reference:
+ * a layout of the per-face values that VorFace::calculate_flux reads,
+ * src/VorFace.cpp:670-760), built once per tessellation from the FP32 face
+ * arrays of m: plane q of face f at float4 index q F + f,
+ *   3D: [left, right, area, rcent] [rRL (3), mc.x] [mc.y, mc.z, dL.x, dL.y]
+ *       [dL.z, dR (3)]
+ *   2D: [left, right, area, rcent] [rRL (2), mc (2)] [dL (2), dR (2)]
+ * with left and right as their int32 bits.  The face kernel then loads a
+ * face with 3 or 4 16-byte reads instead of 13 or 17 4-byte ones; the
+ * fluxes are bit for bit the same.  frec: float4 [planes][F], 16-byte
+ * aligned.  Mixed-precision handles only (SFX_ERR_ARG otherwise). */
+int sfx_mesh_face_records(sfx_solver* h, const sfx_mesh* m, void* frec, void* stream);
+
+/* float4 planes per face record: 4 (3D) or 3 (2D) for the mixed precision,
+ * -1 for other arguments. */
+int sfx_mesh_face_record_planes(int ndim, int precision);
 
 /* The grouped layout of the per-entry records (host): cells in groups of
  * 64 consecutive cells; group g gets record rows egeo_group[g] ..

@@ -40,7 +40,7 @@ class MeshDesc(ct.Structure):
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
                 ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
                 ("gather_entry", vp), ("egeo_group", vp), ("egeo_rows", i64),
-                ("closure", vp)]
+                ("closure", vp), ("frec", vp)]
 
 
 _lib = None
@@ -114,6 +114,8 @@ def _mesh_sigs():
     return {
         "sfx_mesh_entry_geometry": [vp, md, i64, vp, vp],
         "sfx_mesh_entry_record": [ct.c_int, ct.c_int],
+        "sfx_mesh_face_records": [vp, md, vp, vp],
+        "sfx_mesh_face_record_planes": [ct.c_int, ct.c_int],
         "sfx_mesh_entry_groups": [i64, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)],
         "sfx_gradients": [vp, md, vp, vp],
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],

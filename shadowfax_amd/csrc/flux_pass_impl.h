@@ -80,9 +80,6 @@
 #ifndef SFX_GATHER_KB_F64  // FP64 rows: the same registers as 8 FP32 rows
 #define SFX_GATHER_KB_F64 4
 #endif
-#ifndef SFX_FACE_WAVES_2D  // mixed 2D face kernel: waves per SIMD it is compiled for (0: free)
-#define SFX_FACE_WAVES_2D 0
-#endif
 
 namespace sfx {
 namespace flux {
@@ -125,6 +122,9 @@ struct MeshView {
     // [ND][C] every cell's closure residual sum A r_ij/|r_ij| (mixed
     // gradients), or null where the cells' faces close
     const T* closure = nullptr;
+    // mixed pass: the faces' own values as 3 (2D) or 4 (3D) planes of F
+    // float4 (face_record_kernel), or null: read from the arrays above
+    const float4* frec = nullptr;
 };
 
 // Per-entry record of a cell's face, in the cell's walk order.  FP64
@@ -421,6 +421,47 @@ __device__ __forceinline__ void predict_side_offset(float gamma, const double* W
     const bool keep = !(Wp[0] < 0.0 || Wp[P] < 0.0);
 #pragma unroll
     for(int k = 0; k < NC; k++) out[k] = keep ? Wp[k] : Wraw[k] + (double)off[k];
+}
+
+// ------------------------------------------------------- face records --
+
+// Planes of F float4 holding, for face f, the values the mixed face kernel
+// reads of the face itself, so that it loads them with 3 (2D) or 4 (3D)
+// 16-byte loads instead of 13 (2D) or 17 (3D) 4-byte ones (the kernel's
+// address unit, busy for most of its time, handles a wave's load in about
+// the same number of cycles whatever its width):
+//   3D: [left, right, area, rcent] [rRL (3), mc.x] [mc.y, mc.z, dL.x, dL.y]
+//       [dL.z, dR (3)]
+//   2D: [left, right, area, rcent] [rRL (2), mc (2)] [dL (2), dR (2)]
+// (indices as their bits).  Built once per tessellation, as the entry
+// records are; the values are the FP32 arrays' own, so the face fluxes are
+// bit for bit those of the array reads.
+template <int ND>
+constexpr int face_record_planes() {
+    return ND == 3 ? 4 : 3;
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void face_record_kernel(MeshView<float> m,
+                                                          float4* __restrict__ frec) {
+    const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t F = m.F;
+    if(f >= F) return;
+    float v[4 * face_record_planes<ND>()];
+    v[0] = __int_as_float(m.left[f]);
+    v[1] = __int_as_float(m.right[f]);
+    v[2] = m.area[f];
+    v[3] = m.rcent[f];
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        v[4 + l] = m.rRL[l * F + f];
+        v[4 + ND + l] = m.mc[l * F + f];
+        v[4 + 2 * ND + l] = m.dL[l * F + f];
+        v[4 + 3 * ND + l] = m.dR[l * F + f];
+    }
+#pragma unroll
+    for(int q = 0; q < face_record_planes<ND>(); q++)
+        frec[q * F + f] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
 // ------------------------------------------------------------- K2 grads --
@@ -1080,7 +1121,7 @@ static __device__ const uint8_t kAllActive = 1;
 // Face f: its flux goes to dQL[(f + qshift) * NC ...]; the gradients of its
 // cells are read at grad[(cell + gshift) * ND * NC ...]; the solver counters
 // are added to cnt.
-template <int ND, int KIND, int MIXED, typename T, typename R>
+template <int ND, int KIND, int MIXED, typename T, typename R, bool FREC = false>
 __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
                                           const T* __restrict__ grad, int64_t gshift,
                                           const Consts& cd, const mixed::CF& cf,
@@ -1097,19 +1138,52 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
         // its left cell's in their place, unused).  A branch between loads
         // (on activity, on a missing right cell) would make each later load
         // wait for the earlier ones.
-        const int32_t L = m.left[f], Rc = m.right[f];
+        int32_t L, Rc;
+        R A, rr[ND], mcv[ND], dLf[ND], dRf[ND];
+        T rcent_f;
+        if constexpr(FREC) {
+            // the face's record: 3 (2D) or 4 (3D) float4 planes (face_record_kernel)
+            const float4 p0 = m.frec[f], p1 = m.frec[F + f], p2 = m.frec[2 * F + f];
+            L = __float_as_int(p0.x);
+            Rc = __float_as_int(p0.y);
+            A = (R)p0.z;
+            rcent_f = p0.w;
+            if constexpr(ND == 3) {
+                const float4 p3 = m.frec[3 * F + f];
+                const float v[12] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y,
+                                     p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+#pragma unroll
+                for(int l = 0; l < ND; l++) {
+                    rr[l] = (R)v[l];
+                    mcv[l] = (R)v[3 + l];
+                    dLf[l] = (R)v[6 + l];
+                    dRf[l] = (R)v[9 + l];
+                }
+            } else {
+                const float v[8] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
+#pragma unroll
+                for(int l = 0; l < ND; l++) {
+                    rr[l] = (R)v[l];
+                    mcv[l] = (R)v[2 + l];
+                    dLf[l] = (R)v[4 + l];
+                    dRf[l] = (R)v[6 + l];
+                }
+            }
+        } else {
+            L = m.left[f];
+            Rc = m.right[f];
+            A = (R)m.area[f];
+#pragma unroll
+            for(int l = 0; l < ND; l++) {
+                rr[l] = (R)m.rRL[l * F + f];
+                mcv[l] = (R)m.mc[l * F + f];
+                dLf[l] = (R)m.dL[l * F + f];
+                dRf[l] = (R)m.dR[l * F + f];
+            }
+            rcent_f = m.rcent[f];
+        }
         const bool has_r = Rc >= 0;
         const int64_t Rs = has_r ? (int64_t)Rc : (int64_t)L;
-        const R A = (R)m.area[f];
-        R rr[ND], mcv[ND], dLf[ND], dRf[ND];
-#pragma unroll
-        for(int l = 0; l < ND; l++) {
-            rr[l] = (R)m.rRL[l * F + f];
-            mcv[l] = (R)m.mc[l * F + f];
-            dLf[l] = (R)m.dL[l * F + f];
-            dRf[l] = (R)m.dR[l * F + f];
-        }
-        const T rcent_f = m.rcent[f];
         // twin >= 0 marks a periodic copy (-1 on every other face); the left
         // indices stand in for a missing twin array (never used)
         const int32_t twin_f = (m.twin ? m.twin : m.left)[f];
@@ -1348,21 +1422,16 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
     cnt[CNT_NFAIL] += fail;
 }
 
-template <int ND, int MIXED>
-constexpr int face_waves() {
-    return (ND == 2 && MIXED && SFX_FACE_WAVES_2D > 0) ? SFX_FACE_WAVES_2D : 1;
-}
-
-template <int ND, int KIND, int MIXED, typename T, typename R>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(face_waves<ND, MIXED>()))) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
+template <int ND, int KIND, int MIXED, typename T, typename R, bool FREC = false>
+__global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* __restrict__ grad,
                                                         Consts cd, mixed::CF cf,
                                                         T* __restrict__ dQL,
                                                         uint8_t* __restrict__ fstatus,
                                                         unsigned long long* __restrict__ ctr) {
     const int64_t f = xcd_block() * blockDim.x + threadIdx.x;
     unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
-    if(f < m.F) face_item<ND, KIND, MIXED, T, R>(m, f, grad, 0, cd, cf, dQL, 0, fstatus, cnt);
+    if(f < m.F)
+        face_item<ND, KIND, MIXED, T, R, FREC>(m, f, grad, 0, cd, cf, dQL, 0, fstatus, cnt);
     block_add_counters(ctr, cnt);
 }
 

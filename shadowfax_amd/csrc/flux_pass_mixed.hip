@@ -73,14 +73,36 @@ hipError_t launch_face_fluxes<float>(int ndim, int kind, const flux::MeshView<fl
                                      uint8_t* fstatus, unsigned long long* ctr, hipStream_t s) {
     if(m.F <= 0) return hipSuccess;
     const mixed::CF cf = mixed::make_cf(c);
-#define SFX_FACE(ND, K)                                                                   \
-    hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, double>), grid_for(m.F, 256), \
-                       dim3(256), 0, s, m, grad, c, cf, dQL, fstatus, ctr)
+    // with the per-face records, the variant that reads them
+#define SFX_FACE(ND, K)                                                                          \
+    do {                                                                                         \
+        if(m.frec)                                                                               \
+            hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, double, true>),          \
+                               grid_for(m.F, 256), dim3(256), 0, s, m, grad, c, cf, dQL, fstatus, \
+                               ctr);                                                             \
+        else                                                                                     \
+            hipLaunchKernelGGL((flux::face_flux_kernel<ND, K, 1, float, double>),                \
+                               grid_for(m.F, 256), dim3(256), 0, s, m, grad, c, cf, dQL, fstatus, \
+                               ctr);                                                             \
+    } while(0)
     if(ndim == 3 && kind == 0) SFX_FACE(3, 0);
     else if(ndim == 3) SFX_FACE(3, 1);
     else if(kind == 0) SFX_FACE(2, 0);
     else SFX_FACE(2, 1);
 #undef SFX_FACE
+    return hipGetLastError();
+}
+
+hipError_t launch_face_records(int ndim, const flux::MeshView<float>& m, float* frec,
+                               hipStream_t s) {
+    if(m.F <= 0) return hipSuccess;
+    float4* out = reinterpret_cast<float4*>(frec);
+    if(ndim == 3)
+        hipLaunchKernelGGL((flux::face_record_kernel<3>), grid_for(m.F, 256), dim3(256), 0, s, m,
+                           out);
+    else
+        hipLaunchKernelGGL((flux::face_record_kernel<2>), grid_for(m.F, 256), dim3(256), 0, s, m,
+                           out);
     return hipGetLastError();
 }
 

@@ -78,6 +78,10 @@ hipError_t launch_update(int ndim, const Consts& c, int64_t C, double* Q, double
 hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* goff,
                               const int32_t* gent, const double* dQL, double* dQ,
                               hipStream_t s);
+// flux_pass_mixed.hip: the mixed face kernel's per-face records (frec: 3 or
+// 4 planes of m.F float4, see flux::face_record_kernel)
+hipError_t launch_face_records(int ndim, const flux::MeshView<float>& m, float* frec,
+                               hipStream_t s);
 hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* goff,
                                const int32_t* gent, const uint8_t* ghost, const int32_t* twin,
                                const float* dQL, double* dQ, hipStream_t s);

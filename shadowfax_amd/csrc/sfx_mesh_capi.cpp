@@ -63,6 +63,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.gather_off = m->gather_off;
     v.gather_entry = m->gather_entry;
     v.closure = (const T*)m->closure;
+    if constexpr(sizeof(T) == 4) v.frec = (const float4*)m->frec;
     return v;
 }
 
@@ -79,6 +80,8 @@ int check_mesh(const sfx_solver* h, const sfx_mesh* m, const char* who, bool fac
         return set_error(SFX_ERR_ARG, "%s: W must be 16-byte aligned", who);
     if(m->egeo && ((uintptr_t)m->egeo & 15))
         return set_error(SFX_ERR_ARG, "%s: egeo must be 16-byte aligned", who);
+    if(m->frec && ((uintptr_t)m->frec & 15))
+        return set_error(SFX_ERR_ARG, "%s: frec must be 16-byte aligned", who);
     if(m->egeo_group && !m->cf_off)
         return set_error(SFX_ERR_ARG, "%s: egeo_group needs cf_off", who);
     if(m->egeo_rows < 0) return set_error(SFX_ERR_ARG, "%s: egeo_rows < 0", who);
@@ -299,6 +302,29 @@ int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* e
                            : launch_entry_geometry<float>(h->ndim, view<float>(m), E,
                                                           (float*)egeo, (hipStream_t)stream);
     if(e != hipSuccess) return hip_error(e, "entry geometry kernel launch");
+    return SFX_OK;
+}
+
+int sfx_mesh_face_record_planes(int ndim, int precision) {
+    if((ndim != 2 && ndim != 3) || precision != SFX_PREC_MIXED) return -1;
+    return ndim == 3 ? sfx::flux::face_record_planes<3>() : sfx::flux::face_record_planes<2>();
+}
+
+int sfx_mesh_face_records(sfx_solver* h, const sfx_mesh* m, void* frec, void* stream) {
+    if(!h || !m) return set_error(SFX_ERR_ARG, "sfx_mesh_face_records: NULL handle or mesh");
+    if(h->precision != SFX_PREC_MIXED)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_face_records: mixed-precision handles only");
+    int rc = check_mesh(h, m, "sfx_mesh_face_records", false, false, false);
+    if(rc) return rc;
+    if(m->F == 0) return SFX_OK;
+    if(!m->left || !m->right || !m->area || !m->rcent || !m->rRL || !m->mc || !m->dL ||
+       !m->dR || !frec)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_face_records: NULL array");
+    if((uintptr_t)frec & 15)
+        return set_error(SFX_ERR_ARG, "sfx_mesh_face_records: frec must be 16-byte aligned");
+    if((rc = bind_device(h))) return rc;
+    hipError_t e = launch_face_records(h->ndim, view<float>(m), (float*)frec, (hipStream_t)stream);
+    if(e != hipSuccess) return hip_error(e, "face record kernel launch");
     return SFX_OK;
 }
 

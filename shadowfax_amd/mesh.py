@@ -323,9 +323,11 @@ class DeviceMesh:
     *_soa() accessors return host copies in the component-major layout of
     the prepared bundle (and of the oracle)."""
 
-    def __init__(self, b, storage="fp64", device=None, grouped=None):
+    def __init__(self, b, storage="fp64", device=None, grouped=None, face_records=None):
         """grouped (default on): the gradient records in the grouped layout
-        of sfx_mesh_entry_groups."""
+        of sfx_mesh_entry_groups.  face_records (default on, mixed storage
+        only): the face kernel reads each face's own values from the packed
+        records of sfx_mesh_face_records, built with the entry records."""
         import torch
         self.torch = torch
         self.dev = device or torch.device("cuda:0")
@@ -378,6 +380,11 @@ class DeviceMesh:
         self.fstatus = torch.zeros(self.F, dtype=torch.uint8, device=self.dev)
         self.cstatus = torch.zeros(self.C, dtype=torch.uint8, device=self.dev)
         self.use_twin = storage == "mixed"
+        self.face_records = storage == "mixed" and (face_records is None or bool(face_records))
+        self.frec = None
+        if self.face_records:
+            planes = Cp.lib().sfx_mesh_face_record_planes(nd, Cp.SFX_PREC_MIXED)
+            self.frec = torch.zeros((planes, self.F, 4), dtype=torch.float32, device=self.dev)
         # the mixed gradients' closure correction, for face lists whose cells
         # do not close (None: they do)
         if storage == "mixed" and b.get("closure") is not None:
@@ -401,6 +408,7 @@ class DeviceMesh:
         d.egeo_rows = self.egeo_rows if self.grouped else 0
         d.twin = p("twin") if self.use_twin else None
         d.closure = p("closure") if "closure" in a else None
+        d.frec = None  # set once the records are built
         return d
 
     def build_entries(self, solver_handle, stream=None):
@@ -408,6 +416,11 @@ class DeviceMesh:
         Cp.check(Cp.lib().sfx_mesh_entry_geometry(solver_handle, ct.byref(self.desc), self.E,
                                                   ct.c_void_p(self.egeo.data_ptr()), stream),
                  "sfx_mesh_entry_geometry")
+        if self.frec is not None:
+            Cp.check(Cp.lib().sfx_mesh_face_records(solver_handle, ct.byref(self.desc),
+                                                    ct.c_void_p(self.frec.data_ptr()), stream),
+                     "sfx_mesh_face_records")
+            self.desc.frec = ct.c_void_p(self.frec.data_ptr())
         self.egeo_built = True
 
     # host copies in the component-major layout of the bundle / oracle

@@ -99,3 +99,12 @@ def test_entry_record_sizes():
     assert [L.sfx_mesh_entry_record(nd, mixed) for nd in (2, 3)] == [8, 8]
     assert L.sfx_mesh_entry_record(1, f64) == -1
     assert L.sfx_mesh_entry_record(3, 7) == -1
+
+
+def test_face_record_planes():
+    """float4 planes per face record of the mixed face pass: 3 in 2D, 4 in
+    3D (13 and 17 values); no FP64 records."""
+    L = C.lib()
+    assert [L.sfx_mesh_face_record_planes(nd, C.SFX_PREC_MIXED) for nd in (2, 3)] == [3, 4]
+    assert L.sfx_mesh_face_record_planes(3, C.SFX_PREC_FP64) == -1
+    assert L.sfx_mesh_face_record_planes(4, C.SFX_PREC_MIXED) == -1

@@ -270,6 +270,33 @@ def test_grouped_records_bit_identical(device, mesh, kind, prec):
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("mesh", ["per12", "mir10", "p3d", "m3d", "poisson2d", "sedov3d"])
+def test_face_records_bit_identical(device, mesh, kind):
+    """The mixed face kernel reading each face's own values from the packed
+    records of sfx_mesh_face_records gives the face fluxes, status bits and
+    dQ of the kernel reading the face arrays, bit for bit (periodic and
+    mirror faces, 2D and 3D)."""
+    import torch
+    print("case %s kind %d" % (mesh, kind), flush=True)
+    if mesh in ("per12", "mir10", "p3d", "m3d"):
+        b = golden_bundle(mesh, kind)
+    else:
+        b = poisson2d_bundle(600, kind) if mesh == "poisson2d" else sedov_bundle(1500, kind)
+    res = []
+    for frec in (False, True):
+        s = RiemannSolver(kind, b["ndim"], G, 1e-8, -5.0, "mixed")
+        dm = M.DeviceMesh(b, "mixed", device, face_records=frec)
+        fp = FluxPass(s, dm)
+        assert (dm.desc.frec is not None) == frec
+        fp.run()
+        torch.cuda.synchronize()
+        res.append({"dQL": fp.m.dQL_soa(), "dQ": fp.m.dQ_soa(),
+                    "st": fp.m.fstatus.cpu().numpy()})
+    for k in ("dQL", "dQ", "st"):
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
 @pytest.mark.parametrize("shape", [(2, 2), (2, 3), (2, 5), (3, 20), (3, 40)])
 @pytest.mark.parametrize("kind", [0, 1])
 def test_tiny_meshes(device, shape, kind):
