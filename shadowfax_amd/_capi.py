@@ -40,7 +40,7 @@ class MeshDesc(ct.Structure):
                 ("area", vp), ("dL", vp), ("dR", vp), ("rRL", vp), ("mc", vp), ("rcent", vp),
                 ("cf_off", vp), ("cf_entry", vp), ("egeo", vp), ("gather_off", vp),
                 ("gather_entry", vp), ("egeo_group", vp), ("egeo_rows", i64),
-                ("closure", vp), ("frec", vp)]
+                ("egeo_slot", vp), ("closure", vp), ("frec", vp)]
 
 
 _lib = None
@@ -96,6 +96,7 @@ def lib():
         if hasattr(L, name):
             getattr(L, name).argtypes = args
     L.sfx_mesh_entry_groups.restype = ct.c_int64
+    L.sfx_mesh_entry_groups_sorted.restype = ct.c_int64
     _lib = L
     return L
 
@@ -117,6 +118,8 @@ def _mesh_sigs():
         "sfx_mesh_face_records": [vp, md, vp, vp],
         "sfx_mesh_face_record_planes": [ct.c_int, ct.c_int],
         "sfx_mesh_entry_groups": [i64, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)],
+        "sfx_mesh_entry_groups_sorted": [i64, ct.POINTER(ct.c_int64), i64,
+                                         ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)],
         "sfx_gradients": [vp, md, vp, vp],
         "sfx_face_fluxes": [vp, md, vp, vp, vp, vp],
         "sfx_gather_dQ": [vp, md, vp, vp, vp],

@@ -62,6 +62,9 @@
 #ifndef SFX_GRAD_LDS_K_2D  // the same for 2D meshes; 0: off (8 measured 17.7 against 15.8 us on
 #define SFX_GRAD_LDS_K_2D 0  // 512x512: the first walk would load second halves it otherwise skips)
 #endif
+#ifndef SFX_ROW_LOADS3  // FP64 rows of 5 values in 3 loads (else 4, all 16-byte aligned)
+#define SFX_ROW_LOADS3 1
+#endif
 #ifndef SFX_GATHER_KB
 #define SFX_GATHER_KB 8
 #endif
@@ -119,6 +122,9 @@ struct MeshView {
     // bytes of the grouped records, 0 if not known: when known and under
     // 4 GB they are read through a buffer descriptor with that bound
     uint64_t egeo_bytes = 0;
+    // grouped records: slot s (group s >> 6, lane s & 63) holds cell
+    // eslot[s] & 0xffffffff with eslot[s] >> 32 entries; null: cell s
+    const int64_t* eslot = nullptr;
     // [ND][C] every cell's closure residual sum A r_ij/|r_ij| (mixed
     // gradients), or null where the cells' faces close
     const T* closure = nullptr;
@@ -167,6 +173,19 @@ __device__ __forceinline__ void load_row(const double* __restrict__ a, int64_t o
         out[1] = x.y;
         out[2] = y.x;
         out[3] = y.y;
+    } else if constexpr(SFX_ROW_LOADS3) {
+        // three loads: two 16-byte ones at the row's own 8-byte alignment
+        // (global memory takes them unaligned) and one 8-byte one
+        (void)n;
+        typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+        const double* r = a + o * 5;
+        const d2u x = *reinterpret_cast<const d2u*>(r);
+        const d2u y = *reinterpret_cast<const d2u*>(r + 2);
+        out[0] = x.x;
+        out[1] = x.y;
+        out[2] = y.x;
+        out[3] = y.y;
+        out[4] = r[4];
     } else {
         (void)n;
         const int64_t s = o * 5, base = s & ~int64_t(1);
@@ -586,27 +605,49 @@ __device__ __forceinline__ void load_record(const MeshView<T>& m, int64_t base, 
     }
 }
 
-// One block per group of 64 cells (XCD-aware order).  The block's threads
-// walk the group's entries in entry order (the face reads of neighbouring
-// threads then fall on neighbouring faces, as for entry_geometry_kernel):
-// entry j's cell is the last l with off[l] <= j among the group's 65 offsets
-// (held in LDS; a cell without entries never matches), its row
-// r0 + j - off[l], its lane l.  A second loop writes zeros to the rows past
-// each cell's last entry.
+// One block per group of 64 slots (XCD-aware order).  Lane l holds slot
+// 64 g + l: cell c_l (the slot itself, or m.eslot's cell) with n_l entries
+// from cf_off[c_l]; the block's threads walk the group's entries lane after
+// lane, entry after entry (neighbouring threads then read a cell's
+// neighbouring entries): walk index j belongs to the last lane l with
+// off[l] <= j, off the prefix sums of the n_l (in LDS; a lane without
+// entries never matches), as that lane's entry j - off[l], written to row
+// egrp[g] + j - off[l].  A second loop writes zeros to the rows past each
+// lane's last entry.
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T> m,
                                                                      T* __restrict__ egeo) {
     constexpr int ER = entry_rec<ND, T>(), P = rec_planes<ND, T>();
-    __shared__ int64_t off[65];
+    __shared__ int64_t off[65], base[64];
     const int64_t g = xcd_block();
-    const int64_t c0 = g * 64;
-    const int nc = (int)((m.C - c0) < 64 ? (m.C - c0) : 64);
-    if((int)threadIdx.x <= nc) off[threadIdx.x] = m.cf_off[c0 + threadIdx.x];
+    const int64_t s0 = g * 64;
+    const int nc = (int)((m.C - s0) < 64 ? (m.C - s0) : 64);
+    if((int)threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        int64_t n = 0, b = 0;
+        if(l < nc) {
+            if(m.eslot) {
+                const int64_t v = m.eslot[s0 + l];
+                b = m.cf_off[v & 0xffffffff];
+                n = v >> 32;
+            } else {
+                b = m.cf_off[s0 + l];
+                n = m.cf_off[s0 + l + 1] - b;
+            }
+        }
+        base[l] = b;
+        off[l + 1] = n;  // counts, summed below
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        off[0] = 0;
+        for(int l = 0; l < 64; l++) off[l + 1] += off[l];
+    }
     __syncthreads();
     const int64_t r0 = m.egrp[g], K = m.egrp[g + 1] - r0;
-    const int64_t j0 = off[0], j1 = off[nc];
+    const int64_t j1 = off[nc];
     float4* out = reinterpret_cast<float4*>(egeo);
-    for(int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    for(int64_t j = threadIdx.x; j < j1; j += blockDim.x) {
         int lo = 0, hi = nc;  // off[lo] <= j < off[hi]
         while(hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -615,12 +656,12 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
             else
                 hi = mid;
         }
+        const int64_t k = j - off[lo];
         alignas(16) T rec[ER];
-        entry_record<ND, T>(m, m.cf_entry[j], rec);
-        const int64_t r = r0 + (j - off[lo]);
+        entry_record<ND, T>(m, m.cf_entry[base[lo] + k], rec);
 #pragma unroll
         for(int q = 0; q < P; q++)
-            out[(r * P + q) * 64 + lo] = reinterpret_cast<const float4*>(rec)[q];
+            out[((r0 + k) * P + q) * 64 + lo] = reinterpret_cast<const float4*>(rec)[q];
     }
     for(int64_t sl = threadIdx.x; sl < K * 64; sl += blockDim.x) {
         const int64_t k = sl >> 6;
@@ -655,26 +696,34 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
 // way: d = 0 gives dp = 0, or NaN against a non-finite gradient, and neither
 // changes alpha).
 template <int ND, bool GROUPED, bool ACT, bool BUF, int LK = 0>
-__device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t i,
+__device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t s,
                                               float* __restrict__ grad, int64_t gshift,
                                               float* __restrict__ dcache = nullptr) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, float>();
-    // The loads that depend on nothing go out together, before any wait:
-    // the activity flag (ACT: m.active is set), the record range, the cell's
-    // state and volume.  Inactive cells (which keep their gradients) walk no
-    // records and store nothing; a branch on the flag here would put its
-    // load alone in front of all the others.
-    const bool act = !ACT || m.active[i] != 0;
-    int64_t j0, n, own = 0;
+    // Slot s: grouped, the cell of lane s & 63 of group s >> 6 (m.eslot's,
+    // or cell s); else cell s.  The loads that depend on nothing go out
+    // together, before any wait: the record range, the slot's cell and
+    // entry count, then the activity flag (ACT: m.active is set), the
+    // cell's state and volume.  Inactive cells (which keep their gradients)
+    // walk no records and store nothing; a branch on the flag here would
+    // put its load alone in front of all the others.
+    int64_t i = s, j0, n, own = 0;
     if constexpr(GROUPED) {
-        const int64_t g = i >> 6;
+        const int64_t g = s >> 6;
         j0 = m.egrp[g];
         n = m.egrp[g + 1] - j0;
-        if constexpr(BUF) own = m.cf_off[i + 1] - m.cf_off[i];  // the cell's own rows
+        if(m.eslot) {
+            const int64_t v = m.eslot[s];
+            i = v & 0xffffffff;
+            own = v >> 32;  // the cell's own rows
+        } else if constexpr(BUF) {
+            own = m.cf_off[s + 1] - m.cf_off[s];
+        }
     } else {
         j0 = m.cf_off[i];
         n = m.cf_off[i + 1] - j0;
     }
+    const bool act = !ACT || m.active[i] != 0;
     const __amdgpu_buffer_rsrc_t rs = rec_rsrc(m.egeo, BUF ? m.egeo_bytes : 0);
     double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
     load_row<NC>(m.W, i, m.C, W);
@@ -692,7 +741,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     // entries hold area 0 and are skipped, so every lane of a group walks
     // the same rows)
     const float4* E4 = reinterpret_cast<const float4*>(m.egeo);
-    const int lane = (int)(i & 63);
+    const int lane = (int)(s & 63);
     auto half = [&](int64_t p, int h) -> float4 {
         if constexpr(GROUPED && BUF)
             return rec_plane(rs, (2 * (j0 + p) + h) * 64 + lane, p < own);
@@ -881,22 +930,23 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
 // Cell i; its gradients go to grad[(i + gshift) * ND * NC ...] (gshift: 0
 // for a whole mesh, the arena offset of the cell's mesh in an ensemble).
 template <int ND, typename T, typename R, bool DIFF, bool GROUPED, bool ACT>
-__device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
+__device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t s,
                                               T* __restrict__ grad, int64_t gshift) {
     constexpr int NC = ND + 2, ER = entry_rec<ND, T>();
-    // the loads that depend on nothing go out together: the activity flag
-    // (ACT: m.active is set), the record range, the cell's state and
-    // volume; inactive cells (which keep their gradients) walk no records
-    // and store nothing
-    const bool act = !ACT || m.active[i] != 0;
-    int64_t base, n;
+    // slot s (as in grad_cell_f32); the loads that depend on nothing go out
+    // together: the record range and the slot's cell, then the activity
+    // flag (ACT: m.active is set), the cell's state and volume; inactive
+    // cells (which keep their gradients) walk no records and store nothing
+    int64_t i = s, base, n;
     if constexpr(GROUPED) {
-        base = m.egrp[i >> 6];
-        n = m.egrp[(i >> 6) + 1] - base;
+        base = m.egrp[s >> 6];
+        n = m.egrp[(s >> 6) + 1] - base;
+        if(m.eslot) i = m.eslot[s] & 0xffffffff;
     } else {
         base = m.cf_off[i];
         n = m.cf_off[i + 1] - base;
     }
+    const bool act = !ACT || m.active[i] != 0;
     R W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
 #pragma unroll
     for(int k = 0; k < NC; k++) {
@@ -911,7 +961,7 @@ __device__ __forceinline__ void grad_cell_f64(const MeshView<T>& m, int64_t i,
     // cell's group (grouped; rows past the cell's own entries have area 0);
     // positions below 0 of a chunk read position 0 and are skipped where
     // used (u >= pb)
-    const int lane = (int)(i & 63);
+    const int lane = (int)(s & 63);
     // KB records at a time: the records, then the neighbour states of a
     // chunk are loaded before any is used (every neighbour row
     // unconditionally: a mirror neighbour is the cell's own row, reflected
@@ -1028,46 +1078,47 @@ __device__ __forceinline__ bool rec_buffer(const MeshView<T>& m) {
            m.egeo_bytes <= 0x7FFFFFF0ull;
 }
 
+// Slot s: a cell of the grouped layout (see grad_cell_f32), or cell s.
 template <int ND, typename T, typename R, bool DIFF>
-__device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t i, T* __restrict__ grad,
+__device__ __forceinline__ void grad_cell(const MeshView<T>& m, int64_t s, T* __restrict__ grad,
                                           int64_t gshift) {
     if constexpr(sizeof(T) == 4) {
         static_assert(DIFF, "FP32 records carry the difference-form weights");
         if(m.egrp && rec_buffer(m)) {
             if(m.active)
-                grad_cell_f32<ND, true, true, true>(m, i, grad, gshift);
+                grad_cell_f32<ND, true, true, true>(m, s, grad, gshift);
             else
-                grad_cell_f32<ND, true, false, true>(m, i, grad, gshift);
+                grad_cell_f32<ND, true, false, true>(m, s, grad, gshift);
         } else if(m.egrp) {
             if(m.active)
-                grad_cell_f32<ND, true, true, false>(m, i, grad, gshift);
+                grad_cell_f32<ND, true, true, false>(m, s, grad, gshift);
             else
-                grad_cell_f32<ND, true, false, false>(m, i, grad, gshift);
+                grad_cell_f32<ND, true, false, false>(m, s, grad, gshift);
         } else {
             if(m.active)
-                grad_cell_f32<ND, false, true, false>(m, i, grad, gshift);
+                grad_cell_f32<ND, false, true, false>(m, s, grad, gshift);
             else
-                grad_cell_f32<ND, false, false, false>(m, i, grad, gshift);
+                grad_cell_f32<ND, false, false, false>(m, s, grad, gshift);
         }
     } else {
         if(m.egrp) {
             if(m.active)
-                grad_cell_f64<ND, T, R, DIFF, true, true>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, true, true>(m, s, grad, gshift);
             else
-                grad_cell_f64<ND, T, R, DIFF, true, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, true, false>(m, s, grad, gshift);
         } else {
             if(m.active)
-                grad_cell_f64<ND, T, R, DIFF, false, true>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, false, true>(m, s, grad, gshift);
             else
-                grad_cell_f64<ND, T, R, DIFF, false, false>(m, i, grad, gshift);
+                grad_cell_f64<ND, T, R, DIFF, false, false>(m, s, grad, gshift);
         }
     }
 }
 
 template <int ND, typename T, typename R, bool DIFF>
 __global__ __launch_bounds__(256) void gradient_kernel(MeshView<T> m, T* __restrict__ grad) {
-    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if(i < m.C) grad_cell<ND, T, R, DIFF>(m, i, grad, 0);
+    const int64_t s = xcd_block() * blockDim.x + threadIdx.x;  // slot
+    if(s < m.C) grad_cell<ND, T, R, DIFF>(m, s, grad, 0);
 }
 
 // LDS positions per lane and waves per SIMD of gradient_lds_kernel: 3D
@@ -1091,12 +1142,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_lds_wa
 gradient_lds_kernel(MeshView<float> m, float* __restrict__ grad) {
     constexpr int LK = grad_lds_k<ND>() > 0 ? grad_lds_k<ND>() : 1;
     __shared__ float dcache[LK * ND * 256];
-    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if(i >= m.C) return;
+    const int64_t s = xcd_block() * blockDim.x + threadIdx.x;  // slot
+    if(s >= m.C) return;
     if(m.active)
-        grad_cell_f32<ND, true, true, true, LK>(m, i, grad, 0, dcache);
+        grad_cell_f32<ND, true, true, true, LK>(m, s, grad, 0, dcache);
     else
-        grad_cell_f32<ND, true, false, true, LK>(m, i, grad, 0, dcache);
+        grad_cell_f32<ND, true, false, true, LK>(m, s, grad, 0, dcache);
 }
 
 // ------------------------------------------------------- K3 face fluxes --

@@ -57,6 +57,7 @@ flux::MeshView<T> view(const sfx_mesh* m) {
     v.cf_entry = m->cf_entry;
     v.egeo = (const T*)m->egeo;
     v.egrp = m->egeo_group;
+    v.eslot = m->egeo_group ? m->egeo_slot : nullptr;
     if(m->egeo_group && m->egeo_rows > 0)
         v.egeo_bytes = (uint64_t)m->egeo_rows * 64 * flux::entry_rec(m->ndim, sizeof(T) == 4) *
                        sizeof(T);
@@ -272,6 +273,46 @@ int64_t sfx_mesh_entry_groups(int64_t C, const int64_t* cf_off, int64_t* egeo_gr
         int64_t k = 0;
         for(int64_t i = 64 * g; i < std::min(C, 64 * g + 64); i++)
             k = std::max(k, cf_off[i + 1] - cf_off[i]);
+        rows += k;
+    }
+    egeo_group[ng] = rows;
+    return rows;
+}
+
+int64_t sfx_mesh_entry_groups_sorted(int64_t C, const int64_t* cf_off, int64_t window,
+                                     int64_t* egeo_slot, int64_t* egeo_group) {
+    if(C < 0 || C > INT32_MAX || !cf_off || !egeo_slot || !egeo_group || window <= 0 ||
+       window % 64) {
+        set_error(SFX_ERR_ARG, "sfx_mesh_entry_groups_sorted: bad arguments");
+        return -1;
+    }
+    std::vector<int64_t> cells;
+    for(int64_t w0 = 0; w0 < C; w0 += window) {
+        const int64_t w1 = std::min(C, w0 + window);
+        cells.resize(w1 - w0);
+        for(int64_t i = w0; i < w1; i++) {
+            const int64_t n = cf_off[i + 1] - cf_off[i];
+            if(n < 0 || n > INT32_MAX) {
+                set_error(SFX_ERR_ARG, "sfx_mesh_entry_groups_sorted: cf_off not ascending");
+                return -1;
+            }
+            cells[i - w0] = i;
+        }
+        std::stable_sort(cells.begin(), cells.end(), [&](int64_t a, int64_t b) {
+            return cf_off[a + 1] - cf_off[a] > cf_off[b + 1] - cf_off[b];
+        });
+        for(int64_t k = 0; k < w1 - w0; k++) {
+            const int64_t c = cells[k];
+            egeo_slot[w0 + k] = c | ((cf_off[c + 1] - cf_off[c]) << 32);
+        }
+    }
+    const int64_t ng = (C + 63) / 64;
+    int64_t rows = 0;
+    for(int64_t g = 0; g < ng; g++) {
+        egeo_group[g] = rows;
+        int64_t k = 0;
+        for(int64_t s = 64 * g; s < std::min(C, 64 * g + 64); s++)
+            k = std::max(k, egeo_slot[s] >> 32);
         rows += k;
     }
     egeo_group[ng] = rows;

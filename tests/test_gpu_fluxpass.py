@@ -248,8 +248,9 @@ def poisson2d_bundle(n, kind, seed=9):
 @pytest.mark.parametrize("mesh", ["kh2d", "poisson2d", "sedov3d"])
 def test_grouped_records_bit_identical(device, mesh, kind, prec):
     """The gradient records in the grouped layout (64 cells' k-th entries
-    side by side) give the entry-ordered layout's gradients, face fluxes and
-    dQ bit for bit.  The mixed grouped pass keeps the limiter's d vectors of
+    side by side), in cell order or over cells ordered by entry count, give
+    the entry-ordered layout's gradients, face fluxes and dQ bit for bit.
+    The mixed grouped pass keeps the limiter's d vectors of
     a cell's first 8 (2D) or 16 (3D) positions in LDS and reloads the rest:
     the Poisson meshes have cells with more entries than that."""
     import torch
@@ -260,14 +261,17 @@ def test_grouped_records_bit_identical(device, mesh, kind, prec):
     if mesh != "kh2d":
         assert per_cell.max() > (8 if b["ndim"] == 2 else 16)
     res = []
-    for grouped in (False, True):
+    # entry order; groups in cell order; groups over cells sorted by entry
+    # count (windows of 64, 256 and 1024 cells)
+    for grouped, window in ((False, 0), (True, 0), (True, 64), (True, 256), (True, 1024)):
         s = RiemannSolver(kind, b["ndim"], G, 1e-8, -5.0, prec)
-        fp = FluxPass(s, M.DeviceMesh(b, prec, device, grouped=grouped))
+        fp = FluxPass(s, M.DeviceMesh(b, prec, device, grouped=grouped, sort_window=window))
         fp.run()
         torch.cuda.synchronize()
         res.append({"grad": fp.m.grad_soa(), "dQL": fp.m.dQL_soa(), "dQ": fp.m.dQ_soa()})
-    for k in ("grad", "dQL", "dQ"):
-        assert np.array_equal(res[0][k], res[1][k]), k
+    for r in res[1:]:
+        for k in ("grad", "dQL", "dQ"):
+            assert np.array_equal(res[0][k], r[k]), k
 
 
 @pytest.mark.parametrize("kind", [0, 1])

@@ -215,3 +215,70 @@ def test_entry_groups_by_hand():
     rows = C.lib().sfx_mesh_entry_groups(130, p(off), p(grp))
     assert rows == 3 + 5 + 7 and list(grp) == [0, 3, 8, 15]
     assert C.lib().sfx_mesh_entry_groups(-1, p(off), p(grp)) == -1
+
+
+def _sorted_groups(counts, window):
+    import ctypes as ct
+    from shadowfax_amd import _capi as C
+    C_ = len(counts)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    slot = np.zeros(C_, np.int64)
+    grp = np.zeros((C_ + 63) // 64 + 1, np.int64)
+    p = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_int64))
+    rows = C.lib().sfx_mesh_entry_groups_sorted(C_, p(off), window, p(slot), p(grp))
+    return rows, slot, grp
+
+
+def test_sorted_entry_groups_by_hand():
+    """sfx_mesh_entry_groups_sorted: within each window the cells take slots
+    in descending order of entry count, ties in cell order; a group's rows
+    are its first slot's count; bad windows are refused."""
+    counts = np.array([3] * 64 + [5] + [2] * 63 + [7, 1], np.int64)  # 130 cells
+    rows, slot, grp = _sorted_groups(counts, 128)
+    cells, n = slot & 0xFFFFFFFF, slot >> 32
+    # window 0 (cells 0..127): cell 64 (5 entries), then the 3s, then the 2s
+    assert list(cells[:66]) == [64] + list(range(64)) + [65]
+    assert list(n[:3]) == [5, 3, 3] and n[64] == 3 and n[65] == 2
+    # window 1 (cells 128, 129): 7 then 1
+    assert list(cells[128:]) == [128, 129] and list(n[128:]) == [7, 1]
+    # group 1 starts with cell 63 (3 entries): sorting moves the padding, it
+    # cannot remove what the counts force
+    assert rows == 5 + 3 + 7 and list(grp) == [0, 5, 8, 15]
+    for bad in (0, 63, 100, -64):
+        assert _sorted_groups(counts, bad)[0] == -1
+
+
+@pytest.mark.parametrize("window", [64, 1024])
+def test_sorted_entry_groups_on_a_poisson_mesh(window):
+    """On a 3D Poisson-Voronoi mesh the sorted slots are a permutation that
+    keeps every cell in its window, counts descend within each window, the
+    groups' rows are their maxima, and sorting over windows of 1024 cells
+    leaves far fewer padding rows than cell order (windows of 64 cells sort
+    within each group, which leaves the rows as they were)."""
+    import ctypes as ct
+    from shadowfax_amd import _capi as C
+    m = M.voronoi_periodic(M.hilbert_sorted(M.poisson_points(6000, seed=5)))
+    counts = np.diff(np.asarray(m["cf_off"], np.int64))
+    rows, slot, grp = _sorted_groups(counts, window)
+    cells, n = slot & 0xFFFFFFFF, slot >> 32
+    assert np.array_equal(np.sort(cells), np.arange(len(counts)))
+    assert np.array_equal(n, counts[cells])
+    assert np.array_equal(cells // window, np.arange(len(counts)) // window)
+    for w0 in range(0, len(counts), window):
+        nw = n[w0:w0 + window]
+        assert np.all(np.diff(nw) <= 0)
+        cw = cells[w0:w0 + window]
+        tie = np.diff(nw) == 0
+        assert np.all(np.diff(cw)[tie] > 0)
+    ng = (len(counts) + 63) // 64
+    pad = np.zeros(ng * 64, np.int64)
+    pad[:len(n)] = n
+    assert np.array_equal(np.diff(grp), pad.reshape(ng, 64).max(1))
+    grp0 = np.zeros(ng + 1, np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    p = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_int64))
+    rows0 = C.lib().sfx_mesh_entry_groups(len(counts), p(off), p(grp0))
+    if window == 64:
+        assert rows == rows0
+    else:
+        assert rows < 0.8 * rows0
