@@ -508,7 +508,7 @@ def main():
     fp3["host_mesh_s"] = t_mesh
     ms = fp3["mixed"]["ms_mean"]
     F = b3["F"]
-    face_k = "sfx::flux::face_flux_kernel<3, 0, 1, float, double>"
+    face_k = "sfx::flux::face_flux_kernel<3, 0, 1, float, double, true>"
     roof = roofline(b3, fp3, "3d", face_k,
                     ["sfx::flux::gradient_lds_kernel<3>", face_k,
                      "sfx::flux::gather_kernel<3, 1, float>"])
@@ -544,7 +544,7 @@ def main():
         fp2["workload"] = ("2D periodic unit box, 512x512 jittered generators (seed %d) in "
                            "Hilbert order, Kelvin-Helmholtz, HLLC" % SEED_2D)
         fp2["roofline"] = roofline(b2, fp2, "2d", "sfx::flux::face_flux_kernel<2, 1, 1, float, "
-                                                   "double>", [])
+                                                   "double, true>", [])
         out["fluxpass_2d_kh"] = fp2
         del b2
     if not args.no_riemann:
