@@ -763,7 +763,7 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
         }
     };
     // 2D: chunks of 4 records (a cell has about 6), each chunk's neighbour
-    // rows two at a time -- 124 VGPRs, so four waves per SIMD and the 2D
+    // rows two at a time -- 126 VGPRs, so four waves per SIMD and the 2D
     // benchmark mesh's 4,096 waves in one round (chunks of 8, rows four at a
     // time: 152 VGPRs, three waves, 20.3 against 18.1 us); 3D: chunk after
     // chunk, the next chunk's records in flight while this chunk's
