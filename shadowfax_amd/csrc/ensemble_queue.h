@@ -28,6 +28,10 @@
 
 #include "ensemble_impl.h"
 
+#ifndef SFX_QUEUE_WAVES_2D  // waves per SIMD the 2D serving kernel is compiled for
+#define SFX_QUEUE_WAVES_2D 2
+#endif
+
 namespace sfx {
 namespace ens {
 
@@ -52,6 +56,7 @@ struct DevQueue {
     uint32_t bar_count, bar_gen;
     int32_t stop, error;
     uint64_t nbatch;
+    uint64_t consumed;  // mailbox entries pulled so far (the leader's thread 0)
 };
 
 __device__ __forceinline__ uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
@@ -118,20 +123,23 @@ __device__ __forceinline__ void pull(DevQueue* q, const HostBox* box, const int3
 }
 
 // The leader's part of one iteration (all threads of block 0 call it).
-// bits: LDS bitset over mesh ids (the meshes already in the batch).
-template <typename T>
-__device__ void leader(const EnsArgs<T>& a, DevQueue* q, HostBox* box, const int32_t* box_mesh,
-                       int32_t* box_done, const int64_t* mc0, const int64_t* mnc,
-                       const int64_t* mf0, const int64_t* mnf, int64_t arena_cells,
-                       int64_t arena_faces, uint64_t& consumed, uint32_t* bits) {
+// bits: LDS bitset over mesh ids (the meshes already in the batch).  Its
+// arguments are pointers and values, nothing the kernel's other phases
+// read through a reference: where the compiler keeps it a call (it does
+// for 2D), a reference to the kernel's argument block would put that block
+// in scratch memory for the whole kernel.
+static __device__ void leader(Plan* p, int tile, DevQueue* q, HostBox* box,
+                              const int32_t* box_mesh, int32_t* box_done, const int64_t* mc0,
+                              const int64_t* mnc, const int64_t* mf0, const int64_t* mnf,
+                              int64_t arena_cells, int64_t arena_faces, uint32_t* bits) {
     __shared__ int32_t s_req[kStage], s_mesh[kStage];
     __shared__ int64_t s_nc[kStage], s_nf[kStage];
     __shared__ uint32_t s_taken[kStage];
     __shared__ int32_t s_n, s_stop, s_quit;
     __shared__ int64_t s_cells, s_faces;
     __shared__ uint64_t s_pos, s_tail;
-    Plan* p = a.plan;
     if(threadIdx.x == 0) {
+        uint64_t consumed = q->consumed;
         // the previous batch is complete (the update's barrier is behind us)
         const int32_t nb = q->batch_n;
         for(int k = 0; k < nb; k++)
@@ -161,6 +169,7 @@ __device__ void leader(const EnsArgs<T>& a, DevQueue* q, HostBox* box, const int
             }
             __builtin_amdgcn_s_sleep(8);
         }
+        q->consumed = consumed;
         s_n = 0;
         s_cells = s_faces = 0;
         s_stop = 0;
@@ -219,7 +228,6 @@ __device__ void leader(const EnsArgs<T>& a, DevQueue* q, HostBox* box, const int
     // the plan (plan_kernel's arithmetic, sequential over the batch)
     if(threadIdx.x == 0) {
         const int n = s_n;
-        const int tile = a.block;
         int64_t c = 0, f = 0, ct = 0, ft = 0;
         for(int k = 0; k < n; k++) {
             const int32_t m = (int32_t)p->slot[k].cell0;
@@ -286,11 +294,17 @@ __device__ __forceinline__ unsigned q_update(const EnsArgs<T>& a) {
 // The persistent serving kernel.  Launched cooperatively with every block
 // resident; blockDim = a.block.  Dynamic LDS: the leader's bitset over mesh
 // ids (4 bytes per 32 meshes).  One function holding the four phases needs
-// more registers than any of them alone (3D mixed: 245 VGPRs against 191
-// for the gradient drain); it is held to two waves per SIMD (2D: 432 bytes
-// of spills a lane).
+// more registers than any of them alone (3D mixed: 236 VGPRs against 191
+// for the gradient drain, 2D: 208); it runs at two waves per SIMD (held to
+// three or four, the 2D kernel spills and serves fewer requests).
+template <int ND>
+constexpr int queue_waves() {
+    return ND == 2 ? SFX_QUEUE_WAVES_2D : 2;
+}
+
 template <int ND, int KIND, int MIXED, typename T, typename R>
-__global__ __launch_bounds__(256, 2) void queue_kernel(EnsArgs<T> a, mixed::CF cf, DevQueue* q,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(queue_waves<ND>()))) void
+queue_kernel(EnsArgs<T> a, mixed::CF cf, DevQueue* q,
                                                     HostBox* box, const int32_t* box_mesh,
                                                     int32_t* box_done, const int64_t* mc0,
                                                     const int64_t* mnc, const int64_t* mf0,
@@ -301,13 +315,12 @@ __global__ __launch_bounds__(256, 2) void queue_kernel(EnsArgs<T> a, mixed::CF c
     __syncthreads();
     const unsigned nb = gridDim.x;
     unsigned gen = 0;
-    uint64_t consumed = 0;
     unsigned fail = 0;
     unsigned cnt[CNT_N] = {0u, 0u, 0u, 0u, 0u};
     for(;;) {
         if(blockIdx.x == 0)
-            leader<T>(a, q, box, box_mesh, box_done, mc0, mnc, mf0, mnf, arena_cells, arena_faces,
-                      consumed, bits);
+            leader(a.plan, a.block, q, box, box_mesh, box_done, mc0, mnc, mf0, mnf,
+                   arena_cells, arena_faces, bits);
         if(!grid_barrier(q, nb, gen)) break;
         if(__hip_atomic_load(&q->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
         q_gradients<ND, T, R, MIXED != 0>(a);
