@@ -12,14 +12,14 @@
 // batch give every mesh its arena offsets and its range of work tiles, and
 // it zeroes the tile counters) followed by four persistent drains --
 // gradients, face fluxes, gather, update -- each a fixed grid whose blocks
-// claim tiles of `tile` consecutive cells or faces, eight per atomic, until
-// none are left (or, as the autotuner may choose, a grid of one block per
-// possible tile, see drain).
-// The tile range of a phase is cut into one contiguous chunk per XCD; a
-// block drains the chunk of the XCD it runs on (dispatch index % 8) first,
-// then the others, so neighbouring cells mostly meet in one L2.  Every block
-// leaves a chunk once its counter passes the chunk's end, so every drain
-// ends after at most one pass over the eight counters.
+// take tiles of `tile` consecutive cells or faces (or, as the autotuner may
+// choose, a grid of one block per possible tile, see drain).
+// The tile range of a phase is cut into one contiguous chunk per XCD, and
+// the blocks of the XCD a block runs on (dispatch index % 8) share its
+// chunk, every block taking every nx-th tile, so neighbouring cells mostly
+// meet in one L2.  (SFX_ENS_CLAIM: blocks claim tiles through one atomic
+// counter per chunk instead, eight per claim, their own chunk first, then
+// the others'.)
 //
 // The per-item work is exactly the standalone kernels' (grad_cell,
 // face_item, gather_cell, update_cell in flux_pass_impl.h), so a mesh
@@ -30,6 +30,10 @@
 
 namespace sfx {
 namespace ens {
+
+#ifndef SFX_ENS_CLAIM  // persistent drains: 1 = tiles claimed through atomic counters, 0 = static stride
+#define SFX_ENS_CLAIM 0
+#endif
 
 constexpr int kMaxBatch = 1024;
 constexpr int kXcd = 8;
@@ -97,29 +101,48 @@ __device__ __forceinline__ void drain(Plan* __restrict__ p, int phase, int grid,
         if(t < hi) run_tile<FACES>(p, t, p->tile, fn);
         return;
     }
-    // with fewer claims than blocks the first ones take them all
-    if((int64_t)blockIdx.x * kClaim >= T) return;
-    const int tile = p->tile;
-    const int x0 = blockIdx.x % kXcd;
-    for(int c = 0; c < kXcd; c++) {
-        const int x = (x0 + c) % kXcd;
-        const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
-        for(;;) {
-            if(threadIdx.x == 0) {
-                // counters only grow: a chunk seen exhausted by a plain load
-                // is exhausted, and costs no read-modify-write
-                const unsigned seen = __atomic_load_n(&p->ctr[phase][x], __ATOMIC_RELAXED);
-                s_tile = lo + (int64_t)seen >= hi
-                                 ? hi
-                                 : lo + (int64_t)atomicAdd(&p->ctr[phase][x], (unsigned)kClaim);
+    if constexpr(SFX_ENS_CLAIM) {
+        // with fewer claims than blocks the first ones take them all
+        if((int64_t)blockIdx.x * kClaim >= T) return;
+        const int tile = p->tile;
+        const int x0 = blockIdx.x % kXcd;
+        for(int c = 0; c < kXcd; c++) {
+            const int x = (x0 + c) % kXcd;
+            const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
+            for(;;) {
+                if(threadIdx.x == 0) {
+                    // counters only grow: a chunk seen exhausted by a plain
+                    // load is exhausted, and costs no read-modify-write
+                    const unsigned seen = __atomic_load_n(&p->ctr[phase][x], __ATOMIC_RELAXED);
+                    s_tile = lo + (int64_t)seen >= hi
+                                     ? hi
+                                     : lo + (int64_t)atomicAdd(&p->ctr[phase][x], (unsigned)kClaim);
+                }
+                __syncthreads();
+                const int64_t t0 = s_tile;
+                __syncthreads();
+                if(t0 >= hi) break;
+                const int64_t t1 = (t0 + kClaim < hi) ? t0 + kClaim : hi;
+                for(int64_t t = t0; t < t1; t++) run_tile<FACES>(p, t, tile, fn);
             }
-            __syncthreads();
-            const int64_t t0 = s_tile;
-            __syncthreads();
-            if(t0 >= hi) break;
-            const int64_t t1 = (t0 + kClaim < hi) ? t0 + kClaim : hi;
-            for(int64_t t = t0; t < t1; t++) run_tile<FACES>(p, t, tile, fn);
         }
+    } else {
+        // static stride: the blocks of XCD x (dispatch index % 8) take every
+        // nx-th tile of chunk x, nx = the number of such blocks; no atomics
+        // (claims from every block through eight counters serialised where
+        // those counters live, and a run's speed then hung on their
+        // placement)
+        (void)phase;
+        (void)s_tile;
+        const int tile = p->tile;
+        if(gridDim.x < (unsigned)kXcd) {  // a chunk per XCD needs a block on each
+            for(int64_t t = blockIdx.x; t < T; t += gridDim.x) run_tile<FACES>(p, t, tile, fn);
+            return;
+        }
+        const int x = blockIdx.x % kXcd;
+        const int64_t nx = ((int64_t)gridDim.x - x + kXcd - 1) / kXcd;
+        const int64_t lo = T * x / kXcd, hi = T * (x + 1) / kXcd;
+        for(int64_t t = lo + blockIdx.x / kXcd; t < hi; t += nx) run_tile<FACES>(p, t, tile, fn);
     }
 }
 
