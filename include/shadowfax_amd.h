@@ -389,7 +389,10 @@ typedef struct {
  *          (~f: mirror), (2D) pad | dL or dR (ndim), area, (2D) pad],
  *          r = rRL seen from i -- the weights of the difference form of
  *          the same sum, in two 16-byte halves.
- * E = cf_off[C]; egeo: T [E][sfx_mesh_entry_record(ndim, precision)]. */
+ * E = cf_off[C]; egeo: T [E][sfx_mesh_entry_record(ndim, precision)].
+ * Mixed handles read each face's values from m->frec when it is set
+ * (sfx_mesh_face_records, built from the same arrays: the same records,
+ * fewer loads), else from the face arrays. */
 int sfx_mesh_entry_geometry(sfx_solver* h, const sfx_mesh* m, int64_t E, void* egeo,
                             void* stream);
 

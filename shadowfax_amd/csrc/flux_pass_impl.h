@@ -487,6 +487,45 @@ __global__ __launch_bounds__(256) void face_record_kernel(MeshView<float> m,
 
 // One lane per cell entry: the record of entry j (face f = e >> 1 seen from
 // side e & 1) from the face arrays; run once per tessellation.
+// A face's own values as the face records hold them (face_record_kernel):
+// 3 (2D) or 4 (3D) 16-byte loads.
+template <int ND>
+struct FaceVals {
+    int32_t left, right;
+    float area, rcent, rRL[ND], mc[ND], dL[ND], dR[ND];
+};
+template <int ND>
+__device__ __forceinline__ FaceVals<ND> load_face_record(const float4* __restrict__ frec,
+                                                         int64_t F, int64_t f) {
+    FaceVals<ND> o;
+    const float4 p0 = frec[f], p1 = frec[F + f], p2 = frec[2 * F + f];
+    o.left = __float_as_int(p0.x);
+    o.right = __float_as_int(p0.y);
+    o.area = p0.z;
+    o.rcent = p0.w;
+    if constexpr(ND == 3) {
+        const float4 p3 = frec[3 * F + f];
+        const float v[12] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            o.rRL[l] = v[l];
+            o.mc[l] = v[3 + l];
+            o.dL[l] = v[6 + l];
+            o.dR[l] = v[9 + l];
+        }
+    } else {
+        const float v[8] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            o.rRL[l] = v[l];
+            o.mc[l] = v[2 + l];
+            o.dL[l] = v[4 + l];
+            o.dR[l] = v[6 + l];
+        }
+    }
+    return o;
+}
+
 // The FP32 record of cell entry e = 2 face + side (entry_rec(ND, true)).
 template <int ND>
 __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32_t e,
@@ -494,27 +533,44 @@ __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32
     const int64_t F = m.F;
     const int32_t f = ent_face(e);
     const bool isR = ent_right(e);
+    // the face's values: from its record (4 loads in 3D) when the face
+    // records are built, else from the face arrays (the same values)
+    FaceVals<ND> fv;
+    if(m.frec) {
+        fv = load_face_record<ND>(m.frec, F, f);
+    } else {
+        fv.left = m.left[f];
+        fv.right = m.right[f];
+        fv.area = m.area[f];
+#pragma unroll
+        for(int l = 0; l < ND; l++) {
+            fv.rRL[l] = m.rRL[l * F + f];
+            fv.mc[l] = m.mc[l * F + f];
+            fv.dL[l] = m.dL[l * F + f];
+            fv.dR[l] = m.dR[l * F + f];
+        }
+    }
     // g = A (c - r_ij/2)/|r_ij| in FP64
     double r[ND], r2 = 0.0;
 #pragma unroll
     for(int l = 0; l < ND; l++) {
-        const double rr = (double)m.rRL[l * F + f];
+        const double rr = (double)fv.rRL[l];
         r[l] = isR ? rr : -rr;
         r2 += r[l] * r[l];
     }
-    const double s = (double)m.area[f] / sqrt(r2);
+    const double s = (double)fv.area / sqrt(r2);
     using RL = Rec32<ND>;
 #pragma unroll
     for(int k = 0; k < 8; k++) rec[k] = 0.f;
 #pragma unroll
     for(int l = 0; l < ND; l++) {
-        rec[RL::G + l] = (float)(s * ((double)m.mc[l * F + f] - 0.5 * r[l]));
-        rec[RL::D + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+        rec[RL::G + l] = (float)(s * ((double)fv.mc[l] - 0.5 * r[l]));
+        rec[RL::D + l] = isR ? fv.dR[l] : fv.dL[l];
     }
-    const int32_t nb = isR ? m.left[f] : m.right[f];
+    const int32_t nb = isR ? fv.left : fv.right;
     rec[RL::NB] = index_bits(nb >= 0 ? nb : ~f, 0.f);
-    rec[RL::A] = m.area[f];
-    rec[RL::A2] = m.area[f];
+    rec[RL::A] = fv.area;
+    rec[RL::A2] = fv.area;
 }
 
 // The FP64 record of cell entry e (entry_rec(ND, false)).
@@ -1193,32 +1249,18 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
         R A, rr[ND], mcv[ND], dLf[ND], dRf[ND];
         T rcent_f;
         if constexpr(FREC) {
-            // the face's record: 3 (2D) or 4 (3D) float4 planes (face_record_kernel)
-            const float4 p0 = m.frec[f], p1 = m.frec[F + f], p2 = m.frec[2 * F + f];
-            L = __float_as_int(p0.x);
-            Rc = __float_as_int(p0.y);
-            A = (R)p0.z;
-            rcent_f = p0.w;
-            if constexpr(ND == 3) {
-                const float4 p3 = m.frec[3 * F + f];
-                const float v[12] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y,
-                                     p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+            // the face's record (face_record_kernel)
+            const FaceVals<ND> fv = load_face_record<ND>(m.frec, F, f);
+            L = fv.left;
+            Rc = fv.right;
+            A = (R)fv.area;
+            rcent_f = fv.rcent;
 #pragma unroll
-                for(int l = 0; l < ND; l++) {
-                    rr[l] = (R)v[l];
-                    mcv[l] = (R)v[3 + l];
-                    dLf[l] = (R)v[6 + l];
-                    dRf[l] = (R)v[9 + l];
-                }
-            } else {
-                const float v[8] = {p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
-#pragma unroll
-                for(int l = 0; l < ND; l++) {
-                    rr[l] = (R)v[l];
-                    mcv[l] = (R)v[2 + l];
-                    dLf[l] = (R)v[4 + l];
-                    dRf[l] = (R)v[6 + l];
-                }
+            for(int l = 0; l < ND; l++) {
+                rr[l] = (R)fv.rRL[l];
+                mcv[l] = (R)fv.mc[l];
+                dLf[l] = (R)fv.dL[l];
+                dRf[l] = (R)fv.dR[l];
             }
         } else {
             L = m.left[f];

@@ -437,14 +437,15 @@ class DeviceMesh:
 
     def build_entries(self, solver_handle, stream=None):
         """Per-entry face records of the gradient pass (once per mesh)."""
-        Cp.check(Cp.lib().sfx_mesh_entry_geometry(solver_handle, ct.byref(self.desc), self.E,
-                                                  ct.c_void_p(self.egeo.data_ptr()), stream),
-                 "sfx_mesh_entry_geometry")
+        # the face records first: the entry records are then built from them
         if self.frec is not None:
             Cp.check(Cp.lib().sfx_mesh_face_records(solver_handle, ct.byref(self.desc),
                                                     ct.c_void_p(self.frec.data_ptr()), stream),
                      "sfx_mesh_face_records")
             self.desc.frec = ct.c_void_p(self.frec.data_ptr())
+        Cp.check(Cp.lib().sfx_mesh_entry_geometry(solver_handle, ct.byref(self.desc), self.E,
+                                                  ct.c_void_p(self.egeo.data_ptr()), stream),
+                 "sfx_mesh_entry_geometry")
         self.egeo_built = True
 
     # host copies in the component-major layout of the bundle / oracle

@@ -301,6 +301,24 @@ def test_face_records_bit_identical(device, mesh, kind):
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
+def test_face_records_refuse_fp64_handles(device):
+    """sfx_mesh_face_records builds the mixed pass's records only: an FP64
+    handle gets SFX_ERR_ARG and a message, and nothing is launched."""
+    import ctypes as ct
+    import torch
+    from shadowfax_amd import _capi as C
+    b = kh_bundle(8, 1)
+    s = RiemannSolver(1, 2, G, 1e-8, -5.0, "fp64")
+    dm = M.DeviceMesh(b, "fp64", device)
+    out = torch.zeros((3, b["F"], 4), dtype=torch.float32, device=device)
+    rc = C.lib().sfx_mesh_face_records(s.handle, ct.byref(dm.desc), ct.c_void_p(out.data_ptr()),
+                                       None)
+    assert rc == C.SFX_ERR_ARG
+    assert b"mixed" in C.lib().sfx_last_error()
+    torch.cuda.synchronize()
+    assert out.abs().max().item() == 0.0
+
+
 @pytest.mark.parametrize("shape", [(2, 2), (2, 3), (2, 5), (3, 20), (3, 40)])
 @pytest.mark.parametrize("kind", [0, 1])
 def test_tiny_meshes(device, shape, kind):
