@@ -70,11 +70,20 @@ hipError_t launch_gather_fp64(int ndim, int64_t C, int64_t F, const int64_t* gof
                               hipStream_t s) {
     if(C <= 0) return hipSuccess;
     const unsigned lds = flux::gather_lds_bytes<double>(C);
+    // large meshes (lds > 0): the variant with more rows in flight
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 0, double>), grid_for(C, 256), dim3(256), lds, s,
+        if(lds)
+            hipLaunchKernelGGL((flux::gather_kernel<3, 0, double, SFX_GATHER_KB_BIG_F64>), grid_for(C, 256), dim3(256), lds,
+                               s, C, F, goff, gent, nullptr, nullptr, dQL, dQ);
+        else
+            hipLaunchKernelGGL((flux::gather_kernel<3, 0, double>), grid_for(C, 256), dim3(256), lds, s,
                            C, F, goff, gent, nullptr, nullptr, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 0, double>), grid_for(C, 256), dim3(256), lds, s,
+        if(lds)
+            hipLaunchKernelGGL((flux::gather_kernel<2, 0, double, SFX_GATHER_KB_BIG_F64>), grid_for(C, 256), dim3(256), lds,
+                               s, C, F, goff, gent, nullptr, nullptr, dQL, dQ);
+        else
+            hipLaunchKernelGGL((flux::gather_kernel<2, 0, double>), grid_for(C, 256), dim3(256), lds, s,
                            C, F, goff, gent, nullptr, nullptr, dQL, dQ);
     return hipGetLastError();
 }

@@ -83,6 +83,12 @@
 #ifndef SFX_GATHER_KB_F64  // FP64 rows: the same registers as 8 FP32 rows
 #define SFX_GATHER_KB_F64 4
 #endif
+#ifndef SFX_GATHER_KB_BIG  // FP32 rows in flight per lane at the large meshes' bounded occupancy
+#define SFX_GATHER_KB_BIG 16  // (3D Sedov gather 251 -> 218 us; 8: 221, 32: 455, spilling)
+#endif
+#ifndef SFX_GATHER_KB_BIG_F64  // the same for FP64 rows
+#define SFX_GATHER_KB_BIG_F64 8  // (409 -> 374 us; 16: 389)
+#endif
 
 namespace sfx {
 namespace flux {
@@ -1532,7 +1538,7 @@ __global__ __launch_bounds__(256) void face_flux_kernel(MeshView<T> m, const T* 
 
 // Cell i: dQ[i] = its entries' dQL[(f + qshift)] in ascending face order,
 // summed in FP64 (T: storage of dQL).
-template <int ND, int MIXED, typename T>
+template <int ND, int MIXED, typename T, int KB0 = 0>
 __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict__ goff,
                                             const int32_t* __restrict__ gent,
                                             const uint8_t* __restrict__ ghost,
@@ -1548,10 +1554,11 @@ __device__ __forceinline__ void gather_cell(int64_t i, const int64_t* __restrict
 #pragma unroll
     for(int k = 0; k < NC; k++) acc[k] = 0.0;
     const int64_t j0 = goff[i], j1 = goff[i + 1];
-    // SFX_GATHER_KB entries at a time: their indices, then their rows, are
-    // loaded before any is added, so a lane has that many independent loads
-    // in flight; the additions keep ascending face order
-    constexpr int KB = sizeof(T) == 8 ? SFX_GATHER_KB_F64 : SFX_GATHER_KB;
+    // KB entries at a time (KB0, or SFX_GATHER_KB / _F64): their indices,
+    // then their rows, are loaded before any is added, so a lane has that
+    // many independent loads in flight; the additions keep ascending face
+    // order
+    constexpr int KB = KB0 > 0 ? KB0 : (sizeof(T) == 8 ? SFX_GATHER_KB_F64 : SFX_GATHER_KB);
     for(int64_t jb = j0; jb < j1; jb += KB) {
         int32_t f[KB];
         bool neg[KB];
@@ -1611,8 +1618,19 @@ inline unsigned gather_lds_bytes(int64_t C) {
                                SFX_GATHER_BIG, 0);
 }
 
-template <int ND, int MIXED, typename T>
-__global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
+// waves per SIMD a gather kernel is compiled for: KB0 > 0 is the variant for
+// large meshes, which run at the bounded occupancy of gather_lds_bytes, so
+// the compiler may spend that occupancy's registers on rows in flight
+template <typename T, int KB0>
+constexpr int gather_waves() {
+    return KB0 > 0 ? (sizeof(T) == 8 ? SFX_GATHER_WAVES_F64 : SFX_GATHER_WAVES) : 0;
+}
+
+template <int ND, int MIXED, typename T, int KB0 = 0>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+        gather_waves<T, KB0>() > 0 ? gather_waves<T, KB0>() : 1,
+        gather_waves<T, KB0>() > 0 ? gather_waves<T, KB0>() : 10))) void
+gather_kernel(int64_t C, int64_t F,
                                                      const int64_t* __restrict__ goff,
                                                      const int32_t* __restrict__ gent,
                                                      const uint8_t* __restrict__ ghost,
@@ -1621,7 +1639,7 @@ __global__ __launch_bounds__(256) void gather_kernel(int64_t C, int64_t F,
                                                      double* __restrict__ dQ) {
     (void)F;
     const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if(i < C) gather_cell<ND, MIXED, T>(i, goff, gent, ghost, twin, dQL, 0, dQ);
+    if(i < C) gather_cell<ND, MIXED, T, KB0>(i, goff, gent, ghost, twin, dQL, 0, dQ);
 }
 
 // ------------------------------------------------------------ K5 update --

@@ -111,11 +111,20 @@ hipError_t launch_gather_mixed(int ndim, int64_t C, int64_t F, const int64_t* go
                                const float* dQL, double* dQ, hipStream_t s) {
     if(C <= 0) return hipSuccess;
     const unsigned lds = flux::gather_lds_bytes<float>(C);
+    // large meshes (lds > 0): the variant with more rows in flight
     if(ndim == 3)
-        hipLaunchKernelGGL((flux::gather_kernel<3, 1, float>), grid_for(C, 256), dim3(256), lds, s,
+        if(lds)
+            hipLaunchKernelGGL((flux::gather_kernel<3, 1, float, SFX_GATHER_KB_BIG>), grid_for(C, 256), dim3(256), lds,
+                               s, C, F, goff, gent, ghost, twin, dQL, dQ);
+        else
+            hipLaunchKernelGGL((flux::gather_kernel<3, 1, float>), grid_for(C, 256), dim3(256), lds, s,
                            C, F, goff, gent, ghost, twin, dQL, dQ);
     else
-        hipLaunchKernelGGL((flux::gather_kernel<2, 1, float>), grid_for(C, 256), dim3(256), lds, s,
+        if(lds)
+            hipLaunchKernelGGL((flux::gather_kernel<2, 1, float, SFX_GATHER_KB_BIG>), grid_for(C, 256), dim3(256), lds,
+                               s, C, F, goff, gent, ghost, twin, dQL, dQ);
+        else
+            hipLaunchKernelGGL((flux::gather_kernel<2, 1, float>), grid_for(C, 256), dim3(256), lds, s,
                            C, F, goff, gent, ghost, twin, dQL, dQ);
     return hipGetLastError();
 }
