@@ -315,7 +315,8 @@ int sfx_mesh_gather_index(int64_t C, int64_t F, const int32_t* left,
  * face-major (W [C][ndim+2], vel [C][ndim], grad [C][ndim][ndim+2],
  * dQ and Q [C][ndim+2], dQL T [F][ndim+2]) so that one gather is one cache
  * line; per-face geometry, read one face per lane, is component-major
- * ([ndim][F]). */
+ * ([ndim][F]).  Zero-initialise it and set fields by name: optional
+ * arrays are NULL when unused. */
 typedef struct {
     int ndim;
     int64_t C, F;
