@@ -262,8 +262,9 @@ def test_grouped_records_bit_identical(device, mesh, kind, prec):
         assert per_cell.max() > (8 if b["ndim"] == 2 else 16)
     res = []
     # entry order; groups in cell order; groups over cells sorted by entry
-    # count (windows of 64, 256 and 1024 cells)
-    for grouped, window in ((False, 0), (True, 0), (True, 64), (True, 256), (True, 1024)):
+    # count (windows of 64, 256, 512 and 1024 cells)
+    for grouped, window in ((False, 0), (True, 0), (True, 64), (True, 256), (True, 512),
+                            (True, 1024)):
         s = RiemannSolver(kind, b["ndim"], G, 1e-8, -5.0, prec)
         fp = FluxPass(s, M.DeviceMesh(b, prec, device, grouped=grouped, sort_window=window))
         fp.run()
