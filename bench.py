@@ -511,7 +511,7 @@ def main():
     face_k = "sfx::flux::face_flux_kernel<3, 0, 1, float, double, true>"
     roof = roofline(b3, fp3, "3d", face_k,
                     ["sfx::flux::gradient_lds_kernel<3>", face_k,
-                     "sfx::flux::gather_kernel<3, 1, float>"])
+                     "sfx::flux::gather_kernel<3, 1, float, 16>"])
     out = {"metric": METRIC, "value": F / (ms * 1e-3), "unit": "face-fluxes/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -543,8 +543,10 @@ def main():
         fp2["host_mesh_s"] = t2
         fp2["workload"] = ("2D periodic unit box, 512x512 jittered generators (seed %d) in "
                            "Hilbert order, Kelvin-Helmholtz, HLLC" % SEED_2D)
-        fp2["roofline"] = roofline(b2, fp2, "2d", "sfx::flux::face_flux_kernel<2, 1, 1, float, "
-                                                   "double, true>", [])
+        face2 = "sfx::flux::face_flux_kernel<2, 1, 1, float, double, true>"
+        fp2["roofline"] = roofline(b2, fp2, "2d", face2,
+                                   ["sfx::flux::gradient_kernel<2, float, double, true>", face2,
+                                    "sfx::flux::gather_kernel<2, 1, float, 0>"])
         out["fluxpass_2d_kh"] = fp2
         del b2
     if not args.no_riemann:
