@@ -110,6 +110,8 @@ def test_gather_is_ordered_sum_of_gpu_face_fluxes(device, kind, prec):
         if b["right"][f] >= 0 and not b["ghost"][f]:
             acc[:, b["right"][f]] += -dQL[:, f]
     assert np.array_equal(acc, g["dQ"])
+    # the vectorised restatement over the gather entries agrees
+    assert np.array_equal(ordered_sums(b, dQL), g["dQ"])
 
 
 @pytest.mark.parametrize("nd", [2, 3])
@@ -347,6 +349,23 @@ def test_tiny_meshes(device, shape, kind):
     assert errs["conservation"] <= 1e-13
 
 
+def ordered_sums(b, dQL):
+    """Every cell's sum of its gather entries' rows of dQL (the right cell's
+    negated) in ascending face order, added one entry at a time in FP64 (the
+    order the reference's face loop reaches the cell), vectorised over cells."""
+    dQL = np.asarray(dQL, np.float64)
+    off = np.asarray(b["gather_off"], np.int64)
+    ent = np.asarray(b["gather_entry"], np.int64)
+    deg = np.diff(off)
+    acc = np.zeros((dQL.shape[0], len(deg)))
+    for p in range(int(deg.max()) if len(deg) else 0):
+        cells = np.where(deg > p)[0]
+        e = ent[off[cells] + p]
+        x = dQL[:, e >> 1]
+        acc[:, cells] += np.where((e & 1) == 1, -x, x)
+    return acc
+
+
 def test_sedov3d_full_size_mixed_errors(device):
     """The 3D benchmark workload at full size (2,097,152 Poisson generators,
     Sedov blast, exact solver), mixed pass against the FP64 pass: every cell
@@ -366,6 +385,10 @@ def test_sedov3d_full_size_mixed_errors(device):
     print("cells", b["C"], "faces", b["F"], flush=True)
     g64 = run_gpu(b, 0, "fp64", device)
     gmx = run_gpu(b, 0, "mixed", device)
+    # the gathers of meshes this large (their own kernel variants) sum each
+    # cell's rows in face order, bit for bit
+    assert np.array_equal(ordered_sums(b, g64["dQL"]), g64["dQ"])
+    assert np.array_equal(ordered_sums(b, gmx["dQL"]), gmx["dQ"])
     errs = mixed_errors(b, 0, g64, gmx)
     den = np.maximum(np.abs(g64["dQL"]).max(0), face_scale(b, 0))
     ef = np.abs(np.asarray(gmx["dQL"], np.float64) - g64["dQL"]).max(0) / np.where(den > 0, den, 1.0)
