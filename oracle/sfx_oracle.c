@@ -23,9 +23,14 @@
  *       with the reference is UNPINNED by a reference run (calculate_flux
  *       needs a TimeLine, whose constructor lives in TimeLine.cpp, which needs
  *       Boost.property_tree, absent here).  Pinned by hand-computed limiter
- *       cases, brute force on tiny meshes, and invariants: dQR == -dQL
- *       bitwise, conservation to round-off, uniform flow gives zero net dQ,
- *       zero gradients reduce the face flux to solve_for_flux exactly.
+ *       cases, brute force on tiny meshes, the closed forms of linear
+ *       pressure and velocity fields (the predicted face states follow from
+ *       the Euler equations in primitive form: a sign error in d, a dropped
+ *       1/2 or dt fails them; tests/test_oracle_predictor.py), the vacuum
+ *       keep, invariance under a periodic translation, and invariants: dQR
+ *       == -dQL bitwise, conservation to round-off, uniform flow gives zero
+ *       net dQ, zero gradients reduce the face flux to solve_for_flux
+ *       exactly.
  *   hydro time step (TimeLine::calculate_timestep) -- UNPINNED by a reference
  *       run (TimeLine.cpp needs Boost.property_tree).  Pinned by a brute-force
  *       search over all power-of-two fractions of the timeline on small
