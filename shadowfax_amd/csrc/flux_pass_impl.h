@@ -1286,8 +1286,12 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
         // twin >= 0 marks a periodic copy (-1 on every other face); the left
         // indices stand in for a missing twin array (never used)
         const int32_t twin_f = (m.twin ? m.twin : m.left)[f];
-        const uint8_t actLb = *(m.active ? m.active + L : &kAllActive);
-        const uint8_t actRb = *(m.active ? m.active + Rs : &kAllActive);
+        // global loads (a select between two pointers would make them flat
+        // loads, which also wait with the scalar loads)
+        typedef const __attribute__((address_space(1))) uint8_t gu8;
+        const gu8* actp = (const gu8*)(m.active ? m.active : &kAllActive);
+        const uint8_t actLb = actp[m.active ? (int64_t)L : 0];
+        const uint8_t actRb = actp[m.active ? Rs : 0];
         R vl[ND], vr[ND];
 #pragma unroll
         for(int l = 0; l < ND; l++) {
