@@ -787,16 +787,6 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     }
     const bool act = !ACT || m.active[i] != 0;
     const __amdgpu_buffer_rsrc_t rs = rec_rsrc(m.egeo, BUF ? m.egeo_bytes : 0);
-    double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
-    load_row<NC>(m.W, i, m.C, W);
-    const double Vinv = mixed::rcp_nr(m.volume[i]);
-    if(!act) n = 0;
-#pragma unroll
-    for(int k = 0; k < NC; k++) {
-        Wmax[k] = Wmin[k] = W[k];
-#pragma unroll
-        for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
-    }
     // the cell's records, visited in reverse walk order (src/VorCell.cpp:471)
     // as positions n - 1 .. 0: entry j0 + p (entry order), or row j0 + p of
     // the cell's group, lane i % 64 (grouped; rows past the cell's own
@@ -824,6 +814,29 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
             H[u] = half(p >= 0 ? p : 0, 1);
         }
     };
+    // grouped records through the buffer descriptor: the first chunk's
+    // loads go out first, without a test on n or the activity flag (a row
+    // the lane's cell does not have reads as zeros without a fetch; an
+    // inactive cell walks its rows and stores nothing), and the memory
+    // clobber keeps them ahead of the cell's own loads, so both are in
+    // flight together; sunk behind the loop's entry test, they went out
+    // only after the cell's state had arrived
+    constexpr bool EAGER = GROUPED && BUF;
+    float4 lo[KB], hi[KB];
+    if constexpr(EAGER) {
+        load_chunk(n, lo, hi);
+        asm volatile("" ::: "memory");
+    }
+    double W[NC], Wmax[NC], Wmin[NC], delta[ND][NC];
+    load_row<NC>(m.W, i, m.C, W);
+    const double Vinv = mixed::rcp_nr(m.volume[i]);
+    if(!EAGER && !act) n = 0;
+#pragma unroll
+    for(int k = 0; k < NC; k++) {
+        Wmax[k] = Wmin[k] = W[k];
+#pragma unroll
+        for(int l = 0; l < ND; l++) delta[l][k] = 0.0;
+    }
     // 2D: chunks of 4 records (a cell has about 6), each chunk's neighbour
     // rows two at a time -- 126 VGPRs, so four waves per SIMD and the 2D
     // benchmark mesh's 4,096 waves in one round (chunks of 8, rows four at a
@@ -831,8 +844,9 @@ __device__ __forceinline__ void grad_cell_f32(const MeshView<float>& m, int64_t 
     // chunk, the next chunk's records in flight while this chunk's
     // neighbour states load
     constexpr bool PIPE = ND == 3 && SFX_GRAD_PIPE;
-    float4 lo[KB], hi[KB];
-    if(n > 0) load_chunk(n, lo, hi);
+    if constexpr(!EAGER) {
+        if(n > 0) load_chunk(n, lo, hi);
+    }
     for(int64_t pb = n; pb > 0; pb -= KB) {
         // the neighbour rows in batches of KN (all KB at once in 3D)
         constexpr int KN = (ND == 2) ? SFX_GRAD_KN_2D : KB;
