@@ -1315,7 +1315,10 @@ __device__ __forceinline__ void face_item(const MeshView<T>& m, int64_t f,
         const double dtL = m.dt[L], dtR = m.dt[Rs];
         // 3D: the cells' masses and total areas load with the rest; 2D: after
         // the solve (fewer registers held through it: 2D mixed faces
-        // 26.9 -> 25.3 us, while 3D lost 2% that way)
+        // 26.9 -> 25.3 us, while 3D lost 2% that way).  The 3D pin below
+        // waits for these four loads before the states and gradients go
+        // out; pinned after those instead, the 3D mixed face kernel took
+        // 493 against 483 us.
         constexpr bool LATE = ND == 2;
         double massL = 0.0, massR = 0.0, areaL = 0.0, areaR = 0.0;
         if constexpr(!LATE) {
