@@ -433,36 +433,45 @@ def copy_probe(mib, dev):
     log("copy probe: %d MiB read + %d MiB written per launch" % (mib, mib))
 
 
-def cpu_baseline(b, kind):
+def cpu_baseline(b, kind, runs=5):
     """The oracle's FP64 restatement of the same pass (gradients, face
     fluxes, ordered gather) over the same mesh: on the job's CPU share of
     threads over the whole mesh (contiguous cell and face ranges per thread,
     then the per-cell gather in face order), and on one thread over a
     contiguous sample (the first 1/16 of the cells and faces: faces are in
-    creation order, so about the same cells).  A 1/64 sample warms each up."""
+    creation order, so about the same cells).  Each: one warm-up run, then
+    the median of `runs` timed runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.lib()
     T, aff = host_threads()
     C, F = b["C"], b["F"]
     cut = lambda n, k: (0, max(1, n // k))
-    O.threaded_pass(kind, b, T, cells=cut(C, 64), faces=cut(F, 64))
-    times = {}
-    t0 = time.perf_counter()
-    O.threaded_pass(kind, b, T, timer=lambda k, s: times.__setitem__(k, s))
-    tT = time.perf_counter() - t0
-    O.threaded_pass(kind, b, 1, cells=cut(C, 256), faces=cut(F, 256))
-    t0 = time.perf_counter()
-    O.threaded_pass(kind, b, 1, cells=cut(C, 16), faces=cut(F, 16))
-    t1 = time.perf_counter() - t0
+
+    def median_run(threads, **kw):
+        O.threaded_pass(kind, b, threads, **kw)
+        walls, phases = [], []
+        for _ in range(runs):
+            times = {}
+            t0 = time.perf_counter()
+            O.threaded_pass(kind, b, threads, timer=lambda k, s: times.__setitem__(k, s), **kw)
+            walls.append(time.perf_counter() - t0)
+            phases.append(times)
+        k = int(np.argsort(walls)[len(walls) // 2])
+        return walls[k], phases[k]
+
+    tT, times = median_run(T)
+    t1, _ = median_run(1, cells=cut(C, 16), faces=cut(F, 16))
     Fs = cut(F, 16)[1]
     return {"value": F / tT, "unit": "face-fluxes/s", "cores": T, "kind": "port",
             "sample": ("the oracle's FP64 pass over the whole mesh (%d cells, %d faces) on %d "
                        "threads: %.2f s (gradients %.2f, faces %.2f, gather %.2f); one thread "
-                       "over the first %d faces and %d cells: %.2f s, %.3g face-fluxes/s"
+                       "over the first %d faces and %d cells: %.2f s, %.3g face-fluxes/s; "
+                       "medians of %d runs after one warm-up run each"
                        % (C, F, T, tT, times["gradients"], times["faces"], times["gather"],
-                          Fs, cut(C, 16)[1], t1, Fs / t1)),
-            "single_thread": Fs / t1, "affinity_cores": aff, "cpu_model": cpu_model()}
+                          Fs, cut(C, 16)[1], t1, Fs / t1, runs)),
+            "single_thread": Fs / t1, "affinity_cores": aff, "nproc": os.cpu_count(),
+            "cpu_model": cpu_model()}
 
 
 def cpu_model():
