@@ -433,6 +433,70 @@ def copy_probe(mib, dev):
     log("copy probe: %d MiB read + %d MiB written per launch" % (mib, mib))
 
 
+def measured_peaks(dev, gib=2, reps=5):
+    """The GPU's measured ceilings (libsfxprobe.so): a float4 copy of gib GiB
+    (read + written bytes over the launch), and FP64 / FP32 FMA loops (2
+    flops an FMA, the packed FP32 forms off as in every kernel here); the
+    best of reps launches each, timed with HIP events on the launch stream.
+    The copy runs one block of 256 a CU with four non-temporal loads in
+    flight a lane: of 1-4-8 loads a lane at 1-32 blocks a CU, the fastest
+    (5.6-5.8 TB/s; 4.4-4.8 TB/s at 8 or more blocks a CU, torch's copy_
+    5.1)."""
+    import ctypes as ct
+    import torch
+    lib = ct.CDLL(os.path.join(ROOT, "shadowfax_amd", "libsfxprobe.so"))
+    lib.sfxp_copy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p]
+    lib.sfxp_fma.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
+    st = torch.cuda.current_stream()
+    sp = ct.c_void_p(st.cuda_stream)
+    props = torch.cuda.get_device_properties(dev)
+    blocks = props.multi_processor_count * 8
+
+    def best(launch):
+        t = []
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            rc = launch()
+            e1.record(st)
+            torch.cuda.synchronize()
+            if rc != 0:
+                raise RuntimeError("probe launch failed (%d)" % rc)
+            t.append(e0.elapsed_time(e1) * 1e-3)
+        return min(t[1:])
+
+    n4 = gib * (1 << 30) // 16
+    x = torch.ones(n4 * 4, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    tc = best(lambda: lib.sfxp_copy(ct.c_void_p(x.data_ptr()), ct.c_void_p(y.data_ptr()), n4,
+                                    props.multi_processor_count, sp))
+    assert bool(torch.equal(x[-4:], y[-4:]))
+    del x, y
+    iters = 4096
+    out = torch.empty(blocks * 256, dtype=torch.float64, device=dev)
+    flops = blocks * 256 * 16 * iters
+    t64 = best(lambda: lib.sfxp_fma(1, ct.c_void_p(out.data_ptr()), blocks, iters, sp))
+    t32 = best(lambda: lib.sfxp_fma(0, ct.c_void_p(out.data_ptr()), blocks, iters, sp))
+    del out
+    torch.cuda.empty_cache()
+    return {"copy_GBs": 2 * n4 * 16 / tc / 1e9, "fma_f64_TFs": flops / t64 / 1e12,
+            "fma_f32_TFs": flops / t32 / 1e12,
+            "how": "float4 copy of %d GiB (read + written bytes, one block of 256 a CU), FMA "
+                   "loops of %d x 8 chains on %d blocks of 256; best of %d launches"
+                   % (gib, iters, blocks, reps)}
+
+
+def measured_fracs(roof, mp):
+    """A roofline's achieved rates also as fractions of the measured
+    ceilings (the published peaks stay the reference)."""
+    if not mp:
+        return
+    if "hbm" in roof:
+        roof["hbm"]["frac_of_measured"] = roof["hbm"]["achieved_GBs"] / mp["copy_GBs"]
+    if "valu" in roof:
+        roof["valu"]["frac_of_measured"] = roof["valu"]["achieved_TFs"] / mp["fma_f64_TFs"]
+
+
 def cpu_baseline(b, kind, runs=5):
     """The oracle's FP64 restatement of the same pass (gradients, face
     fluxes, ordered gather) over the same mesh: on the job's CPU share of
@@ -494,6 +558,8 @@ def main():
     ap.add_argument("--no-riemann", action="store_true", help="skip the batched Riemann solves")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D flux pass")
     ap.add_argument("--no-serve", action="store_true", help="skip serving (2,000-mesh ensemble)")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip measuring the copy and FMA ceilings")
     ap.add_argument("--riemann-N", type=int, default=None)
     ap.add_argument("--serve-meshes", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=4000)
@@ -537,6 +603,12 @@ def main():
            "fluxpass_3d_sedov": fp3}
     out["mixed_vs_fp64_speedup"] = fp3["mixed_speedup"]
     out["errors"] = fp3["mixed_vs_fp64"]
+    mp = None
+    if not args.no_peaks:
+        log("measured ceilings (copy, FMA loops)")
+        mp = measured_peaks(dev)
+        out["measured_peaks"] = mp
+        measured_fracs(roof, mp)
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb, dev)
     if not args.no_cpu:
@@ -556,6 +628,7 @@ def main():
         fp2["roofline"] = roofline(b2, fp2, "2d", face2,
                                    ["sfx::flux::gradient_kernel<2, float, double, true>", face2,
                                     "sfx::flux::gather_kernel<2, 1, float, 0>"])
+        measured_fracs(fp2["roofline"], mp)
         out["fluxpass_2d_kh"] = fp2
         del b2
     if not args.no_riemann:

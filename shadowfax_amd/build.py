@@ -82,7 +82,23 @@ def build(verbose=False, jobs=None):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_probe(verbose)
     return LIB
+
+
+PROBE = os.path.join(HERE, "libsfxprobe.so")
+
+
+def build_probe(verbose=False):
+    """libsfxprobe.so: the copy and FMA-loop kernels bench.py measures the
+    GPU's ceilings with (not part of the face-flux library)."""
+    tmp = PROBE + ".tmp"
+    cmd = [HIPCC] + COMMON + ["-shared", "-o", tmp, os.path.join(CSRC, "probe.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, PROBE)
+    return PROBE
 
 
 if __name__ == "__main__":
