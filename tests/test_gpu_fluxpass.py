@@ -375,6 +375,34 @@ def ordered_sums(b, dQL):
     return acc
 
 
+def test_kh2d_1024_large_mesh_kernels(device):
+    """A 2D mesh of 2^20 cells (1024 x 1024 jittered KH, HLLC), where the
+    gathers and the record build take their large-mesh variants (bounded
+    occupancy, more rows in flight): the FP64 pass equals the oracle
+    (gradients bit for bit, fluxes within 1e-12), both gathers are the
+    face-ordered sums of their own fluxes bit for bit, and the mixed pass is
+    within 1e-6 of the FP64 pass and conservative."""
+    m = M.voronoi_periodic(M.hilbert_sorted(M.jittered_grid(1024, 0.25, seed=20240607)))
+    b = M.prepare(m, M.kh_ics(m), G, kind=1)
+    del m
+    assert b["C"] >= 1 << 20
+    print("cells", b["C"], "faces", b["F"], flush=True)
+    g64 = run_gpu(b, 1, "fp64", device)
+    gmx = run_gpu(b, 1, "mixed", device)
+    assert np.array_equal(ordered_sums(b, g64["dQL"]), g64["dQ"])
+    assert np.array_equal(ordered_sums(b, gmx["dQL"]), gmx["dQ"])
+    og, odQ, odQL, ost = oracle_pass(b, 1)
+    assert np.array_equal(g64["grad"], og)
+    den = np.maximum(np.abs(odQL).max(0), face_scale(b, 1))
+    err = (np.abs(g64["dQL"] - odQL).max(0) / np.where(den > 0, den, 1.0)).max()
+    errs = mixed_errors(b, 1, g64, gmx)
+    print("fp64 vs oracle %.2e; mixed vs fp64 %s" % (err, errs), flush=True)
+    assert err <= 1e-12
+    assert np.array_equal(g64["fstatus"] != 0, ost != 0)
+    assert errs["face_max"] <= 1e-6 and errs["cell_gross_max"] <= 1e-6
+    assert errs["conservation"] <= 1e-13
+
+
 def test_sedov3d_full_size_mixed_errors(device):
     """The 3D benchmark workload at full size (2,097,152 Poisson generators,
     Sedov blast, exact solver), mixed pass against the FP64 pass: every cell
