@@ -345,7 +345,8 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
     a Poisson process and are served open loop.  Batched mixed serving (the
     captured graph with persistent kernels, admitted by scratch footprint)
     against the plain per-mesh FP64 launch; each at load x its own capacity
-    (capacity: the same trace with every request present at t = 0)."""
+    (capacity: the same trace with every request present at t = 0), and the
+    batched server also at the plain one's offered rate."""
     import torch
     from shadowfax_amd import serving as S
     from shadowfax_amd.riemann import RiemannSolver
@@ -358,8 +359,9 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
     t0 = time.perf_counter()
     classes = S.ensemble_classes(w)
     out["tessellation_s"] = time.perf_counter() - t0
-    for prec, mode, name in (("mixed", S.SERVE_BATCHED, "batched_mixed"),
-                             ("fp64", S.SERVE_PLAIN, "plain_fp64")):
+    plain_rate = None
+    for prec, mode, name in (("fp64", S.SERVE_PLAIN, "plain_fp64"),
+                             ("mixed", S.SERVE_BATCHED, "batched_mixed")):
         t0 = time.perf_counter()
         solver = RiemannSolver(1, 2, 5.0 / 3.0, 1e-8, -5.0, prec)
         solver.enable_stats(False)
@@ -391,6 +393,17 @@ def bench_serving(dev, n_meshes=2000, nreq=4000, load=0.7):
                           "max_ms": st["max_s"] * 1e3, "mean_batch": st["mean_batch"],
                           "steps": st["nbatch"],
                           "face_fluxes_per_s": float(faces[mesh].sum()) / st["wall_s"]}
+        if mode == S.SERVE_PLAIN:
+            plain_rate = rate
+        elif plain_rate:
+            # the same requests at the plain path's offered rate: the two
+            # servers' latencies at equal load
+            lat, bat, st = ens.serve(arr / plain_rate, mesh, mode)
+            r["open_loop_at_plain_rate"] = {
+                "rate_per_s": plain_rate, "p50_ms": st["p50_s"] * 1e3,
+                "p99_ms": st["p99_s"] * 1e3, "mean_ms": st["mean_s"] * 1e3,
+                "max_ms": st["max_s"] * 1e3, "mean_batch": st["mean_batch"],
+                "steps": st["nbatch"]}
         out[name] = r
         log("  %s: capacity %.0f req/s, p50 %.3f ms p99 %.3f ms" %
             (name, cap_req, r["open_loop"]["p50_ms"], r["open_loop"]["p99_ms"]))
