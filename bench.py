@@ -180,13 +180,19 @@ def bench_fluxpass(b, kind, steps, warmup, dev):
         torch.cuda.synchronize()
         ms_build = ev[0].elapsed_time(ev[1])
         fp = FluxPass(s, dm)
-        r = time_variant(lambda fp=fp: fp.run(status=False), steps, warmup, stream)
-        r["ms_build_records"] = ms_build
+        # the three kernels timed one at a time first (each with its own
+        # warm-up), then the pass: the GPU has left its idle clocks by then
+        # (timed after its warm-up alone, a pass's first steps here ran up
+        # to 10% slower than the rest)
         parts = {"gradients": fp.gradients,
                  "faces": lambda fp=fp: fp.face_fluxes(status=False),
                  "gather": fp.gather}
+        ms_parts = {}
         for name, fn in parts.items():
-            r["ms_" + name] = time_variant(fn, steps, 1, stream)["ms_mean"]
+            ms_parts["ms_" + name] = time_variant(fn, steps, 1, stream)["ms_mean"]
+        r = time_variant(lambda fp=fp: fp.run(status=False), steps, warmup, stream)
+        r.update(ms_parts)
+        r["ms_build_records"] = ms_build
         r["face_fluxes_per_s"] = F / (r["ms_mean"] * 1e-3)
         r["fresh_mesh_ms"] = r["ms_mean"] + ms_build
         fp.run()
