@@ -698,12 +698,14 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
             }
         }
         base[l] = b;
-        off[l + 1] = n;  // counts, summed below
-    }
-    __syncthreads();
-    if(threadIdx.x == 0) {
-        off[0] = 0;
-        for(int l = 0; l < 64; l++) off[l + 1] += off[l];
+        // the lanes' counts summed across the first wave (inclusive scan)
+#pragma unroll
+        for(int d = 1; d < 64; d <<= 1) {
+            const int64_t t = __shfl_up(n, d, 64);
+            if(l >= d) n += t;
+        }
+        off[l + 1] = n;
+        if(l == 0) off[0] = 0;
     }
     __syncthreads();
     const int64_t r0 = m.egrp[g], K = m.egrp[g + 1] - r0;
