@@ -534,6 +534,10 @@ __device__ __forceinline__ FaceVals<ND> load_face_record(const float4* __restric
 
 // The FP32 record of cell entry e = 2 face + side (entry_rec(ND, true)).
 template <int ND>
+__device__ __forceinline__ void entry_record_from_face(const FaceVals<ND>& fv, int32_t e,
+                                                       float* rec);
+
+template <int ND>
 __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32_t e,
                                                  float* rec) {
     const int64_t F = m.F;
@@ -556,6 +560,15 @@ __device__ __forceinline__ void entry_record_f32(const MeshView<float>& m, int32
             fv.dR[l] = m.dR[l * F + f];
         }
     }
+    entry_record_from_face<ND>(fv, e, rec);
+}
+
+// The FP32 record of cell entry e from its face's values fv.
+template <int ND>
+__device__ __forceinline__ void entry_record_from_face(const FaceVals<ND>& fv, int32_t e,
+                                                       float* rec) {
+    const int32_t f = ent_face(e);
+    const bool isR = ent_right(e);
     // g = A (c - r_ij/2)/|r_ij| in FP64
     double r[ND], r2 = 0.0;
 #pragma unroll
@@ -711,8 +724,9 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
     const int64_t r0 = m.egrp[g], K = m.egrp[g + 1] - r0;
     const int64_t j1 = off[nc];
     float4* out = reinterpret_cast<float4*>(egeo);
-    for(int64_t j = threadIdx.x; j < j1; j += blockDim.x) {
-        int lo = 0, hi = nc;  // off[lo] <= j < off[hi]
+    // the group's entry j: lane lo with off[lo] <= j < off[lo + 1], its k-th
+    auto find = [&](int64_t j, int& lo_out) {
+        int lo = 0, hi = nc;
         while(hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if(off[mid] <= j)
@@ -720,13 +734,58 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
             else
                 hi = mid;
         }
-        const int64_t k = j - off[lo];
+        lo_out = lo;
+        return j - off[lo];
+    };
+    if constexpr(sizeof(T) == 4) {
+        if(m.frec) {
+            // FP32 records from the face records: a lane's U entries at a
+            // time, their indices, then their faces' records, loaded before
+            // any record is formed (entries past the group's last read the
+            // last one's, unused), so a lane waits for two loads per U
+            // entries instead of two per entry
+            constexpr int U = 4;
+            for(int64_t jb = threadIdx.x; jb < j1; jb += U * blockDim.x) {
+                int lo[U];
+                int64_t k[U];
+                int32_t e[U];
+#pragma unroll
+                for(int u = 0; u < U; u++) {
+                    const int64_t j = jb + u * (int64_t)blockDim.x;
+                    k[u] = find(j < j1 ? j : j1 - 1, lo[u]);
+                }
+#pragma unroll
+                for(int u = 0; u < U; u++) e[u] = m.cf_entry[base[lo[u]] + k[u]];
+                FaceVals<ND> fv[U];
+#pragma unroll
+                for(int u = 0; u < U; u++) fv[u] = load_face_record<ND>(m.frec, m.F, ent_face(e[u]));
+                // the loads stay here, ahead of the tests below (sunk into
+                // them, each entry's would wait for the previous entry's)
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for(int u = 0; u < U; u++) {
+                    if(jb + u * (int64_t)blockDim.x >= j1) continue;
+                    alignas(16) float rec[ER];
+                    entry_record_from_face<ND>(fv[u], e[u], rec);
+#pragma unroll
+                    for(int q = 0; q < P; q++)
+                        out[((r0 + k[u]) * P + q) * 64 + lo[u]] =
+                                reinterpret_cast<const float4*>(rec)[q];
+                }
+            }
+            goto pad;
+        }
+    }
+    for(int64_t j = threadIdx.x; j < j1; j += blockDim.x) {
+        int lo;
+        const int64_t k = find(j, lo);
         alignas(16) T rec[ER];
         entry_record<ND, T>(m, m.cf_entry[base[lo] + k], rec);
 #pragma unroll
         for(int q = 0; q < P; q++)
             out[((r0 + k) * P + q) * 64 + lo] = reinterpret_cast<const float4*>(rec)[q];
     }
+pad:
     for(int64_t sl = threadIdx.x; sl < K * 64; sl += blockDim.x) {
         const int64_t k = sl >> 6;
         const int lane = (int)(sl & 63);
