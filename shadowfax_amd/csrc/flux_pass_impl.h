@@ -77,6 +77,9 @@
 #ifndef SFX_GATHER_BIG  // ... of at least this many cells
 #define SFX_GATHER_BIG (1 << 20)
 #endif
+#ifndef SFX_EGEO_U  // entries a lane of the FP32 record build forms at a time
+#define SFX_EGEO_U 4  // (3D Sedov build: 1.49 ms; 2: 1.51, 8: 1.55, one at a time: 1.69)
+#endif
 #ifndef SFX_EGEO_WAVES  // record builds over large meshes: waves per SIMD (0: all that fit)
 #define SFX_EGEO_WAVES 2
 #endif
@@ -744,7 +747,7 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
             // any record is formed (entries past the group's last read the
             // last one's, unused), so a lane waits for two loads per U
             // entries instead of two per entry
-            constexpr int U = 4;
+            constexpr int U = SFX_EGEO_U;
             for(int64_t jb = threadIdx.x; jb < j1; jb += U * blockDim.x) {
                 int lo[U];
                 int64_t k[U];
