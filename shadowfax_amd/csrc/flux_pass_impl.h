@@ -80,6 +80,9 @@
 #ifndef SFX_EGEO_U  // entries a lane of the FP32 record build forms at a time
 #define SFX_EGEO_U 4  // (3D Sedov build: 1.49 ms; 2: 1.51, 8: 1.55, one at a time: 1.69)
 #endif
+#ifndef SFX_EGEO_U64  // the same for FP64 records
+#define SFX_EGEO_U64 4  // (3D Sedov build: 2.345 ms; 2: 2.465, one at a time: 2.455)
+#endif
 #ifndef SFX_EGEO_WAVES  // record builds over large meshes: waves per SIMD (0: all that fit)
 #define SFX_EGEO_WAVES 2
 #endif
@@ -595,24 +598,49 @@ __device__ __forceinline__ void entry_record_from_face(const FaceVals<ND>& fv, i
     rec[RL::A2] = fv.area;
 }
 
-// The FP64 record of cell entry e (entry_rec(ND, false)).
+// The FP64 record of cell entry e (entry_rec(ND, false)), in two steps: the
+// face's values it needs (its side's d vector and neighbour), then the record.
 template <int ND>
-__device__ __forceinline__ void entry_record_f64(const MeshView<double>& m, int32_t e,
-                                                 double* rec) {
-    constexpr int ER = entry_rec(ND, false);
+struct EntryVals64 {
+    double area, mc[ND], rRL[ND], d[ND];
+    int32_t nb;
+};
+template <int ND>
+__device__ __forceinline__ EntryVals64<ND> load_entry_vals64(const MeshView<double>& m,
+                                                             int32_t e) {
     const int64_t F = m.F;
     const int32_t f = ent_face(e);
     const bool isR = ent_right(e);
-    rec[0] = m.area[f];
+    EntryVals64<ND> v;
+    v.area = m.area[f];
 #pragma unroll
     for(int l = 0; l < ND; l++) {
-        const double rr = m.rRL[l * F + f];
-        rec[1 + l] = m.mc[l * F + f];
-        rec[1 + ND + l] = isR ? rr : -rr;
-        rec[1 + 2 * ND + l] = isR ? m.dR[l * F + f] : m.dL[l * F + f];
+        v.rRL[l] = m.rRL[l * F + f];
+        v.mc[l] = m.mc[l * F + f];
+        v.d[l] = (isR ? m.dR : m.dL)[l * F + f];
     }
-    rec[1 + 3 * ND] = index_bits(isR ? m.left[f] : m.right[f], 0.0);
+    v.nb = (isR ? m.left : m.right)[f];
+    return v;
+}
+template <int ND>
+__device__ __forceinline__ void entry_record_from_vals64(const EntryVals64<ND>& v, int32_t e,
+                                                         double* rec) {
+    constexpr int ER = entry_rec(ND, false);
+    const bool isR = ent_right(e);
+    rec[0] = v.area;
+#pragma unroll
+    for(int l = 0; l < ND; l++) {
+        rec[1 + l] = v.mc[l];
+        rec[1 + ND + l] = isR ? v.rRL[l] : -v.rRL[l];
+        rec[1 + 2 * ND + l] = v.d[l];
+    }
+    rec[1 + 3 * ND] = index_bits(v.nb, 0.0);
     if(ER > 2 + 3 * ND) rec[ER - 1] = 0.0;
+}
+template <int ND>
+__device__ __forceinline__ void entry_record_f64(const MeshView<double>& m, int32_t e,
+                                                 double* rec) {
+    entry_record_from_vals64<ND>(load_entry_vals64<ND>(m, e), e, rec);
 }
 
 template <int ND, typename T>
@@ -778,6 +806,36 @@ __global__ __launch_bounds__(256) void entry_geometry_grouped_kernel(MeshView<T>
             }
             goto pad;
         }
+    } else {
+        // FP64 records: the same, a lane's U64 entries at a time
+        constexpr int U = SFX_EGEO_U64;
+        for(int64_t jb = threadIdx.x; jb < j1; jb += U * blockDim.x) {
+            int lo[U];
+            int64_t k[U];
+            int32_t e[U];
+#pragma unroll
+            for(int u = 0; u < U; u++) {
+                const int64_t j = jb + u * (int64_t)blockDim.x;
+                k[u] = find(j < j1 ? j : j1 - 1, lo[u]);
+            }
+#pragma unroll
+            for(int u = 0; u < U; u++) e[u] = m.cf_entry[base[lo[u]] + k[u]];
+            EntryVals64<ND> v[U];
+#pragma unroll
+            for(int u = 0; u < U; u++) v[u] = load_entry_vals64<ND>(m, e[u]);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for(int u = 0; u < U; u++) {
+                if(jb + u * (int64_t)blockDim.x >= j1) continue;
+                alignas(16) double rec[ER];
+                entry_record_from_vals64<ND>(v[u], e[u], rec);
+#pragma unroll
+                for(int q = 0; q < P; q++)
+                    out[((r0 + k[u]) * P + q) * 64 + lo[u]] =
+                            reinterpret_cast<const float4*>(rec)[q];
+            }
+        }
+        goto pad;
     }
     for(int64_t j = threadIdx.x; j < j1; j += blockDim.x) {
         int lo;
