@@ -628,9 +628,15 @@ def main():
     mp = None
     if not args.no_peaks:
         log("measured ceilings (copy, FMA loops)")
-        mp = measured_peaks(dev)
-        out["measured_peaks"] = mp
-        measured_fracs(roof, mp)
+        try:
+            mp = measured_peaks(dev)
+        except (OSError, RuntimeError) as e:
+            # a missing or failing probe library leaves the ceilings out,
+            # not the line
+            out["measured_peaks"] = {"error": str(e)}
+        else:
+            out["measured_peaks"] = mp
+            measured_fracs(roof, mp)
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb, dev)
     if not args.no_cpu:
