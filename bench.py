@@ -253,7 +253,8 @@ def roofline(b, fp, workload_key, face_kernel, pass_kernels):
         out["hbm"] = {"achieved_GBs": gbs, "frac": f_hbm}
         out["valu"] = {"achieved_TFs": tf, "frac": f_valu,
                        "peak_note": "FP64 vector peak; non-packed FP32 issues at the same rate"}
-        # the whole pass, SURVEY 8(d)'s definition: max(T_hbm, T_valu) / t
+        # the whole pass: max(T_hbm, T_valu) / t, T_hbm its algorithmic bytes at the
+        # HBM peak, T_valu its counted operations at the vector peak
         pass_ops = cnt["pass_ops"]
         for prec in ("fp64", "mixed"):
             tp = fp[prec]["ms_mean"] * 1e-3
