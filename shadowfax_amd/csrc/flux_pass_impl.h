@@ -90,7 +90,8 @@
 #define SFX_GATHER_KB_F64 4
 #endif
 #ifndef SFX_GATHER_KB_BIG  // FP32 rows in flight per lane at the large meshes' bounded occupancy
-#define SFX_GATHER_KB_BIG 16  // (3D Sedov gather 251 -> 218 us; 8: 221, 32: 455, spilling)
+#define SFX_GATHER_KB_BIG 16  // (3D Sedov gather 251 -> 218 us; 8: 221, 32: 455, spilling;
+                              // at 1 wave per SIMD, 32: 318 us, 24: 380)
 #endif
 #ifndef SFX_GATHER_KB_BIG_F64  // the same for FP64 rows
 #define SFX_GATHER_KB_BIG_F64 8  // (409 -> 374 us; 16: 389)
