@@ -1328,9 +1328,12 @@ template <int ND>
 constexpr int grad_lds_k() {
     return ND == 3 ? SFX_GRAD_LDS_K : SFX_GRAD_LDS_K_2D;
 }
+#ifndef SFX_GRAD_WAVES_3D  // (3D Sedov gradients: 382 us; 2 waves with 4 records a chunk: 394)
+#define SFX_GRAD_WAVES_3D 3
+#endif
 template <int ND>
 constexpr int grad_lds_waves() {
-    return ND == 3 ? 3 : 4;
+    return ND == 3 ? SFX_GRAD_WAVES_3D : 4;
 }
 
 // FP32 records, grouped and read through the buffer descriptor (the host
